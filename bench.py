@@ -1,0 +1,186 @@
+"""Headline benchmark: RkNN queries/s at 10M users, 100k facilities, k=10,
+batched (one graph-captured launch sequence per step) versus plain (one
+counting launch, sync and readback per query), bit-exact sets.
+
+A step = one pass of the batched form over one batch of NQ queries whose
+candidate lists (from the host influence-zone filter) are already resident
+in HBM; the users are Morton-sorted and resident.  `value` counts queries/s
+over exactly --steps timed steps.  The JSON line also carries:
+  roofline      dense counting kernel (count-all: every user x every candidate,
+                4 fp32 flops per test) timed with HIP events on its stream
+  cpu_baseline  the FP64 brute-force oracle (rknn-naive semantics) on a
+                bounded sample of the same workload, one host thread
+  plain_*       the one-launch-per-query form on the same queries
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "rt-rknn_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--users", type=int, default=10_000_000)
+    ap.add_argument("--facilities", type=int, default=100_000)
+    ap.add_argument("--queries", type=int, default=1024)
+    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--config", type=int, default=4)
+    ap.add_argument("--plain", type=int, default=1024, help="queries timed in the plain form")
+    ap.add_argument("--roofline-queries", type=int, default=8)
+    ap.add_argument("--cpu-queries", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (queries shard with no "
+              "communication; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import rknn_amd as R
+
+    t0 = time.time()
+    seed = R.default_seed(args.config)
+    ux, uy, uid, fx, fy, fid = R.generate(args.config, args.users, args.facilities, seed=seed,
+                                          nq_ties=args.queries, seed_ties=seed)
+    qs = [int(q) for q in R.generate_queries(args.facilities, args.queries, seed)]
+    log(f"generated config {args.config}: {args.users} users, {args.facilities} facilities "
+        f"in {time.time() - t0:.1f}s")
+
+    ctx = R.Context(0)
+    t0 = time.time()
+    ds = R.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    log(f"dataset resident in {time.time() - t0:.2f}s ({ds.device_bytes() / 2**20:.0f} MiB)")
+
+    # host influence-zone filter (outside the timed region; reported)
+    t0 = time.time()
+    cands = [ds.filter(q, args.k) for q in qs]
+    filter_s = time.time() - t0
+    ncs = np.array([len(c) for c in cands])
+    log(f"filter: {args.queries} queries in {filter_s:.2f}s, |C| mean {ncs.mean():.1f} "
+        f"min {ncs.min()} max {ncs.max()}")
+
+    # ---- batched form: prepared once, replayed per step
+    batch = ds.prepare_batch([(q, args.k, c) for q, c in zip(qs, cands)])
+    for _ in range(args.warmup):
+        batch.run()
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.append(batch.run())
+    wall = time.perf_counter() - t0
+    batched = batch.fetch()
+    ms_step = wall / args.steps * 1e3
+    value = args.queries * args.steps / wall
+    log(f"batched: {ms_step:.3f} ms/step (device {np.mean(dev) * 1e3:.3f} ms), {value:,.0f} queries/s")
+
+    # ---- plain form: one counting launch + sync + readback per query
+    n_plain = min(args.plain, args.queries)
+    lat = []
+    for i in range(min(8, n_plain)):  # warm
+        ds.query_plain(qs[i], args.k, cands[i])
+    mismatches = 0
+    t0 = time.perf_counter()
+    for i in range(n_plain):
+        s = time.perf_counter()
+        got = ds.query_plain(qs[i], args.k, cands[i])
+        lat.append(time.perf_counter() - s)
+        mismatches += int(not np.array_equal(got, batched[i]))
+    plain_wall = time.perf_counter() - t0
+    plain_qps = n_plain / plain_wall
+    lat_ms = np.array(lat) * 1e3
+    log(f"plain: {plain_qps:,.0f} queries/s, p50 {np.percentile(lat_ms, 50):.3f} ms, "
+        f"p99 {np.percentile(lat_ms, 99):.3f} ms; batched == plain on {n_plain - mismatches}/{n_plain}")
+    if mismatches:
+        log("ERROR: batched and plain result sets differ")
+        sys.exit(1)
+
+    # ---- roofline: dense counting kernel (count-all), HIP events on its stream
+    kt, flops, tests = [], 0.0, 0
+    for i in range(min(args.roofline_queries, args.queries)):
+        _, t = ds.count_all(qs[i], cands[i], timing=True)
+        kt.append(t.kernel_s)
+        flops += 4.0 * args.users * t.n_candidates
+        tests += args.users * t.n_candidates
+    achieved = flops / sum(kt) / 1e12
+    log(f"count-all kernel: {achieved:.2f} TFLOP/s over {len(kt)} launches, avg {np.mean(kt) * 1e3:.3f} ms")
+
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                "kernel": "k_count_all", "avg_launch_ms": round(float(np.mean(kt)) * 1e3, 4),
+                "flops_per_launch": flops / len(kt)}
+
+    # ---- CPU baseline: FP64 brute-force oracle, bounded sample, 1 thread
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        oracle.set_threads(1)
+        n_cpu = min(args.cpu_queries, args.queries)
+        t0 = time.perf_counter()
+        ok = True
+        for i in range(n_cpu):
+            exp = oracle.rknn(ux, uy, uid, fx, fy, qs[i], args.k)
+            ok &= np.array_equal(exp, batched[i])
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": round(n_cpu / cpu_s, 4), "unit": "queries/s", "cores": 1, "kind": "port",
+               "sample": f"{n_cpu} of the {args.queries} queries on the full {args.users}-user, "
+                         f"{args.facilities}-facility dataset, k={args.k}: FP64 brute force "
+                         f"(rknn-naive semantics), one host thread",
+               "matches_gpu": bool(ok)}
+        log(f"cpu oracle: {n_cpu / cpu_s:.3f} queries/s; GPU sets equal oracle: {ok}")
+        if not ok:
+            sys.exit(1)
+
+    out = {
+        "metric": "RkNN queries/s @10M users,k=10 batched vs plain; p99 latency; bit-exact sets",
+        "value": round(value, 1),
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded SplitMix64 generator, config %d)" % args.config,
+        "config": {"workload": f"batched RkNN, {args.users} users, {args.facilities} facilities, "
+                               f"{args.queries} queries per step, k={args.k}",
+                   "users": args.users, "facilities": args.facilities, "queries_per_step": args.queries,
+                   "k": args.k, "candidates_mean": round(float(ncs.mean()), 2), "generator_config": args.config},
+        "plain_qps": round(plain_qps, 1),
+        "batched_vs_plain": round(value / plain_qps, 2),
+        "plain_latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 4),
+                             "p99": round(float(np.percentile(lat_ms, 99)), 4)},
+        "batched_device_ms": round(float(np.mean(dev)) * 1e3, 4),
+        "filter_qps": round(args.queries / filter_s, 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+    ds.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+/*
+ * rknn.h -- C ABI of the MI355X (gfx950) RkNN dominance-count engine.
+ *
+ * Plain C, POD arguments, caller-owned host buffers, status codes.  No
+ * function exits the process or throws across this boundary, and there is no
+ * CPU fallback: a missing device is RK_ERR_NO_DEVICE.
+ *
+ * Semantics (every query form returns the same set):
+ *
+ *   R(q,k) = { u : #{ f in C : D(u,f) < D(u,q) } < k }     (ties won by q)
+ *
+ * with D the squared Euclidean distance evaluated in binary64 from the fp32
+ * coordinates exactly as dx*dx + dy*dy, dx = (double)u.x - (double)p.x.  C is
+ * the candidate facility set: all facilities except q (brute-force form),
+ * the host influence-zone filter's output, or a caller-supplied index list.
+ * Result ids are returned sorted ascending.  Zero candidates return every
+ * user (the reference's rknn-inf-gpu/inf_kernel.cu:76-78 returns none).
+ *
+ * q is a facility INDEX (position in the facility arrays), as in
+ * rknn-inf/inf.cpp:218 (fac[q]); k >= 1.
+ */
+#ifndef RKNN_H
+#define RKNN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rk_status {
+    RK_OK = 0,
+    RK_ERR_NO_DEVICE = 1,     /* no HIP device visible / device index out of range      */
+    RK_ERR_INVALID_ARG = 2,   /* bad pointer, q >= N_f, k == 0, bad candidate index ...  */
+    RK_ERR_OUT_OF_MEMORY = 3, /* device allocation failed                                */
+    RK_ERR_CAPACITY = 4,      /* caller buffer too small; the size needed is reported    */
+    RK_ERR_HIP = 5,           /* a HIP runtime call failed (see rk_last_error)          */
+    RK_ERR_IO = 6,            /* file missing or malformed                               */
+    RK_ERR_ADMISSION = 7,     /* serving: request exceeds the memory budget              */
+    RK_ERR_NOT_READY = 8,     /* serving: ticket still in flight                         */
+    RK_ERR_INTERNAL = 9
+} rk_status;
+
+typedef struct rk_ctx rk_ctx;         /* one device, one HIP stream                  */
+typedef struct rk_dataset rk_dataset; /* device-resident users + facilities + index */
+typedef struct rk_batch rk_batch;     /* a prepared, graph-captured batch of queries */
+typedef struct rk_server rk_server;   /* a continuous-batching request loop          */
+
+/* Per-call phase times in seconds (wall clock on the host, the reference's
+ * get_wall_time phases of rknn-inf-gpu/inf.cpp:610-653) plus the device time
+ * of the counting kernels from HIP events. */
+typedef struct rk_timing {
+    double filter_s;     /* host influence-zone filter                       */
+    double upload_s;     /* candidate records host -> device                 */
+    double kernel_s;     /* counting kernels (HIP events)                    */
+    double download_s;   /* result ids device -> host, including the sort   */
+    double total_s;      /* whole call                                       */
+    uint64_t n_candidates;
+    uint64_t n_fp64_rechecks; /* users whose FP32 margin fell in the band   */
+} rk_timing;
+
+/* ---- lifecycle: replaces cuda_init / cuda_cleanup / is_cuda_available
+ *      (rknn-inf-gpu/inf_kernel.h:29-36, inf_kernel.cu:129-160) ---- */
+rk_status rk_init(int device, rk_ctx** out);
+rk_status rk_shutdown(rk_ctx* ctx);
+int rk_device_count(void);                 /* 0 when no device (never fails)   */
+const char* rk_status_string(rk_status s);
+const char* rk_last_error(const rk_ctx* ctx); /* detail of the last failure    */
+
+/* ---- dataset: replaces the per-call H2D copies of users and lines in
+ *      gpu_get_rknn_candidates (inf_kernel.cu:88-95) and gpuReverseKNN
+ *      (rknn-naive-gpu/rknn_kernel.cu:172-195); uploaded and Morton-sorted
+ *      once, then device-resident.  The host keeps the facility index used by
+ *      the filter (the facility R*-tree of rknn-inf/inf.cpp:174-177). ---- */
+rk_status rk_dataset_create(rk_ctx* ctx,
+                            const float* ux, const float* uy, const int32_t* uid, size_t nu,
+                            const float* fx, const float* fy, const int32_t* fid, size_t nf,
+                            rk_dataset** out);
+rk_status rk_dataset_destroy(rk_dataset* ds);
+/* users, facilities, device bytes held */
+rk_status rk_dataset_info(const rk_dataset* ds, uint64_t* nu, uint64_t* nf, uint64_t* device_bytes);
+
+/* ---- host influence-zone filter: replaces the bisector-generation loop of
+ *      rknn-inf/inf.cpp:201-585 (= rknn-rt/rt.cpp:289-664).  Writes candidate
+ *      facility indices, nearest first.  RK_ERR_CAPACITY if cap < needed
+ *      (*nc then holds the count needed). ---- */
+rk_status rk_filter(rk_dataset* ds, uint32_t q, uint32_t k,
+                    uint32_t* cand, size_t cap, size_t* nc);
+
+/* Same filter on host arrays, no device involved (builds the facility index
+ * per call).  Users only bound the zone (their bounding box). */
+rk_status rk_filter_points(const float* fx, const float* fy, size_t nf,
+                           const float* ux, const float* uy, size_t nu,
+                           uint32_t q, uint32_t k, uint32_t* cand, size_t cap, size_t* nc);
+
+/* ---- plain form, one counting launch per query: replaces
+ *      gpu_get_rknn_candidates (rknn-inf-gpu/inf_kernel.h:22-27) and the
+ *      OptiX launch of rknn-rt (rt.cpp:1116-1148, Params rt.h:40-49).
+ *      cand/nc is the candidate list (facility indices, e.g. rk_filter's
+ *      output).  out_ids receives the sorted result ids; RK_ERR_CAPACITY if
+ *      cap < |R| (*n_out then holds |R|). ---- */
+rk_status rk_query_plain(rk_dataset* ds, uint32_t q, uint32_t k,
+                         const uint32_t* cand, size_t nc,
+                         int32_t* out_ids, size_t cap, size_t* n_out, rk_timing* t);
+
+/* ---- brute-force form: C = all facilities but q; replaces gpuReverseKNN
+ *      (rknn-naive-gpu/rknn_kernel.cuh:40-46) without its N_u x N_f matrix. ---- */
+rk_status rk_query_bruteforce(rk_dataset* ds, uint32_t q, uint32_t k,
+                              int32_t* out_ids, size_t cap, size_t* n_out, rk_timing* t);
+
+/* ---- filter + plain: the whole rknn-inf-gpu pipeline (inf.cpp:201-653). ---- */
+rk_status rk_query(rk_dataset* ds, uint32_t q, uint32_t k,
+                   int32_t* out_ids, size_t cap, size_t* n_out, rk_timing* t);
+
+/* ---- count-all: per-user count of candidates strictly closer than q, no
+ *      early exit (the rknn-rt any-hit counter rt.cu:95-111 with k = inf).
+ *      counts[i] belongs to the i-th user of rk_dataset_create's arrays. ---- */
+rk_status rk_count_all(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc,
+                       uint32_t* counts, rk_timing* t);
+
+/* ---- batched form: many queries, uneven candidate counts and mixed k in
+ *      one graph-captured launch sequence.  No reference counterpart beyond
+ *      looping gpu_get_rknn_candidates per query. ---- */
+typedef struct rk_query_desc {
+    uint32_t q;            /* facility index                                */
+    uint32_t k;            /* >= 1                                          */
+    const uint32_t* cand;  /* candidate facility indices; NULL = run filter */
+    uint64_t nc;
+} rk_query_desc;
+
+/* One call: prepare + run + fetch.  offsets has nq+1 entries; the ids of
+ * query i are out_ids[offsets[i] .. offsets[i+1]), sorted ascending. */
+rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq,
+                         int32_t* out_ids, size_t cap, uint64_t* offsets, rk_timing* t);
+
+/* Split form for resident repeat runs: prepare uploads the candidate lists
+ * and captures the hipGraph; run replays it (results stay in HBM); fetch
+ * copies them out. */
+rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, rk_batch** out);
+rk_status rk_batch_run(rk_batch* b, double* device_seconds);
+rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets);
+rk_status rk_batch_destroy(rk_batch* b);
+
+/* ---- serving: continuous batching over resident datasets. ---- */
+typedef struct rk_serve_config {
+    uint32_t max_batch;          /* queries per device step (default 256)          */
+    uint64_t memory_budget;      /* device bytes for in-flight work (default 4 GiB) */
+    double max_wait_s;           /* longest a request waits for batch-mates         */
+} rk_serve_config;
+
+rk_status rk_serve_start(rk_ctx* ctx, const rk_serve_config* cfg, rk_server** out);
+/* Thread-safe.  Returns a ticket; RK_ERR_ADMISSION if the request can never
+ * fit the budget. */
+rk_status rk_serve_submit(rk_server* s, rk_dataset* ds, uint32_t q, uint32_t k, uint64_t* ticket);
+/* RK_ERR_NOT_READY while in flight; on RK_OK the ticket is retired.
+ * latency_s = completion - submission. */
+rk_status rk_serve_poll(rk_server* s, uint64_t ticket, int32_t* out_ids, size_t cap,
+                        size_t* n_out, double* latency_s);
+rk_status rk_serve_stop(rk_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RKNN_H */

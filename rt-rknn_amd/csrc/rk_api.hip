@@ -1,0 +1,467 @@
+// rk_api.hip -- C ABI of include/rknn.h: contexts, resident datasets and the
+// plain, brute-force, filtered and count-all query forms.  The batched form
+// is in rk_batch.hip and the serving loop in rk_serve.cpp.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rk_internal.hpp"
+#include "rknn.h"
+
+namespace rk {
+
+rk_status fail(rk_ctx* c, rk_status s, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return s;
+}
+
+rk_status fail_hip(rk_ctx* c, hipError_t e, const char* what, int line)
+{
+    std::string m = std::string(what) + " failed at rk line " + std::to_string(line) + ": " + hipGetErrorString(e);
+    return fail(c, e == hipErrorOutOfMemory ? RK_ERR_OUT_OF_MEMORY : RK_ERR_HIP, m);
+}
+
+rk_status clean_candidates(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc, std::vector<uint32_t>& out)
+{
+    out.clear();
+    if (nc && !cand) return fail(ds->ctx, RK_ERR_INVALID_ARG, "candidate list is NULL with nc > 0");
+    out.reserve(nc);
+    const float qx = ds->hfx[q], qy = ds->hfy[q];
+    for (size_t i = 0; i < nc; ++i) {
+        const uint32_t f = cand[i];
+        if (f >= ds->nf) return fail(ds->ctx, RK_ERR_INVALID_ARG, "candidate index out of range");
+        if (f == q || (ds->hfx[f] == qx && ds->hfy[f] == qy)) continue;
+        out.push_back(f);
+    }
+    return RK_OK;
+}
+
+} // namespace rk
+
+using namespace rk;
+
+namespace {
+
+bool finite_bounded(const float* v, size_t n)
+{
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(v[i]) || std::fabs(v[i]) > 1e18f) return false;
+    return true;
+}
+
+rk_status check_query(rk_dataset* ds, uint32_t q, uint32_t k)
+{
+    if (!ds) return RK_ERR_INVALID_ARG;
+    if (q >= ds->nf) return fail(ds->ctx, RK_ERR_INVALID_ARG, "query index q must be < number of facilities");
+    if (k == 0) return fail(ds->ctx, RK_ERR_INVALID_ARG, "k must be a positive integer");
+    return RK_OK;
+}
+
+// Deliver a sorted id list to the caller's buffer.
+rk_status deliver(rk_dataset* ds, const int32_t* ids, size_t n, int32_t* out, size_t cap, size_t* n_out)
+{
+    *n_out = n;
+    if (n > cap) return fail(ds->ctx, RK_ERR_CAPACITY, "output buffer too small; *n_out holds the size needed");
+    if (n) std::memcpy(out, ids, n * sizeof(int32_t));
+    return RK_OK;
+}
+
+// Host side of the plain and count-all forms: binary64 records of the
+// cleaned candidate list into pinned memory, then one async upload.
+rk_status upload_records(rk_dataset* ds, uint32_t q, const std::vector<uint32_t>& cl)
+{
+    rk_ctx* c = ds->ctx;
+    const size_t n = cl.size();
+    RK_HIP(c, ds->h_rec.reserve(n));
+    RK_HIP(c, ds->h_orig.reserve(n));
+    RK_HIP(c, ds->rec.reserve(n));
+    RK_HIP(c, ds->orig.reserve(n));
+    const float qx = ds->hfx[q], qy = ds->hfy[q];
+    double wx, wy;
+    axis_bounds(ds->ubox[0], ds->ubox[1], ds->ubox[2], ds->ubox[3], qx, qy, &wx, &wy);
+    for (size_t j = 0; j < n; ++j) {
+        const uint32_t f = cl[j];
+        ds->h_rec.p[j] = make_record(qx, qy, ds->hfx[f], ds->hfy[f], wx, wy);
+        ds->h_orig.p[j] = make_float2(ds->hfx[f], ds->hfy[f]);
+    }
+    if (n) {
+        RK_HIP(c, hipMemcpyAsync(ds->rec.p, ds->h_rec.p, n * sizeof(CandRec), hipMemcpyHostToDevice, c->stream));
+        RK_HIP(c, hipMemcpyAsync(ds->orig.p, ds->h_orig.p, n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+    }
+    return RK_OK;
+}
+
+QDev make_qdev(rk_dataset* ds, uint32_t q, uint32_t k, size_t nc)
+{
+    QDev d{};
+    d.qx = ds->hfx[q];
+    d.qy = ds->hfy[q];
+    d.k = k;
+    d.nc = (uint32_t)nc;
+    d.off = 0;
+    d.trivial = k > nc ? 1u : 0u;
+    return d;
+}
+
+rk_status plain_core(rk_dataset* ds, uint32_t q, uint32_t k, const std::vector<uint32_t>& cl, int32_t* out_ids,
+                     size_t cap, size_t* n_out, rk_timing* t, double t_start)
+{
+    rk_ctx* c = ds->ctx;
+    if (ds->nu == 0) {
+        *n_out = 0;
+        return RK_OK;
+    }
+    if (k > cl.size()) { // fewer candidates than k: every user qualifies
+        rk_status s = deliver(ds, ds->all_ids_sorted.data(), ds->all_ids_sorted.size(), out_ids, cap, n_out);
+        if (t) t->total_s = now_s() - t_start;
+        return s;
+    }
+    const double t0 = now_s();
+    rk_status s = upload_records(ds, q, cl);
+    if (s != RK_OK) return s;
+    RK_HIP(c, ds->ctr.reserve(2));
+    RK_HIP(c, ds->h_ctr.reserve(2));
+    RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, 2 * sizeof(uint32_t), c->stream));
+    const double t1 = now_s();
+    QDev qd = make_qdev(ds, q, k, cl.size());
+    RK_HIP(c, hipEventRecord(c->ev0, c->stream));
+    launch_plain(ds->view, qd, ds->rec.p, ds->orig.p, ds->out.p, ds->ctr.p, device_grid(c->n_cu, 16), c->stream);
+    RK_HIP(c, hipGetLastError());
+    RK_HIP(c, hipEventRecord(c->ev1, c->stream));
+    RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    RK_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    const uint32_t n = ds->h_ctr.p[0];
+    if (n > ds->n_pad) return fail(c, RK_ERR_INTERNAL, "result count exceeds the user count");
+    RK_HIP(c, ds->h_out.reserve(n));
+    if (n) {
+        RK_HIP(c, hipMemcpyAsync(ds->h_out.p, ds->out.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        RK_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    std::sort(ds->h_out.p, ds->h_out.p + n);
+    s = deliver(ds, ds->h_out.p, n, out_ids, cap, n_out);
+    if (t) {
+        t->upload_s = t1 - t0;
+        t->kernel_s = ms * 1e-3;
+        t->n_candidates = cl.size();
+        t->n_fp64_rechecks = ds->h_ctr.p[1];
+        t->total_s = now_s() - t_start;
+        t->download_s = t->total_s - (t1 - t_start) - t->kernel_s;
+    }
+    return s;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* rk_status_string(rk_status s)
+{
+    switch (s) {
+    case RK_OK: return "ok";
+    case RK_ERR_NO_DEVICE: return "no HIP device available";
+    case RK_ERR_INVALID_ARG: return "invalid argument";
+    case RK_ERR_OUT_OF_MEMORY: return "device out of memory";
+    case RK_ERR_CAPACITY: return "output buffer too small";
+    case RK_ERR_HIP: return "HIP runtime error";
+    case RK_ERR_IO: return "input/output error";
+    case RK_ERR_ADMISSION: return "request rejected by memory admission";
+    case RK_ERR_NOT_READY: return "request still in flight";
+    case RK_ERR_INTERNAL: return "internal error";
+    }
+    return "unknown status";
+}
+
+const char* rk_last_error(const rk_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int rk_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+rk_status rk_init(int device, rk_ctx** out)
+{
+    if (!out) return RK_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return RK_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return RK_ERR_NO_DEVICE;
+    rk_ctx* c = new rk_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+        delete c;
+        return RK_ERR_HIP;
+    }
+    if (c->n_cu <= 0) c->n_cu = 1;
+    *out = c;
+    return RK_OK;
+}
+
+rk_status rk_shutdown(rk_ctx* c)
+{
+    if (!c) return RK_ERR_INVALID_ARG;
+    {
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        (void)hipSetDevice(c->device);
+        if (c->stream) (void)hipStreamSynchronize(c->stream);
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->ev1) (void)hipEventDestroy(c->ev1);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+    }
+    delete c;
+    return RK_OK;
+}
+
+rk_status rk_dataset_create(rk_ctx* c, const float* ux, const float* uy, const int32_t* uid, size_t nu,
+                            const float* fx, const float* fy, const int32_t* fid, size_t nf, rk_dataset** out)
+{
+    if (!c || !out) return RK_ERR_INVALID_ARG;
+    *out = nullptr;
+    if ((nu && (!ux || !uy || !uid)) || (nf && (!fx || !fy || !fid)))
+        return fail(c, RK_ERR_INVALID_ARG, "NULL coordinate or id array");
+    if (nu > 0x7fffff00ull || nf > 0x7fffff00ull) return fail(c, RK_ERR_INVALID_ARG, "too many points");
+    if (!finite_bounded(ux, nu) || !finite_bounded(uy, nu) || !finite_bounded(fx, nf) || !finite_bounded(fy, nf))
+        return fail(c, RK_ERR_INVALID_ARG, "coordinates must be finite with magnitude <= 1e18");
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    RK_HIP(c, hipSetDevice(c->device));
+    std::unique_ptr<rk_dataset> ds(new rk_dataset());
+    ds->ctx = c;
+    ds->nu = nu;
+    ds->nf = nf;
+    ds->hfx.assign(fx, fx + nf);
+    ds->hfy.assign(fy, fy + nf);
+    ds->hfid.assign(fid, fid + nf);
+    if (nu) {
+        double b[4] = {ux[0], uy[0], ux[0], uy[0]};
+        for (size_t i = 1; i < nu; ++i) {
+            b[0] = std::min(b[0], (double)ux[i]);
+            b[1] = std::min(b[1], (double)uy[i]);
+            b[2] = std::max(b[2], (double)ux[i]);
+            b[3] = std::max(b[3], (double)uy[i]);
+        }
+        std::memcpy(ds->ubox, b, sizeof b);
+        ds->all_ids_sorted.assign(uid, uid + nu);
+        std::sort(ds->all_ids_sorted.begin(), ds->all_ids_sorted.end());
+    }
+    hipStream_t st = c->stream;
+    if (nf) {
+        RK_HIP(c, ds->fx.reserve(nf));
+        RK_HIP(c, ds->fy.reserve(nf));
+        RK_HIP(c, hipMemcpyAsync(ds->fx.p, fx, nf * sizeof(float), hipMemcpyHostToDevice, st));
+        RK_HIP(c, hipMemcpyAsync(ds->fy.p, fy, nf * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+    ds->index.reset(new FacilityIndex(fx, fy, nf, ds->ubox));
+
+    ds->n_tiles = (uint32_t)((nu + kTile - 1) / kTile);
+    ds->n_pad = ds->n_tiles * kTile;
+    if (nu) {
+        const uint32_t n = (uint32_t)nu;
+        DevBuf<float> tx, ty;
+        DevBuf<int32_t> tid;
+        DevBuf<uint32_t> keys, keys2, idx;
+        RK_HIP(c, tx.reserve(n));
+        RK_HIP(c, ty.reserve(n));
+        RK_HIP(c, tid.reserve(n));
+        RK_HIP(c, keys.reserve(n));
+        RK_HIP(c, keys2.reserve(n));
+        RK_HIP(c, idx.reserve(n));
+        RK_HIP(c, ds->perm.reserve(n));
+        RK_HIP(c, hipMemcpyAsync(tx.p, ux, n * sizeof(float), hipMemcpyHostToDevice, st));
+        RK_HIP(c, hipMemcpyAsync(ty.p, uy, n * sizeof(float), hipMemcpyHostToDevice, st));
+        RK_HIP(c, hipMemcpyAsync(tid.p, uid, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+        const double w = ds->ubox[2] - ds->ubox[0], h = ds->ubox[3] - ds->ubox[1];
+        const float sx = w > 0 ? (float)(65535.0 / w) : 0.f, sy = h > 0 ? (float)(65535.0 / h) : 0.f;
+        launch_morton_keys(tx.p, ty.p, n, (float)ds->ubox[0], (float)ds->ubox[1], sx, sy, keys.p, idx.p, st);
+        RK_HIP(c, hipGetLastError());
+        size_t tmp_bytes = 0;
+        RK_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.p, keys2.p, idx.p, ds->perm.p, (int)n,
+                                                     0, 32, st));
+        DevBuf<unsigned char> tmp;
+        RK_HIP(c, tmp.reserve(tmp_bytes));
+        RK_HIP(c, hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, keys.p, keys2.p, idx.p, ds->perm.p, (int)n, 0,
+                                                     32, st));
+        RK_HIP(c, ds->ux.reserve(ds->n_pad));
+        RK_HIP(c, ds->uy.reserve(ds->n_pad));
+        RK_HIP(c, ds->uid.reserve(ds->n_pad));
+        launch_gather_users(ds->perm.p, n, ds->n_pad, tx.p, ty.p, tid.p, ds->ux.p, ds->uy.p, ds->uid.p, st);
+        RK_HIP(c, hipGetLastError());
+        // bounding-rectangle tree: tiles, then groups of 64 until <= 64 nodes
+        uint32_t cnt = ds->n_tiles;
+        int lvl = 0;
+        RK_HIP(c, ds->mbr[0].reserve(cnt));
+        launch_tile_mbr(ds->ux.p, ds->uy.p, cnt, ds->mbr[0].p, st);
+        RK_HIP(c, hipGetLastError());
+        ds->view.nlev[0] = cnt;
+        while (cnt > (uint32_t)kFanout) {
+            if (lvl + 1 >= kMaxLevels) return fail(c, RK_ERR_INVALID_ARG, "dataset too large for the tree depth");
+            const uint32_t up = (cnt + kFanout - 1) / kFanout;
+            RK_HIP(c, ds->mbr[lvl + 1].reserve(up));
+            launch_level_mbr(ds->mbr[lvl].p, cnt, ds->mbr[lvl + 1].p, up, st);
+            RK_HIP(c, hipGetLastError());
+            ++lvl;
+            cnt = up;
+            ds->view.nlev[lvl] = cnt;
+        }
+        ds->view.n_levels = (uint32_t)lvl + 1;
+        RK_HIP(c, hipStreamSynchronize(st)); // temporaries are freed on return
+        RK_HIP(c, ds->out.reserve(ds->n_pad));
+    }
+    ds->view.ux = ds->ux.p;
+    ds->view.uy = ds->uy.p;
+    ds->view.uid = ds->uid.p;
+    for (int l = 0; l < kMaxLevels; ++l) ds->view.mbr[l] = ds->mbr[l].p;
+    ds->view.nu = (uint32_t)nu;
+    ds->view.n_tiles = ds->n_tiles;
+    RK_HIP(c, hipStreamSynchronize(st));
+    *out = ds.release();
+    return RK_OK;
+}
+
+rk_status rk_dataset_destroy(rk_dataset* ds)
+{
+    if (!ds) return RK_ERR_INVALID_ARG;
+    rk_ctx* c = ds->ctx;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    delete ds;
+    return RK_OK;
+}
+
+rk_status rk_dataset_info(const rk_dataset* ds, uint64_t* nu, uint64_t* nf, uint64_t* device_bytes)
+{
+    if (!ds) return RK_ERR_INVALID_ARG;
+    if (nu) *nu = ds->nu;
+    if (nf) *nf = ds->nf;
+    if (device_bytes) {
+        uint64_t b = ds->ux.bytes() + ds->uy.bytes() + ds->uid.bytes() + ds->perm.bytes() + ds->fx.bytes() +
+                     ds->fy.bytes() + ds->out.bytes();
+        for (int l = 0; l < kMaxLevels; ++l) b += ds->mbr[l].bytes();
+        *device_bytes = b;
+    }
+    return RK_OK;
+}
+
+rk_status rk_filter(rk_dataset* ds, uint32_t q, uint32_t k, uint32_t* cand, size_t cap, size_t* nc)
+{
+    rk_status s = check_query(ds, q, k);
+    if (s != RK_OK) return s;
+    if (!nc || (cap && !cand)) return fail(ds->ctx, RK_ERR_INVALID_ARG, "NULL output");
+    std::vector<uint32_t> v;
+    ds->index->filter(q, k, v);
+    *nc = v.size();
+    if (v.size() > cap) return fail(ds->ctx, RK_ERR_CAPACITY, "candidate buffer too small; *nc holds the size needed");
+    if (!v.empty()) std::memcpy(cand, v.data(), v.size() * sizeof(uint32_t));
+    return RK_OK;
+}
+
+rk_status rk_query_plain(rk_dataset* ds, uint32_t q, uint32_t k, const uint32_t* cand, size_t nc, int32_t* out_ids,
+                         size_t cap, size_t* n_out, rk_timing* t)
+{
+    const double t0 = now_s();
+    rk_status s = check_query(ds, q, k);
+    if (s != RK_OK) return s;
+    if (!n_out || (cap && !out_ids)) return fail(ds->ctx, RK_ERR_INVALID_ARG, "NULL output");
+    std::vector<uint32_t> cl;
+    s = clean_candidates(ds, q, cand, nc, cl);
+    if (s != RK_OK) return s;
+    if (t) *t = rk_timing{};
+    std::lock_guard<std::recursive_mutex> lk(ds->ctx->mu);
+    RK_HIP(ds->ctx, hipSetDevice(ds->ctx->device));
+    return plain_core(ds, q, k, cl, out_ids, cap, n_out, t, t0);
+}
+
+rk_status rk_query_bruteforce(rk_dataset* ds, uint32_t q, uint32_t k, int32_t* out_ids, size_t cap, size_t* n_out,
+                              rk_timing* t)
+{
+    const double t0 = now_s();
+    rk_status s = check_query(ds, q, k);
+    if (s != RK_OK) return s;
+    if (!n_out || (cap && !out_ids)) return fail(ds->ctx, RK_ERR_INVALID_ARG, "NULL output");
+    std::vector<uint32_t> all(ds->nf), cl;
+    for (uint64_t i = 0; i < ds->nf; ++i) all[i] = (uint32_t)i;
+    s = clean_candidates(ds, q, all.data(), all.size(), cl);
+    if (s != RK_OK) return s;
+    if (t) *t = rk_timing{};
+    std::lock_guard<std::recursive_mutex> lk(ds->ctx->mu);
+    RK_HIP(ds->ctx, hipSetDevice(ds->ctx->device));
+    return plain_core(ds, q, k, cl, out_ids, cap, n_out, t, t0);
+}
+
+rk_status rk_query(rk_dataset* ds, uint32_t q, uint32_t k, int32_t* out_ids, size_t cap, size_t* n_out, rk_timing* t)
+{
+    const double t0 = now_s();
+    rk_status s = check_query(ds, q, k);
+    if (s != RK_OK) return s;
+    if (!n_out || (cap && !out_ids)) return fail(ds->ctx, RK_ERR_INVALID_ARG, "NULL output");
+    std::vector<uint32_t> cand, cl;
+    ds->index->filter(q, k, cand);
+    const double t1 = now_s();
+    s = clean_candidates(ds, q, cand.data(), cand.size(), cl);
+    if (s != RK_OK) return s;
+    if (t) *t = rk_timing{};
+    std::lock_guard<std::recursive_mutex> lk(ds->ctx->mu);
+    RK_HIP(ds->ctx, hipSetDevice(ds->ctx->device));
+    s = plain_core(ds, q, k, cl, out_ids, cap, n_out, t, t0);
+    if (t) t->filter_s = t1 - t0;
+    return s;
+}
+
+rk_status rk_count_all(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc, uint32_t* counts, rk_timing* t)
+{
+    const double t0 = now_s();
+    rk_status s = check_query(ds, q, 1);
+    if (s != RK_OK) return s;
+    if (ds->nu && !counts) return fail(ds->ctx, RK_ERR_INVALID_ARG, "NULL output");
+    std::vector<uint32_t> cl;
+    s = clean_candidates(ds, q, cand, nc, cl);
+    if (s != RK_OK) return s;
+    if (t) *t = rk_timing{};
+    if (ds->nu == 0) return RK_OK;
+    rk_ctx* c = ds->ctx;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    RK_HIP(c, hipSetDevice(c->device));
+    const double t1 = now_s();
+    s = upload_records(ds, q, cl);
+    if (s != RK_OK) return s;
+    RK_HIP(c, ds->counts.reserve(ds->n_pad));
+    RK_HIP(c, ds->counts_orig.reserve(ds->nu));
+    RK_HIP(c, ds->ctr.reserve(2));
+    RK_HIP(c, ds->h_ctr.reserve(2));
+    RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, 2 * sizeof(uint32_t), c->stream));
+    QDev qd = make_qdev(ds, q, 1, cl.size());
+    RK_HIP(c, hipEventRecord(c->ev0, c->stream));
+    launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, device_grid(c->n_cu, 16),
+                     c->stream);
+    RK_HIP(c, hipGetLastError());
+    RK_HIP(c, hipEventRecord(c->ev1, c->stream));
+    launch_scatter_u32(ds->counts.p, ds->perm.p, (uint32_t)ds->nu, ds->counts_orig.p, c->stream);
+    RK_HIP(c, hipGetLastError());
+    RK_HIP(c, hipMemcpyAsync(counts, ds->counts_orig.p, ds->nu * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    RK_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    if (t) {
+        t->kernel_s = ms * 1e-3;
+        t->n_candidates = cl.size();
+        t->n_fp64_rechecks = ds->h_ctr.p[1];
+        t->upload_s = 0;
+        t->total_s = now_s() - t0;
+        t->filter_s = t1 - t0;
+    }
+    return RK_OK;
+}
+
+} // extern "C"

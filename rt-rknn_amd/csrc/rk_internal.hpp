@@ -1,0 +1,166 @@
+// rk_internal.hpp -- library-private object layouts behind the opaque
+// handles of include/rknn.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rk_common.hpp"
+#include "rk_filter.hpp"
+#include "rk_kernels.hpp"
+#include "rknn.h"
+
+namespace rk {
+
+inline double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Owning device allocation that only grows.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t reserve(size_t want)
+    {
+        if (want <= n && p) return hipSuccess;
+        release();
+        if (want == 0) want = 1;
+        hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        n = want;
+        return hipSuccess;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Owning pinned host allocation that only grows.
+template <class T>
+struct HostBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t reserve(size_t want)
+    {
+        if (want <= n && p) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        if (want == 0) want = 1;
+        hipError_t e = hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        n = want;
+        return hipSuccess;
+    }
+};
+
+} // namespace rk
+
+struct rk_ctx {
+    int device = 0;
+    int n_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::recursive_mutex mu; // serialises device work on this context
+    std::string err;
+};
+
+struct rk_dataset {
+    rk_ctx* ctx = nullptr;
+    uint64_t nu = 0, nf = 0;
+    uint32_t n_tiles = 0, n_pad = 0;
+    rk::DevBuf<float> ux, uy;        // Morton-sorted, padded
+    rk::DevBuf<int32_t> uid;
+    rk::DevBuf<uint32_t> perm;       // sorted position -> caller's user index
+    rk::DevBuf<float4> mbr[rk::kMaxLevels];
+    rk::DevBuf<float> fx, fy;        // facilities, caller order
+    std::vector<float> hfx, hfy;
+    std::vector<int32_t> hfid;
+    std::vector<int32_t> all_ids_sorted; // result of a query where every user qualifies
+    double ubox[4] = {0, 0, -1, -1};     // users' bounding box (minx, miny, maxx, maxy)
+    std::unique_ptr<rk::FacilityIndex> index;
+    rk::DsDev view{};
+    // scratch of the plain / count-all forms
+    rk::DevBuf<rk::CandRec> rec;
+    rk::DevBuf<float2> orig;
+    rk::DevBuf<int32_t> out;
+    rk::DevBuf<uint32_t> ctr;
+    rk::DevBuf<uint32_t> counts, counts_orig;
+    rk::HostBuf<rk::CandRec> h_rec;
+    rk::HostBuf<float2> h_orig;
+    rk::HostBuf<int32_t> h_out;
+    rk::HostBuf<uint32_t> h_ctr;
+};
+
+struct rk_batch {
+    rk_dataset* ds = nullptr;
+    uint32_t nq = 0;
+    std::vector<rk::QDev> hq;
+    uint64_t total_cand = 0;
+    rk::DevBuf<rk::QDev> q;
+    rk::DevBuf<uint32_t> cand;
+    rk::DevBuf<double2> wb;
+    rk::DevBuf<rk::CandRec> rec;
+    rk::DevBuf<float2> orig;
+    rk::DevBuf<uint64_t> items[rk::kMaxLevels + 1];
+    rk::DevBuf<int32_t> pool_uid;
+    rk::DevBuf<uint64_t> pool_tag;
+    rk::DevBuf<uint32_t> qcount;
+    rk::DevBuf<uint64_t> qoff;
+    rk::DevBuf<int32_t> out;
+    rk::DevBuf<rk::BatchCounters> ctr;
+    rk::HostBuf<rk::BatchCounters> h_ctr;
+    rk::BatchDev view{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool ran = false;
+    double filter_s = 0.0;
+    ~rk_batch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace rk {
+rk_status fail(rk_ctx* c, rk_status s, const std::string& msg);
+rk_status fail_hip(rk_ctx* c, hipError_t e, const char* what, int line);
+// candidate list cleaning shared by every form: validates indices, drops q
+// and facilities at q's exact position (never strictly closer)
+rk_status clean_candidates(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc, std::vector<uint32_t>& out);
+} // namespace rk
+
+#define RK_HIP(ctx, call)                                                                                      \
+    do {                                                                                                       \
+        hipError_t e_ = (call);                                                                                \
+        if (e_ != hipSuccess) return rk::fail_hip((ctx), e_, #call, __LINE__);                                \
+    } while (0)

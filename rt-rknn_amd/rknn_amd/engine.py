@@ -1,0 +1,304 @@
+"""Python mirror of the engine's C ABI (include/rknn.h) for tests and bench.
+
+The host program of the engine is the C++ CLI (bin/rknn-hip); this module
+only moves numpy buffers across the C ABI.  Every call runs the gfx950
+kernels; errors raise RkError carrying the rk_status code.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+
+
+class RkError(RuntimeError):
+    def __init__(self, status, message=""):
+        self.status = status
+        super().__init__(f"rk status {status}: {message}")
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def device_count():
+    return L.load().rk_device_count()
+
+
+class Context:
+    """One HIP device and stream (rk_init / rk_shutdown)."""
+
+    def __init__(self, device=0):
+        self._lib = L.load()
+        h = C.c_void_p()
+        st = self._lib.rk_init(device, C.byref(h))
+        if st != L.RK_OK:
+            raise RkError(st, self._lib.rk_status_string(st).decode())
+        self.h = h
+
+    def check(self, st):
+        if st != L.RK_OK:
+            raise RkError(st, self._lib.rk_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self._lib.rk_shutdown(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Dataset:
+    """Device-resident users and facilities (rk_dataset_create)."""
+
+    def __init__(self, ctx, ux, uy, uid, fx, fy, fid):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.ux, self.uy, self.uid = _f32(ux), _f32(uy), _i32(uid)
+        self.fx, self.fy, self.fid = _f32(fx), _f32(fy), _i32(fid)
+        h = C.c_void_p()
+        st = self._lib.rk_dataset_create(
+            ctx.h, _ptr(self.ux, C.c_float), _ptr(self.uy, C.c_float), _ptr(self.uid, C.c_int32), len(self.ux),
+            _ptr(self.fx, C.c_float), _ptr(self.fy, C.c_float), _ptr(self.fid, C.c_int32), len(self.fx),
+            C.byref(h))
+        ctx.check(st)
+        self.h = h
+        self.nu, self.nf = len(self.ux), len(self.fx)
+
+    def close(self):
+        if self.h:
+            self._lib.rk_dataset_destroy(self.h)
+            self.h = None
+
+    def device_bytes(self):
+        b = C.c_uint64()
+        self.ctx.check(self._lib.rk_dataset_info(self.h, None, None, C.byref(b)))
+        return b.value
+
+    def filter(self, q, k):
+        n = C.c_size_t()
+        st = self._lib.rk_filter(self.h, q, k, None, 0, C.byref(n))
+        if st not in (L.RK_OK, L.RK_ERR_CAPACITY):
+            self.ctx.check(st)
+        out = np.zeros(max(1, n.value), dtype=np.uint32)
+        self.ctx.check(self._lib.rk_filter(self.h, q, k, _ptr(out, C.c_uint32), len(out), C.byref(n)))
+        return out[: n.value].copy()
+
+    def _ids_call(self, fn, *args):
+        cap = max(1, self.nu)
+        out = np.empty(cap, dtype=np.int32)
+        n = C.c_size_t()
+        t = L.rk_timing()
+        st = fn(self.h, *args, _ptr(out, C.c_int32), cap, C.byref(n), C.byref(t))
+        self.ctx.check(st)
+        return out[: n.value].copy(), t
+
+    def query_plain(self, q, k, cand, timing=False):
+        cand = _u32(cand)
+        ids, t = self._ids_call(self._lib.rk_query_plain, q, k, _ptr(cand, C.c_uint32), len(cand))
+        return (ids, t) if timing else ids
+
+    def query_bruteforce(self, q, k, timing=False):
+        ids, t = self._ids_call(self._lib.rk_query_bruteforce, q, k)
+        return (ids, t) if timing else ids
+
+    def query(self, q, k, timing=False):
+        ids, t = self._ids_call(self._lib.rk_query, q, k)
+        return (ids, t) if timing else ids
+
+    def count_all(self, q, cand, timing=False):
+        cand = _u32(cand)
+        counts = np.zeros(max(1, self.nu), dtype=np.uint32)
+        t = L.rk_timing()
+        self.ctx.check(self._lib.rk_count_all(self.h, q, _ptr(cand, C.c_uint32), len(cand),
+                                              _ptr(counts, C.c_uint32), C.byref(t)))
+        counts = counts[: self.nu]
+        return (counts, t) if timing else counts
+
+    @staticmethod
+    def _descs(queries):
+        keep = []
+        descs = (L.rk_query_desc * max(1, len(queries)))()
+        for i, qd in enumerate(queries):
+            q, k = int(qd[0]), int(qd[1])
+            cand = qd[2] if len(qd) > 2 else None
+            descs[i].q, descs[i].k = q, k
+            if cand is None:
+                descs[i].cand, descs[i].nc = None, 0
+            else:
+                c = _u32(cand)
+                keep.append(c)
+                descs[i].cand, descs[i].nc = _ptr(c, C.c_uint32), len(c)
+        return descs, keep
+
+    def query_batch(self, queries, timing=False):
+        """queries: iterable of (q, k) or (q, k, cand); cand None runs the filter."""
+        queries = list(queries)
+        descs, keep = self._descs(queries)
+        nq = len(queries)
+        offs = np.zeros(nq + 1, dtype=np.uint64)
+        cap = max(1, self.nu)
+        t = L.rk_timing()
+        while True:
+            out = np.empty(cap, dtype=np.int32)
+            st = self._lib.rk_query_batch(self.h, descs, nq, _ptr(out, C.c_int32), cap, _ptr(offs, C.c_uint64),
+                                          C.byref(t))
+            if st == L.RK_ERR_CAPACITY and int(offs[nq]) > cap:
+                cap = int(offs[nq])
+                continue
+            self.ctx.check(st)
+            break
+        res = [out[int(offs[i]): int(offs[i + 1])].copy() for i in range(nq)]
+        return (res, t) if timing else res
+
+    def prepare_batch(self, queries):
+        return Batch(self, list(queries))
+
+
+class Batch:
+    """Prepared graph-captured batch (rk_batch_prepare / run / fetch)."""
+
+    def __init__(self, ds, queries):
+        self.ds = ds
+        self._lib = ds._lib
+        self.nq = len(queries)
+        self._descs, self._keep = Dataset._descs(queries)
+        h = C.c_void_p()
+        ds.ctx.check(self._lib.rk_batch_prepare(ds.h, self._descs, self.nq, C.byref(h)))
+        self.h = h
+
+    def run(self):
+        s = C.c_double()
+        self.ds.ctx.check(self._lib.rk_batch_run(self.h, C.byref(s)))
+        return s.value
+
+    def fetch(self):
+        offs = np.zeros(self.nq + 1, dtype=np.uint64)
+        cap = max(1, self.ds.nu)
+        while True:
+            out = np.empty(cap, dtype=np.int32)
+            st = self._lib.rk_batch_fetch(self.h, _ptr(out, C.c_int32), cap, _ptr(offs, C.c_uint64))
+            if st == L.RK_ERR_CAPACITY and int(offs[self.nq]) > cap:
+                cap = int(offs[self.nq])
+                continue
+            self.ds.ctx.check(st)
+            break
+        return [out[int(offs[i]): int(offs[i + 1])].copy() for i in range(self.nq)]
+
+    def close(self):
+        if self.h:
+            self._lib.rk_batch_destroy(self.h)
+            self.h = None
+
+
+class Server:
+    """Continuous-batching request loop (rk_serve_*)."""
+
+    def __init__(self, ctx, max_batch=256, memory_budget=4 << 30, max_wait_s=0.0005):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        cfg = L.rk_serve_config(max_batch, memory_budget, max_wait_s)
+        h = C.c_void_p()
+        ctx.check(self._lib.rk_serve_start(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+
+    def submit(self, ds, q, k):
+        t = C.c_uint64()
+        st = self._lib.rk_serve_submit(self.h, ds.h, q, k, C.byref(t))
+        self.ctx.check(st)
+        return t.value
+
+    def poll(self, ticket, cap):
+        """(ids, latency_s) when done, None while in flight."""
+        out = np.empty(max(1, cap), dtype=np.int32)
+        n = C.c_size_t()
+        lat = C.c_double()
+        st = self._lib.rk_serve_poll(self.h, ticket, _ptr(out, C.c_int32), len(out), C.byref(n), C.byref(lat))
+        if st == L.RK_ERR_NOT_READY:
+            return None
+        if st == L.RK_ERR_CAPACITY:
+            return self.poll(ticket, n.value)
+        self.ctx.check(st)
+        return out[: n.value].copy(), lat.value
+
+    def stop(self):
+        if self.h:
+            self._lib.rk_serve_stop(self.h)
+            self.h = None
+
+
+# ---------------------------------------------------------------- data sets
+
+def default_seed(config):
+    return L.load().rk_default_seed(config)
+
+
+def generate(config, nu, nf, seed=None, nq_ties=0, seed_ties=None):
+    """Seeded synthetic dataset (rk_generate): (ux, uy, uid, fx, fy, fid)."""
+    lib = L.load()
+    seed = default_seed(config) if seed is None else seed
+    seed_ties = seed if seed_ties is None else seed_ties
+    ux = np.empty(nu, np.float32)
+    uy = np.empty(nu, np.float32)
+    uid = np.empty(nu, np.int32)
+    fx = np.empty(nf, np.float32)
+    fy = np.empty(nf, np.float32)
+    fid = np.empty(nf, np.int32)
+    st = lib.rk_generate(config, nu, nf, seed, nq_ties, seed_ties, _ptr(ux, C.c_float), _ptr(uy, C.c_float),
+                         _ptr(uid, C.c_int32), _ptr(fx, C.c_float), _ptr(fy, C.c_float), _ptr(fid, C.c_int32))
+    if st != L.RK_OK:
+        raise RkError(st, "rk_generate")
+    return ux, uy, uid, fx, fy, fid
+
+
+def generate_queries(nf, nq, seed):
+    lib = L.load()
+    out = np.empty(max(1, nq), np.uint32)
+    st = lib.rk_generate_queries(nf, nq, seed, _ptr(out, C.c_uint32))
+    if st != L.RK_OK:
+        raise RkError(st, "rk_generate_queries")
+    return out[:nq]
+
+
+def load_text(path):
+    lib = L.load()
+    fx, fy, ux, uy = (C.POINTER(C.c_float)() for _ in range(4))
+    fid, uid = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)()
+    nf, nu = C.c_uint64(), C.c_uint64()
+    st = lib.rk_load_text(path.encode(), C.byref(fx), C.byref(fy), C.byref(fid), C.byref(nf), C.byref(ux),
+                          C.byref(uy), C.byref(uid), C.byref(nu))
+    if st != L.RK_OK:
+        raise RkError(st, f"rk_load_text({path})")
+    try:
+        arr = lambda p, n, dt: np.ctypeslib.as_array(p, shape=(n,)).astype(dt).copy() if n else np.empty(0, dt)
+        res = (arr(ux, nu.value, np.float32), arr(uy, nu.value, np.float32), arr(uid, nu.value, np.int32),
+               arr(fx, nf.value, np.float32), arr(fy, nf.value, np.float32), arr(fid, nf.value, np.int32))
+    finally:
+        for p in (fx, fy, fid, ux, uy, uid):
+            lib.rk_free(C.cast(p, C.c_void_p))
+    return res
+
+
+def write_text(path, ux, uy, uid, fx, fy, fid):
+    lib = L.load()
+    ux, uy, uid, fx, fy, fid = _f32(ux), _f32(uy), _i32(uid), _f32(fx), _f32(fy), _i32(fid)
+    st = lib.rk_write_text(path.encode(), _ptr(fx, C.c_float), _ptr(fy, C.c_float), _ptr(fid, C.c_int32), len(fx),
+                           _ptr(ux, C.c_float), _ptr(uy, C.c_float), _ptr(uid, C.c_int32), len(ux))
+    if st != L.RK_OK:
+        raise RkError(st, f"rk_write_text({path})")

@@ -1,0 +1,174 @@
+"""GPU parity: every query form of the gfx950 engine against the FP64 oracle
+and the committed golden fixtures.  Integer/index work: bit-exact sets."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def golden(name):
+    d = np.load(os.path.join(HERE, "golden", name))
+    res = [d["ids"][d["offs"][i]: d["offs"][i + 1]] for i in range(len(d["qk"]))]
+    return d, [(int(q), int(k)) for q, k in d["qk"]], res
+
+
+@pytest.fixture(scope="module")
+def g1(rk, ctx):
+    d, qk, res = golden("config1.npz")
+    ds = rk.Dataset(ctx, d["ux"], d["uy"], d["uid"], d["fx"], d["fy"], d["fid"])
+    yield d, ds, qk, res
+    ds.close()
+
+
+@pytest.fixture(scope="module")
+def gties(rk, ctx):
+    d, qk, res = golden("ties.npz")
+    ds = rk.Dataset(ctx, d["ux"], d["uy"], d["uid"], d["fx"], d["fy"], d["fid"])
+    yield d, ds, qk, res
+    ds.close()
+
+
+def test_config1_bruteforce_golden(g1):
+    d, ds, qk, res = g1
+    for (q, k), exp in zip(qk, res):
+        print("config1 bruteforce q", q, "k", k, flush=True)
+        assert np.array_equal(ds.query_bruteforce(q, k), exp)
+
+
+def test_config1_filtered_golden(g1):
+    d, ds, qk, res = g1
+    for (q, k), exp in zip(qk, res):
+        cand = ds.filter(q, k)
+        assert np.array_equal(ds.query_plain(q, k, cand), exp)
+        assert np.array_equal(ds.query(q, k), exp)
+
+
+def test_config1_batch_golden(g1):
+    d, ds, qk, res = g1
+    got = ds.query_batch([(q, k) for q, k in qk])
+    for g, exp in zip(got, res):
+        assert np.array_equal(g, exp)
+    # explicit candidate lists, brute-force lists, in one batch
+    mixed = [(q, k, np.arange(ds.nf, dtype=np.uint32)) for q, k in qk] + [(q, k) for q, k in qk]
+    got = ds.query_batch(mixed)
+    for i, g in enumerate(got):
+        assert np.array_equal(g, res[i % len(res)])
+
+
+def test_ties_golden_every_form(gties):
+    d, ds, qk, res = gties
+    rechecks = 0
+    for (q, k), exp in zip(qk, res):
+        print("ties q", q, "k", k, flush=True)
+        ids, t = ds.query(q, k, timing=True)
+        rechecks += t.n_fp64_rechecks
+        assert np.array_equal(ids, exp)
+        assert np.array_equal(ds.query_bruteforce(q, k), exp)
+    assert rechecks > 0  # users on bisectors reached the binary64 path
+    got = ds.query_batch(qk)
+    for g, exp in zip(got, res):
+        assert np.array_equal(g, exp)
+
+
+def test_count_all_matches_oracle(g1, gties):
+    for d, ds, qk, _ in (g1, gties):
+        for q, k in qk[:6]:
+            cand = ds.filter(q, k)
+            exp = O.counts(d["ux"], d["uy"], d["fx"], d["fy"], q, cand=cand)
+            assert np.array_equal(ds.count_all(q, cand), exp)
+            allf = np.arange(ds.nf, dtype=np.uint32)
+            exp = O.counts(d["ux"], d["uy"], d["fx"], d["fy"], q, cand=allf)
+            assert np.array_equal(ds.count_all(q, allf), exp)
+
+
+def test_hand_cases_on_gpu(rk, ctx):
+    from test_oracle import hand_cases
+
+    for U, F, q, k, exp in hand_cases():
+        U = np.array(U, np.float32)
+        F = np.array(F, np.float32)
+        uid = np.arange(100, 100 + len(U), dtype=np.int32)
+        ds = rk.Dataset(ctx, U[:, 0], U[:, 1], uid, F[:, 0], F[:, 1], np.arange(len(F), dtype=np.int32))
+        try:
+            assert list(ds.query_bruteforce(q, k)) == exp
+            assert list(ds.query(q, k)) == exp
+            assert list(ds.query_batch([(q, k)])[0]) == exp
+        finally:
+            ds.close()
+
+
+def test_edge_cases(rk, ctx):
+    r = np.random.default_rng(5)
+    nu, nf = 1000, 50
+    ux, uy = r.random(nu, np.float32) * 100, r.random(nu, np.float32) * 100
+    uid = np.arange(nf, nf + nu, dtype=np.int32)
+    fx, fy = r.random(nf, np.float32) * 100, r.random(nf, np.float32) * 100
+    fid = np.arange(nf, dtype=np.int32)
+    ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    try:
+        # zero candidates: every user qualifies (rknn-inf-gpu returns none here)
+        assert np.array_equal(ds.query_plain(3, 1, np.zeros(0, np.uint32)), np.sort(uid))
+        assert np.array_equal(ds.query_batch([(3, 1, np.zeros(0, np.uint32))])[0], np.sort(uid))
+        # k >= N_f: everyone
+        assert np.array_equal(ds.query(3, nf), np.sort(uid))
+        assert np.array_equal(ds.query_bruteforce(3, 10 ** 6), np.sort(uid))
+        # candidate list holding q itself and duplicates of q are ignored
+        exp = O.rknn(ux, uy, uid, fx, fy, 3, 2, cand=np.array([3, 3, 7, 9], np.uint32))
+        assert np.array_equal(ds.query_plain(3, 2, np.array([3, 3, 7, 9], np.uint32)), exp)
+        # invalid arguments
+        for bad in [(nf, 1), (0, 0)]:
+            with pytest.raises(rk.RkError) as e:
+                ds.query(*bad)
+            assert e.value.status == 2
+        with pytest.raises(rk.RkError):
+            ds.query_plain(0, 1, np.array([nf + 5], np.uint32))
+        with pytest.raises(rk.RkError):
+            ds.query_batch([(nf, 1)])
+    finally:
+        ds.close()
+    # one user; all users at one point; facilities co-located with q
+    for U, F in [([(5.0, 5.0)], [(0.0, 0.0), (10.0, 10.0)]),
+                 ([(1.0, 1.0)] * 130, [(0.0, 0.0), (2.0, 2.0), (1.0, 0.0)]),
+                 ([(3.0, 4.0), (0.5, 0.5), (7.0, 1.0)] * 50, [(1.0, 1.0), (1.0, 1.0), (1.0, 1.0), (5.0, 5.0)])]:
+        U, F = np.array(U, np.float32), np.array(F, np.float32)
+        uid = np.arange(len(U), dtype=np.int32) * 3 + 7
+        ds = rk.Dataset(ctx, U[:, 0], U[:, 1], uid, F[:, 0], F[:, 1], np.arange(len(F), dtype=np.int32))
+        try:
+            for q in range(len(F)):
+                for k in (1, 2, 3):
+                    exp = O.rknn(U[:, 0], U[:, 1], uid, F[:, 0], F[:, 1], q, k)
+                    assert np.array_equal(ds.query(q, k), exp)
+                    assert np.array_equal(ds.query_bruteforce(q, k), exp)
+                    assert np.array_equal(ds.query_batch([(q, k)])[0], exp)
+        finally:
+            ds.close()
+
+
+@pytest.mark.parametrize("config", [2, 3, 4])
+def test_medium_random_vs_oracle(rk, ctx, config):
+    nu, nf = 200_000, 5_000
+    seed = rk.default_seed(config)
+    ux, uy, uid, fx, fy, fid = rk.generate(config, nu, nf, seed=seed, nq_ties=16, seed_ties=seed)
+    qs = rk.generate_queries(nf, 16, seed)
+    ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    O.set_threads(16)
+    try:
+        ks = [1, 5, 10, 25, 50, 3, 7, 10, 10, 2, 40, 10, 15, 20, 30, 10]
+        batch = ds.query_batch([(int(q), k) for q, k in zip(qs, ks)])
+        for i, (q, k) in enumerate(zip(qs, ks)):
+            print("config", config, "q", int(q), "k", k, flush=True)
+            cand = ds.filter(int(q), k)
+            exp = O.rknn(ux, uy, uid, fx, fy, int(q), k, cand=cand)
+            got = ds.query_plain(int(q), k, cand)
+            assert np.array_equal(got, exp)
+            assert np.array_equal(batch[i], exp)
+            if i < 4:  # the filter is sufficient: same set as brute force
+                assert np.array_equal(O.rknn(ux, uy, uid, fx, fy, int(q), k), exp)
+    finally:
+        ds.close()

@@ -1,0 +1,285 @@
+"""Pins of the CPU oracle (oracle/oracle.c).  CPU only.
+
+Each check is chosen so that a plausible mistake fails it; test_mutants_fail
+proves that for five planted mutants (ties lost by q, transposed operand,
+off-by-one keep rule, dropped term, wrong exclusion index).
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+try:
+    from scipy.spatial import cKDTree
+except ImportError:  # pragma: no cover
+    cKDTree = None
+
+
+def _ids(n, base=0):
+    return np.arange(base, base + n, dtype=np.int32)
+
+
+# ----------------------------------------------------------------- hand cases
+
+def hand_cases():
+    """(users xy, facilities xy, q, k, expected sorted user ids); ids = 100+i."""
+    cases = []
+    # on a line: q=(0,0), f=(10,0); u=(5,0) is an exact tie -> q wins
+    U = [(2, 0), (8, 0), (5, 0), (12, 0)]
+    F = [(0, 0), (10, 0)]
+    cases.append((U, F, 0, 1, [100, 102]))
+    cases.append((U, F, 1, 1, [101, 102, 103]))
+    cases.append((U, F, 0, 2, [100, 101, 102, 103]))          # k >= N_f: everyone
+    # vertical pair -> horizontal bisector y = 5
+    U = [(3, 4), (3, 6), (7, 5), (-50, 5)]
+    F = [(0, 0), (0, 10)]
+    cases.append((U, F, 0, 1, [100, 102, 103]))
+    cases.append((U, F, 1, 1, [101, 102, 103]))
+    # horizontal pair -> vertical bisector x = 5 (transposed variant of the above)
+    U = [(4, 3), (6, 3), (5, 7), (5, -50)]
+    F = [(0, 0), (10, 0)]
+    cases.append((U, F, 0, 1, [100, 102, 103]))
+    cases.append((U, F, 1, 1, [101, 102, 103]))
+    # facility co-located with q never counts; duplicate users share a fate
+    U = [(1, 1), (1, 1), (9, 9)]
+    F = [(0, 0), (0, 0), (10, 10)]
+    cases.append((U, F, 0, 1, [100, 101]))
+    cases.append((U, F, 1, 1, [100, 101]))
+    cases.append((U, F, 2, 1, [102]))
+    # user equidistant to q and two others (circle through three facilities)
+    U = [(0, 0)]
+    F = [(5, 0), (-5, 0), (0, 5), (3, 3)]
+    cases.append((U, F, 0, 1, []))          # (3,3) is strictly closer (d2=18 < 25)
+    cases.append((U, F, 0, 2, [100]))       # ties with (-5,0),(0,5) do not count
+    cases.append((U, F, 3, 1, [100]))
+    # diagonal: bisector y = -x + 10 through (5,5); u on it ties
+    U = [(5, 5), (4, 5), (6, 5)]
+    F = [(0, 0), (10, 10)]
+    cases.append((U, F, 0, 1, [100, 101]))
+    cases.append((U, F, 1, 1, [100, 102]))
+    # wrong-index mutant: a third facility near the query matters
+    U = [(0, 1), (20, 1)]
+    F = [(0, 0), (0, 2), (20, 0)]
+    cases.append((U, F, 0, 1, [100]))
+    cases.append((U, F, 1, 1, [100]))
+    cases.append((U, F, 2, 1, [101]))
+    # anisotropic: only y differs; a transposed operand gets it wrong
+    U = [(0, 3), (100, 3)]
+    F = [(0, 0), (0, 4)]
+    cases.append((U, F, 0, 1, []))
+    cases.append((U, F, 1, 1, [100, 101]))
+    return cases
+
+
+def run_hand(mutant=0):
+    bad = []
+    for i, (U, F, q, k, exp) in enumerate(hand_cases()):
+        U = np.array(U, np.float32)
+        F = np.array(F, np.float32)
+        got = O.rknn(U[:, 0], U[:, 1], _ids(len(U), 100), F[:, 0], F[:, 1], q, k, mutant=mutant)
+        if list(got) != exp:
+            bad.append((i, list(got), exp))
+    return bad
+
+
+def test_hand_cases():
+    assert run_hand() == []
+
+
+# ------------------------------------------------------- invariants (tie-free)
+
+def tiefree(seed, nu=600, nf=40):
+    r = np.random.default_rng(seed)
+    ux, uy = (r.random(nu) * 1000).astype(np.float32), (r.random(nu) * 1000).astype(np.float32)
+    fx, fy = (r.random(nf) * 1000).astype(np.float32), (r.random(nf) * 1000).astype(np.float32)
+    return ux, uy, _ids(nu, nf), fx, fy, _ids(nf)
+
+
+def run_invariants(mutant=0, seeds=(1, 2)):
+    bad = []
+    for seed in seeds:
+        ux, uy, uid, fx, fy, fid = tiefree(seed)
+        nu, nf = len(ux), len(fx)
+        for k in (1, 3, 7):
+            tot = sum(len(O.rknn(ux, uy, uid, fx, fy, q, k, mutant=mutant)) for q in range(nf))
+            if tot != nu * min(k, nf):
+                bad.append(("sum", seed, k, tot))
+        for q in (0, 5):
+            prev = set()
+            for k in range(1, 8):
+                cur = set(O.rknn(ux, uy, uid, fx, fy, q, k, mutant=mutant).tolist())
+                if not prev <= cur:
+                    bad.append(("monotone", seed, q, k))
+                prev = cur
+            allk = O.rknn(ux, uy, uid, fx, fy, q, nf, mutant=mutant)
+            if len(allk) != nu:
+                bad.append(("k>=nf", seed, q))
+    return bad
+
+
+def test_invariants():
+    assert run_invariants() == []
+
+
+def run_kdtree(mutant=0):
+    """k-NN by a KD-tree (scipy): u in R(q,k) iff q is among its k nearest."""
+    bad = []
+    for seed in (3, 4):
+        ux, uy, uid, fx, fy, fid = tiefree(seed, nu=800, nf=60)
+        tree = cKDTree(np.stack([fx, fy], 1).astype(np.float64))
+        for k in (1, 4, 9):
+            _, nn = tree.query(np.stack([ux, uy], 1).astype(np.float64), k=k)
+            nn = nn.reshape(len(ux), -1)
+            for q in (0, 17, 59):
+                exp = np.sort(uid[(nn == q).any(1)])
+                got = O.rknn(ux, uy, uid, fx, fy, q, k, mutant=mutant)
+                if not np.array_equal(got, exp):
+                    bad.append((seed, k, q))
+    return bad
+
+
+@pytest.mark.skipif(cKDTree is None, reason="scipy missing")
+def test_kdtree_rank():
+    assert run_kdtree() == []
+
+
+def test_k1_is_closed_voronoi_cell():
+    ux, uy, uid, fx, fy, fid = tiefree(5, nu=1000, nf=30)
+    d = (ux[:, None].astype(np.float64) - fx[None]) ** 2 + (uy[:, None].astype(np.float64) - fy[None]) ** 2
+    owner = d.argmin(1)
+    for q in range(30):
+        assert np.array_equal(O.rknn(ux, uy, uid, fx, fy, q, 1), np.sort(uid[owner == q]))
+
+
+def test_candidate_subset_and_counts():
+    ux, uy, uid, fx, fy, fid = tiefree(6, nu=300, nf=25)
+    full = O.counts(ux, uy, fx, fy, 3)
+    # count-all equals the number of facilities strictly closer than q
+    d = (ux[:, None].astype(np.float64) - fx[None]) ** 2 + (uy[:, None].astype(np.float64) - fy[None]) ** 2
+    assert np.array_equal(full, (d < d[:, [3]]).sum(1))
+    # a candidate list equal to F\{q} (any order, q included or not) is the brute force
+    perm = np.random.default_rng(0).permutation(25).astype(np.uint32)
+    assert np.array_equal(O.counts(ux, uy, fx, fy, 3, cand=perm), full)
+    assert np.array_equal(O.rknn(ux, uy, uid, fx, fy, 3, 4, cand=perm), O.rknn(ux, uy, uid, fx, fy, 3, 4))
+    # the cap stops counting
+    assert np.array_equal(O.counts(ux, uy, fx, fy, 3, cap=2), np.minimum(full, 2))
+
+
+def test_invalid_arguments():
+    ux, uy, uid, fx, fy, fid = tiefree(7, nu=10, nf=5)
+    with pytest.raises(ValueError):
+        O.rknn(ux, uy, uid, fx, fy, 5, 1)
+    with pytest.raises(ValueError):
+        O.rknn(ux, uy, uid, fx, fy, 0, 0)
+
+
+# ---------------------------------------------------- reference CPU programs
+
+def _write_int_file(path, ux, uy, uid, fx, fy, fid):
+    with open(path, "w") as f:
+        f.write(f"{len(fx)}\n")
+        for i, x, y in zip(fid, fx, fy):
+            f.write(f"{i} {x} {y}\n")
+        f.write(f"{len(ux)}\n")
+        for i, x, y in zip(uid, ux, uy):
+            f.write(f"{i} {x} {y}\n")
+
+
+def _ref_count(binary, path, q, k):
+    out = subprocess.run([binary, "-if", path, "-k", str(k), "-q", str(q)], capture_output=True, text=True,
+                         check=True).stdout
+    for line in out.splitlines():
+        if line.startswith("Found "):
+            return int(line.split()[1])
+    raise AssertionError(out)
+
+
+def int_data(seed, nu=400, nf=30, side=1000):
+    r = np.random.default_rng(seed)
+    return (r.integers(0, side, nu), r.integers(0, side, nu), _ids(nu, nf),
+            r.integers(0, side, nf), r.integers(0, side, nf), _ids(nf))
+
+
+def has_q_tie(ux, uy, fx, fy, q):
+    d = (ux[:, None].astype(np.int64) - fx[None]) ** 2 + (uy[:, None].astype(np.int64) - fy[None]) ** 2
+    dq = d[:, [q]]
+    d[:, q] = -1
+    return bool((d == dq).any())
+
+
+def run_reference(mutant=0, with_inf=True):
+    """Counts of the reference rknn-naive and rknn-inf binaries (built from
+    their own sources) against the FP64 oracle on tie-free integer data.
+
+    rknn-naive must agree exactly.  rknn-inf reports a superset: its filter
+    can drop a bisector the verification needs (a vertex's pruning count is
+    taken over bisectors that include the vertex's own generating line,
+    rknn-inf/inf.cpp:384-389 and :414-419, so rounding can inflate it and
+    the vertex is erased at :436-443), which only ever adds users.  So its
+    count may exceed the oracle's but never fall below it."""
+    naive, inf = O.ref_binary("rknn-naive"), O.ref_binary("rknn-inf")
+    bad, over = [], []
+    with tempfile.TemporaryDirectory() as td:
+        for seed in (11, 12):
+            ux, uy, uid, fx, fy, fid = int_data(seed)
+            path = os.path.join(td, f"d{seed}.txt")
+            _write_int_file(path, ux, uy, uid, fx, fy, fid)
+            for q in (0, 7, 22):
+                if has_q_tie(ux, uy, fx, fy, q):
+                    continue
+                for k in (1, 4, 10):
+                    got = len(O.rknn(ux, uy, uid, fx, fy, q, k, mutant=mutant))
+                    if got != _ref_count(naive, path, q, k):
+                        bad.append(("naive", seed, q, k))
+                    if with_inf:
+                        ic = _ref_count(inf, path, q, k)
+                        if ic < got:
+                            bad.append(("inf-subset", seed, q, k))
+                        elif ic > got:
+                            over.append((seed, q, k, ic, got))
+    return bad, over
+
+
+need_ref = pytest.mark.skipif(O.ref_binary("rknn-naive") is None or O.ref_binary("rknn-inf") is None,
+                              reason="reference CPU programs not built (oracle/_ref)")
+
+
+@need_ref
+def test_reference_programs_agree():
+    bad, over = run_reference()
+    assert bad == []
+    # most queries agree exactly; the over-reports are the rknn-inf filter's
+    assert len(over) <= 6, over
+
+
+@need_ref
+def test_literal_naive_matches_reference_with_ties():
+    """Integer data full of ties: the literal rknn-naive restatement (float
+    distance, (dist,x,y,id) order) reproduces the reference binary's counts."""
+    with tempfile.TemporaryDirectory() as td:
+        ux, uy, uid, fx, fy, fid = int_data(21, nu=300, nf=25, side=40)
+        path = os.path.join(td, "ties.txt")
+        _write_int_file(path, ux, uy, uid, fx, fy, fid)
+        ties = 0
+        for q in (0, 3, 9, 24):
+            ties += has_q_tie(ux, uy, fx, fy, q)
+            for k in (1, 2, 5):
+                lit = O.rknn_naive_int(ux, uy, uid, fx, fy, fid, q, k)
+                assert len(lit) == _ref_count(O.ref_binary("rknn-naive"), path, q, k)
+        assert ties > 0  # the data really exercises the tie order
+
+
+# -------------------------------------------------------------------- mutants
+
+@pytest.mark.parametrize("mutant", [1, 2, 3, 4, 5])
+def test_mutants_fail(mutant):
+    caught = bool(run_hand(mutant)) or bool(run_invariants(mutant, seeds=(1,)))
+    if not caught and cKDTree is not None:
+        caught = bool(run_kdtree(mutant))
+    if not caught and O.ref_binary("rknn-naive") and O.ref_binary("rknn-inf"):
+        caught = bool(run_reference(mutant, with_inf=False)[0])
+    assert caught, f"planted mutant {mutant} passed every pin"
