@@ -68,8 +68,8 @@ __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, ui
     const uint32_t lane = lane_id();
     const uint32_t u = tile * kTile + lane;
     const bool valid = u < ds.nu;
-    const float wx = ds.ux[u] - qx; // arrays are padded to whole tiles
-    const float wy = ds.uy[u] - qy;
+    float wx = 0.f, wy = 0.f; // user offsets, loaded on the first crossing candidate
+    bool loaded = false;      // wave-uniform
     const float4 m = ds.mbr[0][tile];
     const float lx = m.x - qx, ly = m.y - qy, hx = m.z - qx, hy = m.w - qy;
 
@@ -93,6 +93,11 @@ __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, ui
         }
         uint64_t cm = __ballot(cross);
         if (cm) {
+            if (!loaded) { // culled tiles never touch their users (arrays padded to whole tiles)
+                wx = ds.ux[u] - qx;
+                wy = ds.uy[u] - qy;
+                loaded = true;
+            }
             wlds[lane] = r;
             __builtin_amdgcn_wave_barrier();
             while (cm) {
@@ -146,38 +151,95 @@ __global__ __launch_bounds__(kBlock) void k_plain(DsDev ds, QDev q, const CandRe
 
 // ------------------------------------------------------------ count-all form
 
-// Every (user, candidate) pair is evaluated; no culling, no early exit.
+// Every (user, candidate) pair is evaluated: no culling, no early exit.  A
+// wave owns 4 consecutive tiles (256 users, 4 per lane) so that each
+// broadcast LDS read of a record feeds 4 users and the loop is VALU-bound.
+// Per 64-candidate chunk the candidates are split against the group's
+// rectangle: "clean" ones, whose computed t is >= 19/16 (or <= -3/16) at the
+// rectangle's extreme corner, give every user t >= 1 (or <= 0) because the
+// fp32 error is < 1/16 per evaluation, so the saturated value is exactly 1
+// or 0 and a float sum counts them exactly (2 FMAs with clamp + 1 add per
+// test); the others take the banded path with the binary64 re-check.
+constexpr int kCountUsers = 4;
+
 __global__ __launch_bounds__(kBlock) void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec,
                                                       const float2* __restrict__ orig,
                                                       uint32_t* __restrict__ counts, uint32_t* __restrict__ rech_out)
 {
-    __shared__ CandRec s_rec[kWaves][64];
+    __shared__ CandRec s_rec[kWaves][128];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
     const uint32_t stride = gridDim.x * kWaves;
+    const uint32_t n_groups = (ds.n_tiles + kCountUsers - 1) / kCountUsers;
     uint32_t rech = 0;
     CandRec* wlds = s_rec[wave];
-    for (uint32_t tile = blockIdx.x * kWaves + wave; tile < ds.n_tiles; tile += stride) {
-        const uint32_t u = tile * kTile + lane;
-        const float ux = ds.ux[u], uy = ds.uy[u];
-        const float wx = ux - q.qx, wy = uy - q.qy;
-        uint32_t cnt = 0;
-        bool band = false;
+    for (uint32_t g = blockIdx.x * kWaves + wave; g < n_groups; g += stride) {
+        const uint32_t t0 = g * kCountUsers;
+        const uint32_t nt = min((uint32_t)kCountUsers, ds.n_tiles - t0);
+        float4 m = ds.mbr[0][t0];
+        float wx[kCountUsers], wy[kCountUsers], acc[kCountUsers];
+        uint32_t cnt[kCountUsers];
+        bool band[kCountUsers];
+#pragma unroll
+        for (int j = 0; j < kCountUsers; ++j) {
+            wx[j] = wy[j] = acc[j] = 0.f;
+            cnt[j] = 0;
+            band[j] = false;
+            if ((uint32_t)j < nt) {
+                const uint32_t u = (t0 + j) * kTile + lane;
+                wx[j] = ds.ux[u] - q.qx;
+                wy[j] = ds.uy[u] - q.qy;
+                if (j) {
+                    const float4 b = ds.mbr[0][t0 + j];
+                    m.x = fminf(m.x, b.x);
+                    m.y = fminf(m.y, b.y);
+                    m.z = fmaxf(m.z, b.z);
+                    m.w = fmaxf(m.w, b.w);
+                }
+            }
+        }
+        const float lx = m.x - q.qx, ly = m.y - q.qy, hx = m.z - q.qx, hy = m.w - q.qy;
         for (uint32_t c0 = 0; c0 < q.nc; c0 += 64) {
-            const uint32_t n = min(64u, q.nc - c0);
-            if (lane < n) wlds[lane] = rec[c0 + lane];
+            const uint32_t jc = c0 + lane;
+            const bool have = jc < q.nc;
+            CandRec r = {0.f, 0.f, 0.f, 0.f};
+            if (have) r = rec[jc];
+            const float tlo = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
+            const float thi = fmaf(r.gx >= 0.f ? hx : lx, r.gx, fmaf(r.gy >= 0.f ? hy : ly, r.gy, -r.sh));
+            const bool clean = have && (tlo >= 1.1875f || thi <= -0.1875f);
+            const bool slow = have && !clean;
+            const uint64_t mc = __ballot(clean), ms = __ballot(slow);
+            if (clean) wlds[rank_below(mc)] = r;
+            if (slow) wlds[64 + rank_below(ms)] = r;
             __builtin_amdgcn_wave_barrier();
-            for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t n_clean = (uint32_t)__popcll(mc), n_slow = (uint32_t)__popcll(ms);
+#pragma unroll 4
+            for (uint32_t i = 0; i < n_clean; ++i) {
                 const CandRec c = wlds[i];
-                const float t = fmaf(wx, c.gx, fmaf(wy, c.gy, -c.sh));
-                cnt += (t >= kCertainHi) ? 1u : 0u;
-                band |= (t > kCertainLo) && (t < kCertainHi);
+#pragma unroll
+                for (int j = 0; j < kCountUsers; ++j) // med3(t,0,1) folds into the FMA's clamp bit
+                    acc[j] += __builtin_amdgcn_fmed3f(fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh)), 0.0f, 1.0f);
+            }
+            for (uint32_t i = 0; i < n_slow; ++i) {
+                const CandRec c = wlds[64 + i];
+#pragma unroll
+                for (int j = 0; j < kCountUsers; ++j) {
+                    const float t = fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh));
+                    cnt[j] += (t >= kCertainHi) ? 1u : 0u;
+                    band[j] |= (t > kCertainLo) && (t < kCertainHi);
+                }
             }
             __builtin_amdgcn_wave_barrier();
         }
-        if (band) cnt = exact_count(ux, uy, q.qx, q.qy, orig, q.nc, 0xffffffffu);
-        rech += (uint32_t)__popcll(__ballot(band && u < ds.nu));
-        counts[u] = cnt;
+#pragma unroll
+        for (int j = 0; j < kCountUsers; ++j) {
+            if ((uint32_t)j >= nt) break;
+            const uint32_t u = (t0 + j) * kTile + lane;
+            uint32_t c = (uint32_t)acc[j] + cnt[j];
+            if (band[j]) c = exact_count(ds.ux[u], ds.uy[u], q.qx, q.qy, orig, q.nc, 0xffffffffu);
+            rech += (uint32_t)__popcll(__ballot(band[j] && u < ds.nu));
+            counts[u] = c;
+        }
     }
     if (lane == 0 && rech) atomicAdd(rech_out, rech);
 }
@@ -520,7 +582,8 @@ void launch_plain(const DsDev& ds, QDev q, const CandRec* rec, const float2* ori
 void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, uint32_t* counts,
                       uint32_t* rechecks, int grid, hipStream_t st)
 {
-    k_count_all<<<grid_for(ds.n_tiles, kWaves, grid), kBlock, 0, st>>>(ds, q, rec, orig, counts, rechecks);
+    const uint32_t groups = (ds.n_tiles + kCountUsers - 1) / kCountUsers;
+    k_count_all<<<grid_for(groups, kWaves, grid), kBlock, 0, st>>>(ds, q, rec, orig, counts, rechecks);
 }
 
 void launch_scatter_u32(const uint32_t* src_sorted, const uint32_t* perm, uint32_t n, uint32_t* dst_orig,
