@@ -57,7 +57,8 @@ typedef struct rk_timing {
     double download_s;   /* result ids device -> host, including the sort   */
     double total_s;      /* whole call                                       */
     uint64_t n_candidates;
-    uint64_t n_fp64_rechecks; /* users whose FP32 margin fell in the band   */
+    uint64_t n_fp64_rechecks; /* (user, candidate) tests whose FP32 value fell
+                                 in the uncertainty band, settled in binary64 */
 } rk_timing;
 
 /* ---- lifecycle: replaces cuda_init / cuda_cleanup / is_cuda_available
@@ -70,7 +71,7 @@ const char* rk_last_error(const rk_ctx* ctx); /* detail of the last failure    *
 
 /* ---- dataset: replaces the per-call H2D copies of users and lines in
  *      gpu_get_rknn_candidates (inf_kernel.cu:88-95) and gpuReverseKNN
- *      (rknn-naive-gpu/rknn_kernel.cu:172-195); uploaded and Morton-sorted
+ *      (rknn-naive-gpu/rknn_kernel.cu:172-195); uploaded and Hilbert-sorted
  *      once, then device-resident.  The host keeps the facility index used by
  *      the filter (the facility R*-tree of rknn-inf/inf.cpp:174-177). ---- */
 rk_status rk_dataset_create(rk_ctx* ctx,
@@ -141,6 +142,11 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
 rk_status rk_batch_run(rk_batch* b, double* device_seconds);
 rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets);
 rk_status rk_batch_destroy(rk_batch* b);
+/* Work counters of the last run: items[l] = (query, node) items processed at
+ * tree level l (0 = leaf tiles; n_levels+1 entries, at most 9), results
+ * appended, users re-counted in binary64. */
+rk_status rk_batch_stats(const rk_batch* b, uint64_t* items, size_t n_items_cap, size_t* n_levels,
+                         uint64_t* results, uint64_t* rechecks);
 
 /* ---- serving: continuous batching over resident datasets. ---- */
 typedef struct rk_serve_config {

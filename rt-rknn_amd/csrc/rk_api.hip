@@ -125,20 +125,23 @@ rk_status plain_core(rk_dataset* ds, uint32_t q, uint32_t k, const std::vector<u
     const double t0 = now_s();
     rk_status s = upload_records(ds, q, cl);
     if (s != RK_OK) return s;
-    RK_HIP(c, ds->ctr.reserve(2));
-    RK_HIP(c, ds->h_ctr.reserve(2));
-    RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, 2 * sizeof(uint32_t), c->stream));
+    RK_HIP(c, ds->ctr.reserve(kPlainCounters));
+    RK_HIP(c, ds->h_ctr.reserve(kPlainCounters));
+    RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, kPlainCounters * sizeof(uint32_t), c->stream));
     const double t1 = now_s();
     QDev qd = make_qdev(ds, q, k, cl.size());
     RK_HIP(c, hipEventRecord(c->ev0, c->stream));
-    launch_plain(ds->view, qd, ds->rec.p, ds->orig.p, ds->out.p, ds->ctr.p, device_grid(c->n_cu, 16), c->stream);
+    launch_plain(ds->view, qd, ds->rec.p, ds->orig.p, ds->out.p, ds->ctr.p, device_grid(c->n_cu, 32), c->stream);
     RK_HIP(c, hipGetLastError());
     RK_HIP(c, hipEventRecord(c->ev1, c->stream));
-    RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, kPlainCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
     RK_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     const uint32_t n = ds->h_ctr.p[0];
+    uint64_t fp64 = 0;
+    for (int j = 1; j < kPlainCounters; ++j) fp64 += ds->h_ctr.p[j];
     if (n > ds->n_pad) return fail(c, RK_ERR_INTERNAL, "result count exceeds the user count");
     RK_HIP(c, ds->h_out.reserve(n));
     if (n) {
@@ -151,7 +154,7 @@ rk_status plain_core(rk_dataset* ds, uint32_t q, uint32_t k, const std::vector<u
         t->upload_s = t1 - t0;
         t->kernel_s = ms * 1e-3;
         t->n_candidates = cl.size();
-        t->n_fp64_rechecks = ds->h_ctr.p[1];
+        t->n_fp64_rechecks = fp64;
         t->total_s = now_s() - t_start;
         t->download_s = t->total_s - (t1 - t_start) - t->kernel_s;
     }
@@ -282,7 +285,7 @@ rk_status rk_dataset_create(rk_ctx* c, const float* ux, const float* uy, const i
         RK_HIP(c, hipMemcpyAsync(tid.p, uid, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
         const double w = ds->ubox[2] - ds->ubox[0], h = ds->ubox[3] - ds->ubox[1];
         const float sx = w > 0 ? (float)(65535.0 / w) : 0.f, sy = h > 0 ? (float)(65535.0 / h) : 0.f;
-        launch_morton_keys(tx.p, ty.p, n, (float)ds->ubox[0], (float)ds->ubox[1], sx, sy, keys.p, idx.p, st);
+        launch_curve_keys(tx.p, ty.p, n, (float)ds->ubox[0], (float)ds->ubox[1], sx, sy, keys.p, idx.p, st);
         RK_HIP(c, hipGetLastError());
         size_t tmp_bytes = 0;
         RK_HIP(c, hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.p, keys2.p, idx.p, ds->perm.p, (int)n,
@@ -437,26 +440,29 @@ rk_status rk_count_all(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t 
     if (s != RK_OK) return s;
     RK_HIP(c, ds->counts.reserve(ds->n_pad));
     RK_HIP(c, ds->counts_orig.reserve(ds->nu));
-    RK_HIP(c, ds->ctr.reserve(2));
-    RK_HIP(c, ds->h_ctr.reserve(2));
-    RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, 2 * sizeof(uint32_t), c->stream));
+    RK_HIP(c, ds->ctr.reserve(kPlainCounters));
+    RK_HIP(c, ds->h_ctr.reserve(kPlainCounters));
+    RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, kPlainCounters * sizeof(uint32_t), c->stream));
     QDev qd = make_qdev(ds, q, 1, cl.size());
     RK_HIP(c, hipEventRecord(c->ev0, c->stream));
-    launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, device_grid(c->n_cu, 16),
+    launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, device_grid(c->n_cu, 32),
                      c->stream);
     RK_HIP(c, hipGetLastError());
     RK_HIP(c, hipEventRecord(c->ev1, c->stream));
     launch_scatter_u32(ds->counts.p, ds->perm.p, (uint32_t)ds->nu, ds->counts_orig.p, c->stream);
     RK_HIP(c, hipGetLastError());
     RK_HIP(c, hipMemcpyAsync(counts, ds->counts_orig.p, ds->nu * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, kPlainCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
     RK_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (t) {
+        uint64_t fp64 = 0;
+        for (int j = 1; j < kPlainCounters; ++j) fp64 += ds->h_ctr.p[j];
         t->kernel_s = ms * 1e-3;
         t->n_candidates = cl.size();
-        t->n_fp64_rechecks = ds->h_ctr.p[1];
+        t->n_fp64_rechecks = fp64;
         t->upload_s = 0;
         t->total_s = now_s() - t0;
         t->filter_s = t1 - t0;

@@ -8,7 +8,9 @@
 //           items (query, node at level p); the node's 64 children on lanes;
 //           children not certainly covered by k candidates are pushed down
 //   leaf    items (query, tile); per-user count as in the plain form
-//   finish  per-query scan, scatter into per-query segments, LDS bitonic sort
+//   finish  per-query LDS bitonic sort of each result segment, offsets scan
+// Leaf survivors go straight into per-query result segments (one atomic per
+// tile with survivors); the fetch compacts the segments.
 // Every kernel is a grid-stride loop over a device-side item count, so the
 // graph is captured once and replayed unchanged for any data; nothing waits
 // on another wave, so every launch drains.
@@ -26,7 +28,8 @@ using namespace rk;
 namespace {
 
 constexpr uint64_t kItemBudget = 1ull << 25; // per level
-constexpr uint64_t kPoolBudget = 1ull << 26; // result entries per batch
+constexpr uint64_t kPoolBudget = 1ull << 26; // result ids per batch (all segments)
+constexpr int kWavesPerCU = 32;             // persistent grids: 8 waves per SIMD
 
 rk_status build_graph(rk_batch* b)
 {
@@ -34,7 +37,7 @@ rk_status build_graph(rk_batch* b)
     rk_ctx* c = ds->ctx;
     hipStream_t st = c->stream;
     const uint32_t root = ds->view.n_levels; // parent level whose single node holds the top level
-    const int grid = device_grid(c->n_cu, 16);
+    const int grid = device_grid(c->n_cu, kWavesPerCU);
     RK_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     if (b->total_cand)
         launch_prep_records(b->q.p, b->nq, b->cand.p, ds->fx.p, ds->fy.p, b->wb.p, b->rec.p, b->orig.p, b->total_cand,
@@ -42,7 +45,7 @@ rk_status build_graph(rk_batch* b)
     launch_batch_init(b->view, root, st);
     for (uint32_t p = root; p >= 1; --p) launch_batch_internal(ds->view, b->view, p, grid, st);
     launch_batch_leaf(ds->view, b->view, grid, st);
-    launch_batch_finish(b->view, grid, st);
+    launch_batch_finish(b->view, st);
     hipError_t e = hipStreamEndCapture(st, &b->graph);
     if (e != hipSuccess) return fail_hip(c, e, "hipStreamEndCapture", __LINE__);
     RK_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -112,18 +115,21 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
     RK_HIP(c, b->ctr.reserve(1));
     RK_HIP(c, b->h_ctr.reserve(1));
     // item lists: parent level p holds nodes of level p (p < root) or the root
+    // (the root level keeps all nq root items in bucket 0)
     for (uint32_t p = 0; p <= root; ++p) {
         uint64_t cap = (p == root) ? (uint64_t)nq : (uint64_t)nq * ds->view.nlev[p];
         cap = std::max<uint64_t>(1, std::min(cap, kItemBudget));
-        if (p == root) cap = std::max<uint64_t>(cap, nq);
-        RK_HIP(c, b->items[p].reserve(cap));
+        uint64_t bcap = (p == root) ? std::max<uint64_t>(1, nq) : (cap + kBuckets - 1) / kBuckets;
+        // a bucket also takes one wave's whole staged flush
+        bcap = std::max<uint64_t>(bcap, 256);
+        RK_HIP(c, b->items[p].reserve(bcap * kBuckets));
         b->view.items[p] = b->items[p].p;
-        b->view.item_cap[p] = (uint32_t)cap;
+        b->view.item_cap[p] = (uint32_t)bcap;
     }
-    const uint64_t pool = std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)nq * ds->nu, kPoolBudget));
-    RK_HIP(c, b->pool_uid.reserve(pool));
-    RK_HIP(c, b->pool_tag.reserve(pool));
-    RK_HIP(c, b->out.reserve(pool));
+    // result segment per query: the whole budget split evenly, never more than the users
+    uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? kPoolBudget / nq : 1));
+    seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
+    RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
     if (nq) RK_HIP(c, hipMemcpyAsync(b->q.p, b->hq.data(), nq * sizeof(QDev), hipMemcpyHostToDevice, st));
     if (nq) RK_HIP(c, hipMemcpyAsync(b->wb.p, wb.data(), nq * sizeof(double2), hipMemcpyHostToDevice, st));
     if (!all.empty())
@@ -135,9 +141,8 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
     v.nq = b->nq;
     v.rec = b->rec.p;
     v.orig = b->orig.p;
-    v.pool_uid = b->pool_uid.p;
-    v.pool_tag = b->pool_tag.p;
-    v.pool_cap = (uint32_t)pool;
+    v.seg = b->seg.p;
+    v.seg_cap = (uint32_t)seg_cap;
     v.qcount = b->qcount.p;
     v.qoff = b->qoff.p;
     v.out = b->out.p;
@@ -199,8 +204,13 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
     if (total > cap) return fail(c, RK_ERR_CAPACITY, "output buffer too small; offsets[nq] holds the size needed");
     std::vector<int32_t> dev;
     if (b->exec && qoff[nq]) {
+        RK_HIP(c, b->out.reserve(qoff[nq]));
+        b->view.out = b->out.p; // compaction is outside the graph, so the pointer may change
+        launch_batch_compact(b->view, c->stream);
+        RK_HIP(c, hipGetLastError());
         dev.resize(qoff[nq]);
-        RK_HIP(c, hipMemcpy(dev.data(), b->out.p, qoff[nq] * sizeof(int32_t), hipMemcpyDeviceToHost));
+        RK_HIP(c, hipMemcpyAsync(dev.data(), b->out.p, qoff[nq] * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        RK_HIP(c, hipStreamSynchronize(c->stream));
     }
     for (uint32_t i = 0; i < nq; ++i) {
         int32_t* dst = out_ids + offsets[i];
@@ -211,6 +221,27 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
             std::memcpy(dst, dev.data() + qoff[i], n * sizeof(int32_t));
             if (n > 8192) std::sort(dst, dst + n); // larger than the device LDS sort
         }
+    }
+    return RK_OK;
+}
+
+rk_status rk_batch_stats(const rk_batch* b, uint64_t* items, size_t n_items_cap, size_t* n_levels, uint64_t* results,
+                         uint64_t* rechecks)
+{
+    if (!b || !b->ran) return RK_ERR_INVALID_ARG;
+    const BatchCounters& c = *b->h_ctr.p;
+    const size_t nl = b->ds->view.n_levels + 1;
+    if (n_levels) *n_levels = nl;
+    for (size_t l = 0; l < nl && l < n_items_cap && items; ++l) {
+        uint64_t t = 0;
+        for (int j = 0; j < kBuckets; ++j) t += std::min(c.items[l][j], b->view.item_cap[l]);
+        items[l] = t;
+    }
+    if (results) *results = c.results;
+    if (rechecks) {
+        uint64_t t = 0;
+        for (int j = 0; j < kBuckets; ++j) t += c.rechecks[j];
+        *rechecks = t;
     }
     return RK_OK;
 }
@@ -274,7 +305,9 @@ rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq, int
         t->kernel_s = dev;
         t->download_s = now_s() - t2;
         t->n_candidates = b->total_cand;
-        t->n_fp64_rechecks = b->h_ctr.p ? b->h_ctr.p->rechecks : 0;
+        uint64_t rech = 0;
+        for (int j = 0; j < kBuckets && b->h_ctr.p; ++j) rech += b->h_ctr.p->rechecks[j];
+        t->n_fp64_rechecks = rech;
         t->total_s = now_s() - t0;
     }
     rk_batch_destroy(b);

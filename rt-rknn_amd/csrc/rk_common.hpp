@@ -107,7 +107,7 @@ RK_HD inline double d2_exact(float ax, float ay, float bx, float by)
     return x2 + y2;
 }
 
-// Device view of a resident dataset (users Morton-sorted, padded to whole
+// Device view of a resident dataset (users Hilbert-sorted, padded to whole
 // tiles by repeating the last user, which leaves every MBR unchanged).
 struct DsDev {
     const float* ux;

@@ -98,7 +98,7 @@ struct rk_dataset {
     rk_ctx* ctx = nullptr;
     uint64_t nu = 0, nf = 0;
     uint32_t n_tiles = 0, n_pad = 0;
-    rk::DevBuf<float> ux, uy;        // Morton-sorted, padded
+    rk::DevBuf<float> ux, uy;        // Hilbert-sorted, padded
     rk::DevBuf<int32_t> uid;
     rk::DevBuf<uint32_t> perm;       // sorted position -> caller's user index
     rk::DevBuf<float4> mbr[rk::kMaxLevels];
@@ -132,8 +132,7 @@ struct rk_batch {
     rk::DevBuf<rk::CandRec> rec;
     rk::DevBuf<float2> orig;
     rk::DevBuf<uint64_t> items[rk::kMaxLevels + 1];
-    rk::DevBuf<int32_t> pool_uid;
-    rk::DevBuf<uint64_t> pool_tag;
+    rk::DevBuf<int32_t> seg;     // per-query result segments
     rk::DevBuf<uint32_t> qcount;
     rk::DevBuf<uint64_t> qoff;
     rk::DevBuf<int32_t> out;

@@ -1,6 +1,6 @@
 // rk_kernels.hip -- gfx950 kernels of the RkNN dominance-count engine.
 //
-// Wave64 throughout: a leaf tile is 64 Morton-adjacent users (one per lane),
+// Wave64 throughout: a leaf tile is 64 Hilbert-adjacent users (one per lane),
 // a tree node has 64 children (one per lane), and candidate records are
 // staged 64 at a time in a wave-private LDS slot and read back as
 // broadcasts.  Blocks are 256 threads = 4 independent waves; no kernel uses
@@ -31,57 +31,74 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t mask)
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-__device__ __forceinline__ uint32_t wave_bcast(uint32_t v, int src) { return (uint32_t)__shfl((int)v, src); }
-
-// Exact count of candidates strictly closer than q under the oracle's
-// binary64 expression (rknn-naive/rknn.cpp:45-47 distances, strict '<' of
-// rknn-inf/inf.h:77-93), stopping at cap.
-__device__ uint32_t exact_count(float ux, float uy, float qx, float qy, const float2* __restrict__ orig,
-                                uint32_t nc, uint32_t cap)
+// Values every lane holds identically, moved to scalar registers so that the
+// loops and loads depending on them compile to scalar branches and loads.
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uniform64(uint64_t v)
 {
-    const double dq = d2_exact(ux, uy, qx, qy);
-    uint32_t c = 0;
-    for (uint32_t j = 0; j < nc; ++j) {
-        const float2 f = orig[j];
-        if (d2_exact(ux, uy, f.x, f.y) < dq) {
-            if (++c >= cap) break;
-        }
-    }
-    return c;
+    return ((uint64_t)uniform((uint32_t)(v >> 32)) << 32) | uniform((uint32_t)v);
+}
+__device__ __forceinline__ float uniformf(float v) { return __int_as_float((int)uniform((uint32_t)__float_as_int(v))); }
+__device__ __forceinline__ QDev uniform_q(const QDev& q)
+{
+    QDev r;
+    r.qx = uniformf(q.qx);
+    r.qy = uniformf(q.qy);
+    r.k = uniform(q.k);
+    r.nc = uniform(q.nc);
+    r.off = uniform64(q.off);
+    r.trivial = uniform(q.trivial);
+    r.pad = 0;
+    return r;
 }
 
 struct LeafOut {
     uint64_t keep;     // lanes (users) of the tile in R(q,k)
-    uint32_t rechecks; // lanes resolved in binary64
+    uint32_t fp64;     // (user, candidate) tests resolved in binary64
+};
+
+// Wave-private LDS staging of one 64-candidate chunk.
+struct WaveLds {
+    CandRec rec[64];
+    float2 orig[64];
 };
 
 // One query against one tile of 64 users.  Phase A puts candidates on lanes
 // and classifies each against the tile's bounding rectangle: certainly
 // covering the whole tile (counted once for every user), certainly missing
 // it, or crossing it.  Phase B puts users on lanes and evaluates only the
-// crossing candidates.  Users whose certain count stays below k but who hit
-// the band of some candidate are re-counted in binary64.
+// crossing candidates; a (user, candidate) test whose fp32 value lands in the
+// band is settled on the spot by the oracle's binary64 expression, so the
+// count is exact when the loop ends.
 __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, uint32_t nc,
                              const CandRec* __restrict__ rec, const float2* __restrict__ orig, uint32_t tile,
-                             CandRec* wlds)
+                             WaveLds* L)
 {
     const uint32_t lane = lane_id();
+    k = uniform(k);
+    nc = uniform(nc);
+    tile = uniform(tile);
     const uint32_t u = tile * kTile + lane;
     const bool valid = u < ds.nu;
-    float wx = 0.f, wy = 0.f; // user offsets, loaded on the first crossing candidate
-    bool loaded = false;      // wave-uniform
+    float ux = 0.f, uy = 0.f, wx = 0.f, wy = 0.f; // loaded on the first crossing candidate
+    double dq = 0.0;
+    bool loaded = false; // wave-uniform
     const float4 m = ds.mbr[0][tile];
     const float lx = m.x - qx, ly = m.y - qy, hx = m.z - qx, hy = m.w - qy;
 
     uint32_t base = 0; // wave-uniform: candidates covering the whole tile
-    uint32_t cnt = 0;  // per user: crossing candidates certainly closer
-    bool band = false;
-    bool dead = false; // every user already has >= k certain
+    uint32_t cnt = 0;  // per user: crossing candidates closer (exact)
+    uint32_t fp64 = 0; // wave-uniform
+    bool dead = false; // every user already has >= k
     for (uint32_t c0 = 0; c0 < nc; c0 += 64) {
         const uint32_t j = c0 + lane;
         const bool have = j < nc;
         CandRec r = {0.f, 0.f, 0.f, 0.f};
-        if (have) r = rec[j];
+        float2 o = make_float2(0.f, 0.f);
+        if (have) {
+            r = rec[j];
+            o = orig[j];
+        }
         const float tlo = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
         const float thi = fmaf(r.gx >= 0.f ? hx : lx, r.gx, fmaf(r.gy >= 0.f ? hy : ly, r.gy, -r.sh));
         const bool full = have && tlo >= kCertainHi;
@@ -94,19 +111,29 @@ __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, ui
         uint64_t cm = __ballot(cross);
         if (cm) {
             if (!loaded) { // culled tiles never touch their users (arrays padded to whole tiles)
-                wx = ds.ux[u] - qx;
-                wy = ds.uy[u] - qy;
+                ux = ds.ux[u];
+                uy = ds.uy[u];
+                wx = ux - qx;
+                wy = uy - qy;
+                dq = d2_exact(ux, uy, qx, qy);
                 loaded = true;
             }
-            wlds[lane] = r;
+            L->rec[lane] = r;
+            L->orig[lane] = o;
             __builtin_amdgcn_wave_barrier();
             while (cm) {
                 const int b = __ffsll((unsigned long long)cm) - 1;
                 cm &= cm - 1;
-                const CandRec c = wlds[b];
+                const CandRec c = L->rec[b];
                 const float t = fmaf(wx, c.gx, fmaf(wy, c.gy, -c.sh));
                 cnt += (t >= kCertainHi) ? 1u : 0u;
-                band |= (t > kCertainLo) && (t < kCertainHi);
+                const bool inband = (t > kCertainLo) && (t < kCertainHi);
+                const uint64_t bm = __ballot(inband);
+                if (bm) { // rare: settle the band in binary64
+                    const float2 f = L->orig[b];
+                    if (inband && d2_exact(ux, uy, f.x, f.y) < dq) ++cnt;
+                    fp64 += (uint32_t)__popcll(bm & __ballot(valid));
+                }
             }
             __builtin_amdgcn_wave_barrier();
         }
@@ -115,14 +142,10 @@ __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, ui
             break;
         }
     }
-    LeafOut o;
-    const bool alive = !dead && valid && (base + cnt < k);
-    bool keep = alive;
-    const bool need = alive && band;
-    if (need) keep = exact_count(ds.ux[u], ds.uy[u], qx, qy, orig, nc, k) < k;
-    o.keep = __ballot(keep);
-    o.rechecks = (uint32_t)__popcll(__ballot(need));
-    return o;
+    LeafOut out;
+    out.keep = __ballot(!dead && valid && base + cnt < k);
+    out.fp64 = fp64;
+    return out;
 }
 
 // ---------------------------------------------------------------- plain form
@@ -131,22 +154,22 @@ __global__ __launch_bounds__(kBlock) void k_plain(DsDev ds, QDev q, const CandRe
                                                   const float2* __restrict__ orig, int32_t* __restrict__ out,
                                                   uint32_t* __restrict__ ctr)
 {
-    __shared__ CandRec s_rec[kWaves][64];
+    __shared__ WaveLds s_w[kWaves];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
     const uint32_t stride = gridDim.x * kWaves;
     uint32_t rech = 0;
     for (uint32_t tile = blockIdx.x * kWaves + wave; tile < ds.n_tiles; tile += stride) {
-        const LeafOut r = leaf_tile(ds, q.qx, q.qy, q.k, q.nc, rec, orig, tile, s_rec[wave]);
-        rech += r.rechecks;
+        const LeafOut r = leaf_tile(ds, q.qx, q.qy, q.k, q.nc, rec, orig, tile, &s_w[wave]);
+        rech += r.fp64;
         if (r.keep) {
             uint32_t pos = 0;
             if (lane == 0) pos = atomicAdd(&ctr[0], (uint32_t)__popcll(r.keep));
-            pos = wave_bcast(pos, 0);
+            pos = uniform(pos);
             if ((r.keep >> lane) & 1ull) out[pos + rank_below(r.keep)] = ds.uid[tile * kTile + lane];
         }
     }
-    if (lane == 0 && rech) atomicAdd(&ctr[1], rech);
+    if (lane == 0 && rech) atomicAdd(&ctr[1 + (blockIdx.x * kWaves + wave) % kBuckets], rech);
 }
 
 // ------------------------------------------------------------ count-all form
@@ -167,28 +190,33 @@ __global__ __launch_bounds__(kBlock) void k_count_all(DsDev ds, QDev q, const Ca
                                                       uint32_t* __restrict__ counts, uint32_t* __restrict__ rech_out)
 {
     __shared__ CandRec s_rec[kWaves][128];
+    __shared__ float2 s_orig[kWaves][64];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
     const uint32_t stride = gridDim.x * kWaves;
     const uint32_t n_groups = (ds.n_tiles + kCountUsers - 1) / kCountUsers;
     uint32_t rech = 0;
     CandRec* wlds = s_rec[wave];
+    float2* worig = s_orig[wave];
     for (uint32_t g = blockIdx.x * kWaves + wave; g < n_groups; g += stride) {
         const uint32_t t0 = g * kCountUsers;
         const uint32_t nt = min((uint32_t)kCountUsers, ds.n_tiles - t0);
         float4 m = ds.mbr[0][t0];
-        float wx[kCountUsers], wy[kCountUsers], acc[kCountUsers];
+        float px[kCountUsers], py[kCountUsers], wx[kCountUsers], wy[kCountUsers], acc[kCountUsers];
+        double dq[kCountUsers];
         uint32_t cnt[kCountUsers];
-        bool band[kCountUsers];
 #pragma unroll
         for (int j = 0; j < kCountUsers; ++j) {
-            wx[j] = wy[j] = acc[j] = 0.f;
+            px[j] = py[j] = wx[j] = wy[j] = acc[j] = 0.f;
+            dq[j] = 0.0;
             cnt[j] = 0;
-            band[j] = false;
             if ((uint32_t)j < nt) {
                 const uint32_t u = (t0 + j) * kTile + lane;
-                wx[j] = ds.ux[u] - q.qx;
-                wy[j] = ds.uy[u] - q.qy;
+                px[j] = ds.ux[u];
+                py[j] = ds.uy[u];
+                wx[j] = px[j] - q.qx;
+                wy[j] = py[j] - q.qy;
+                dq[j] = d2_exact(px[j], py[j], q.qx, q.qy);
                 if (j) {
                     const float4 b = ds.mbr[0][t0 + j];
                     m.x = fminf(m.x, b.x);
@@ -210,7 +238,10 @@ __global__ __launch_bounds__(kBlock) void k_count_all(DsDev ds, QDev q, const Ca
             const bool slow = have && !clean;
             const uint64_t mc = __ballot(clean), ms = __ballot(slow);
             if (clean) wlds[rank_below(mc)] = r;
-            if (slow) wlds[64 + rank_below(ms)] = r;
+            if (slow) {
+                wlds[64 + rank_below(ms)] = r;
+                worig[rank_below(ms)] = orig[jc];
+            }
             __builtin_amdgcn_wave_barrier();
             const uint32_t n_clean = (uint32_t)__popcll(mc), n_slow = (uint32_t)__popcll(ms);
 #pragma unroll 4
@@ -226,7 +257,13 @@ __global__ __launch_bounds__(kBlock) void k_count_all(DsDev ds, QDev q, const Ca
                 for (int j = 0; j < kCountUsers; ++j) {
                     const float t = fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh));
                     cnt[j] += (t >= kCertainHi) ? 1u : 0u;
-                    band[j] |= (t > kCertainLo) && (t < kCertainHi);
+                    const bool inband = (t > kCertainLo) && (t < kCertainHi);
+                    const uint64_t bm = __ballot(inband);
+                    if (bm) { // rare: settle the band in binary64
+                        const float2 f = worig[i];
+                        if (inband && d2_exact(px[j], py[j], f.x, f.y) < dq[j]) ++cnt[j];
+                        rech += (uint32_t)__popcll(bm);
+                    }
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -234,14 +271,10 @@ __global__ __launch_bounds__(kBlock) void k_count_all(DsDev ds, QDev q, const Ca
 #pragma unroll
         for (int j = 0; j < kCountUsers; ++j) {
             if ((uint32_t)j >= nt) break;
-            const uint32_t u = (t0 + j) * kTile + lane;
-            uint32_t c = (uint32_t)acc[j] + cnt[j];
-            if (band[j]) c = exact_count(ds.ux[u], ds.uy[u], q.qx, q.qy, orig, q.nc, 0xffffffffu);
-            rech += (uint32_t)__popcll(__ballot(band[j] && u < ds.nu));
-            counts[u] = c;
+            counts[(t0 + j) * kTile + lane] = (uint32_t)acc[j] + cnt[j];
         }
     }
-    if (lane == 0 && rech) atomicAdd(rech_out, rech);
+    if (lane == 0 && rech) atomicAdd(&rech_out[(blockIdx.x * kWaves + wave) % kBuckets], rech);
 }
 
 __global__ void k_scatter_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm, uint32_t n,
@@ -253,24 +286,38 @@ __global__ void k_scatter_u32(const uint32_t* __restrict__ src, const uint32_t* 
 
 // ------------------------------------------------------ dataset construction
 
-__device__ __forceinline__ uint32_t spread16(uint32_t v)
+// Hilbert index of a 16-bit x 16-bit cell (the iterative quadrant walk of
+// Hilbert's curve).  Consecutive keys are adjacent cells, so 64 consecutive
+// users form a compact tile; a Z-order (Morton) key jumps between quadrants
+// and gives some tiles rectangles as wide as the whole map.
+__device__ __forceinline__ uint32_t hilbert16(uint32_t x, uint32_t y)
 {
-    v &= 0xffffu;
-    v = (v | (v << 8)) & 0x00ff00ffu;
-    v = (v | (v << 4)) & 0x0f0f0f0fu;
-    v = (v | (v << 2)) & 0x33333333u;
-    v = (v | (v << 1)) & 0x55555555u;
-    return v;
+    uint32_t d = 0;
+    for (uint32_t s = 1u << 15; s > 0; s >>= 1) {
+        const uint32_t rx = (x & s) ? 1u : 0u;
+        const uint32_t ry = (y & s) ? 1u : 0u;
+        d += s * s * ((3u * rx) ^ ry);
+        if (ry == 0) { // rotate the quadrant
+            if (rx == 1) {
+                x = 0xffffu - x;
+                y = 0xffffu - y;
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
 }
 
-__global__ void k_morton(const float* __restrict__ ux, const float* __restrict__ uy, uint32_t n, float minx,
-                         float miny, float sx, float sy, uint32_t* __restrict__ keys, uint32_t* __restrict__ idx)
+__global__ void k_curve_keys(const float* __restrict__ ux, const float* __restrict__ uy, uint32_t n, float minx,
+                             float miny, float sx, float sy, uint32_t* __restrict__ keys, uint32_t* __restrict__ idx)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float fx = (ux[i] - minx) * sx, fy = (uy[i] - miny) * sy;
-        uint32_t ix = (uint32_t)fminf(fmaxf(fx, 0.f), 65535.f);
-        uint32_t iy = (uint32_t)fminf(fmaxf(fy, 0.f), 65535.f);
-        keys[i] = spread16(ix) | (spread16(iy) << 1);
+        const float fx = (ux[i] - minx) * sx, fy = (uy[i] - miny) * sy;
+        const uint32_t ix = (uint32_t)fminf(fmaxf(fx, 0.f), 65535.f);
+        const uint32_t iy = (uint32_t)fminf(fmaxf(fy, 0.f), 65535.f);
+        keys[i] = hilbert16(ix, iy);
         idx[i] = i;
     }
 }
@@ -349,11 +396,12 @@ __global__ void k_batch_init(BatchDev b, uint32_t root_level)
 {
     __shared__ uint32_t s_n;
     if (threadIdx.x == 0) {
-        for (int l = 0; l <= kMaxLevels; ++l) b.ctr->items[l] = 0;
+        for (int l = 0; l <= kMaxLevels; ++l)
+            for (int j = 0; j < kBuckets; ++j) b.ctr->items[l][j] = 0;
         b.ctr->overflow = 0;
-        b.ctr->pool = 0;
+        b.ctr->results = 0;
         b.ctr->sort_overflow = 0;
-        b.ctr->rechecks = 0;
+        for (int j = 0; j < kBuckets; ++j) b.ctr->rechecks[j] = 0;
         s_n = 0;
     }
     __syncthreads();
@@ -365,31 +413,91 @@ __global__ void k_batch_init(BatchDev b, uint32_t root_level)
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) b.ctr->items[root_level] = s_n;
+    if (threadIdx.x == 0) b.ctr->items[root_level][0] = s_n; // every root item sits in bucket 0
 }
 
 // Items at parent level p are (query, node); the node's <= 64 children at
 // level p-1 go on lanes.  A child is culled once k candidates certainly cover
 // its whole rectangle (the internal-node test of rknn-inf/inf.h:219-237,
-// with certified corner evaluation instead of a 4-corner FP64 test);
-// survivors are pushed to level p-1.
+// with certified corner evaluation instead of a 4-corner FP64 test).
+// Survivors are staged in a wave-private LDS buffer and appended to level
+// p-1 with one atomic per flush instead of one per item.
+constexpr uint32_t kStage = 192;
+
+// Per-wave view of a bucketed work list: lane b holds bucket b's exclusive
+// and inclusive prefix of the append counts (clamped to the bucket size).
+struct ListView {
+    const uint64_t* base;
+    uint32_t bcap;
+    uint32_t excl, incl, total;
+};
+
+__device__ ListView list_view(const uint64_t* base, const uint32_t* counts, uint32_t bcap)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t c = min(counts[lane], bcap);
+    uint32_t incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
+        if ((int)lane >= o) incl += v;
+    }
+    ListView v{base, bcap, incl - c, incl, 0};
+    v.total = uniform((uint32_t)__builtin_amdgcn_readlane((int)incl, 63));
+    return v;
+}
+
+// Item i of the list (all lanes call it with the same i < total).
+__device__ uint64_t list_item(const ListView& v, uint32_t i)
+{
+    const uint32_t bucket = uniform((uint32_t)__popcll(__ballot(v.incl <= i)));
+    const uint32_t start = (uint32_t)__builtin_amdgcn_readlane((int)v.excl, (int)bucket);
+    return uniform64(v.base[(uint64_t)bucket * v.bcap + (i - start)]);
+}
+
+__device__ void flush_items(uint64_t* stage, uint32_t& n, uint32_t* counters, uint64_t* next, uint32_t bcap,
+                            uint32_t* overflow)
+{
+    if (n == 0) return;
+    const uint32_t lane = lane_id();
+    const uint32_t bucket = (blockIdx.x * kWaves + (threadIdx.x >> 6)) % kBuckets;
+    uint32_t pos = 0;
+    if (lane == 0) pos = atomicAdd(&counters[bucket], n);
+    pos = uniform(pos);
+    __builtin_amdgcn_wave_barrier();
+    uint64_t* dst = next + (uint64_t)bucket * bcap;
+    for (uint32_t i = lane; i < n; i += 64) {
+        if (pos + i < bcap)
+            dst[pos + i] = stage[i];
+        else
+            *overflow = 1;
+    }
+    __builtin_amdgcn_wave_barrier();
+    n = 0;
+}
+
 __global__ __launch_bounds__(kBlock) void k_batch_internal(DsDev ds, BatchDev b, uint32_t p)
 {
     __shared__ CandRec s_rec[kWaves][64];
+    __shared__ uint64_t s_out[kWaves][kStage];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
     const uint32_t stride = gridDim.x * kWaves;
-    const uint32_t n_items = min(b.ctr->items[p], b.item_cap[p]);
+    const ListView lv = list_view(b.items[p], b.ctr->items[p], b.item_cap[p]);
+    const uint32_t n_items = lv.total;
     const uint32_t cl = p - 1;
     const uint32_t n_child = ds.nlev[cl];
     const float4* __restrict__ cm = ds.mbr[cl];
     uint64_t* __restrict__ next = b.items[cl];
     const uint32_t next_cap = b.item_cap[cl];
     CandRec* wlds = s_rec[wave];
-    for (uint32_t it = blockIdx.x * kWaves + wave; it < n_items; it += stride) {
-        const uint64_t item = b.items[p][it];
-        const uint32_t qi = (uint32_t)(item >> 32), node = (uint32_t)item;
-        const QDev Q = b.q[qi];
+    uint64_t* stage = s_out[wave];
+    uint32_t staged = 0; // wave-uniform
+    uint32_t it = blockIdx.x * kWaves + wave;
+    uint64_t item = it < n_items ? list_item(lv, it) : 0;
+    for (; it < n_items; it += stride) {
+        const uint64_t nxt = it + stride < n_items ? list_item(lv, it + stride) : 0; // prefetch
+        const uint32_t qi = uniform((uint32_t)(item >> 32)), node = uniform((uint32_t)item);
+        const QDev Q = uniform_q(b.q[qi]);
         const CandRec* __restrict__ rec = b.rec + Q.off;
         const uint32_t c = node * kFanout + lane;
         const bool valid = c < n_child;
@@ -414,60 +522,56 @@ __global__ __launch_bounds__(kBlock) void k_batch_internal(DsDev ds, BatchDev b,
             if (!live) break;
         }
         const uint64_t surv = __ballot(valid && full < Q.k);
-        if (surv) {
-            uint32_t pos = 0;
-            if (lane == 0) pos = atomicAdd(&b.ctr->items[cl], (uint32_t)__popcll(surv));
-            pos = wave_bcast(pos, 0);
-            if ((surv >> lane) & 1ull) {
-                const uint32_t dst = pos + rank_below(surv);
-                if (dst < next_cap)
-                    next[dst] = ((uint64_t)qi << 32) | c;
+        const uint32_t ns = (uint32_t)__popcll(surv);
+        if (ns) {
+            if (staged + ns > kStage) flush_items(stage, staged, b.ctr->items[cl], next, next_cap, &b.ctr->overflow);
+            if ((surv >> lane) & 1ull) stage[staged + rank_below(surv)] = ((uint64_t)qi << 32) | c;
+            staged += ns;
+            __builtin_amdgcn_wave_barrier();
+        }
+        item = nxt;
+    }
+    flush_items(stage, staged, b.ctr->items[cl], next, next_cap, &b.ctr->overflow);
+}
+
+// Leaf items (query, tile): the plain form's tile routine; survivors go to
+// the query's own result segment (one atomic per item with survivors).
+__global__ __launch_bounds__(kBlock) void k_batch_leaf(DsDev ds, BatchDev b)
+{
+    __shared__ WaveLds s_w[kWaves];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = lane_id();
+    const uint32_t stride = gridDim.x * kWaves;
+    const ListView lv = list_view(b.items[0], b.ctr->items[0], b.item_cap[0]);
+    const uint32_t n_items = lv.total;
+    uint32_t rech = 0;
+    uint32_t it = blockIdx.x * kWaves + wave;
+    uint64_t item = it < n_items ? list_item(lv, it) : 0;
+    for (; it < n_items; it += stride) {
+        const uint64_t nxt = it + stride < n_items ? list_item(lv, it + stride) : 0; // prefetch
+        const uint32_t qi = uniform((uint32_t)(item >> 32)), tile = uniform((uint32_t)item);
+        const QDev Q = uniform_q(b.q[qi]);
+        const LeafOut r = leaf_tile(ds, Q.qx, Q.qy, Q.k, Q.nc, b.rec + Q.off, b.orig + Q.off, tile, &s_w[wave]);
+        rech += r.fp64;
+        if (r.keep) {
+            uint32_t rank0 = 0;
+            if (lane == 0) rank0 = atomicAdd(&b.qcount[qi], (uint32_t)__popcll(r.keep));
+            rank0 = uniform(rank0);
+            if ((r.keep >> lane) & 1ull) {
+                const uint32_t rk = rank0 + rank_below(r.keep);
+                if (rk < b.seg_cap)
+                    b.seg[(uint64_t)qi * b.seg_cap + rk] = ds.uid[tile * kTile + lane];
                 else
                     b.ctr->overflow = 1;
             }
         }
+        item = nxt;
     }
+    if (lane == 0 && rech) atomicAdd(&b.ctr->rechecks[(blockIdx.x * kWaves + wave) % kBuckets], rech);
 }
 
-__global__ __launch_bounds__(kBlock) void k_batch_leaf(DsDev ds, BatchDev b)
-{
-    __shared__ CandRec s_rec[kWaves][64];
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t lane = lane_id();
-    const uint32_t stride = gridDim.x * kWaves;
-    const uint32_t n_items = min(b.ctr->items[0], b.item_cap[0]);
-    uint32_t rech = 0;
-    for (uint32_t it = blockIdx.x * kWaves + wave; it < n_items; it += stride) {
-        const uint64_t item = b.items[0][it];
-        const uint32_t qi = (uint32_t)(item >> 32), tile = (uint32_t)item;
-        const QDev Q = b.q[qi];
-        const LeafOut r = leaf_tile(ds, Q.qx, Q.qy, Q.k, Q.nc, b.rec + Q.off, b.orig + Q.off, tile, s_rec[wave]);
-        rech += r.rechecks;
-        if (r.keep) {
-            const uint32_t n = (uint32_t)__popcll(r.keep);
-            uint32_t rank0 = 0, pos = 0;
-            if (lane == 0) {
-                rank0 = atomicAdd(&b.qcount[qi], n);
-                pos = atomicAdd(&b.ctr->pool, n);
-            }
-            rank0 = wave_bcast(rank0, 0);
-            pos = wave_bcast(pos, 0);
-            if ((r.keep >> lane) & 1ull) {
-                const uint32_t rk = rank_below(r.keep);
-                const uint32_t dst = pos + rk;
-                if (dst < b.pool_cap) {
-                    b.pool_uid[dst] = ds.uid[tile * kTile + lane];
-                    b.pool_tag[dst] = ((uint64_t)qi << 32) | (rank0 + rk);
-                } else {
-                    b.ctr->overflow = 1;
-                }
-            }
-        }
-    }
-    if (lane == 0 && rech) atomicAdd(&b.ctr->rechecks, rech);
-}
-
-// Exclusive scan of per-query result counts (one block).
+// Exclusive scan of per-query result counts (one block) for the compacted
+// layout; also the batch total.
 __global__ __launch_bounds__(1024) void k_batch_scan(BatchDev b)
 {
     __shared__ uint64_t s_part[1024];
@@ -475,7 +579,7 @@ __global__ __launch_bounds__(1024) void k_batch_scan(BatchDev b)
     const uint32_t per = (b.nq + 1023) / 1024;
     const uint32_t lo = min(b.nq, t * per), hi = min(b.nq, lo + per);
     uint64_t sum = 0;
-    for (uint32_t i = lo; i < hi; ++i) sum += b.qcount[i];
+    for (uint32_t i = lo; i < hi; ++i) sum += min(b.qcount[i], b.seg_cap);
     s_part[t] = sum;
     __syncthreads();
     for (uint32_t o = 1; o < 1024; o <<= 1) {
@@ -487,36 +591,29 @@ __global__ __launch_bounds__(1024) void k_batch_scan(BatchDev b)
     uint64_t run = s_part[t] - sum;
     for (uint32_t i = lo; i < hi; ++i) {
         b.qoff[i] = run;
-        run += b.qcount[i];
+        run += min(b.qcount[i], b.seg_cap);
     }
-    if (t == 1023) b.qoff[b.nq] = s_part[1023];
-}
-
-__global__ void k_batch_scatter(BatchDev b)
-{
-    const uint32_t n = min(b.ctr->pool, b.pool_cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint64_t tag = b.pool_tag[i];
-        const uint64_t dst = b.qoff[tag >> 32] + (uint32_t)tag;
-        if (dst < b.pool_cap) b.out[dst] = b.pool_uid[i]; // an overflowed batch is discarded by the host
+    if (t == 1023) {
+        b.qoff[b.nq] = s_part[1023];
+        b.ctr->results = (uint32_t)min<uint64_t>(s_part[1023], 0xffffffffull);
     }
 }
 
-// One block per query: bitonic sort of its ids in LDS.  Larger results are
-// flagged and sorted by the host when fetched.
+// One block per query: bitonic sort of its segment in LDS.  Larger results
+// are flagged and sorted by the host when fetched.
 __global__ __launch_bounds__(1024) void k_batch_sort(BatchDev b)
 {
     __shared__ int32_t s[kSortCap];
     for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
-        const uint32_t n = b.qcount[qi];
-        if (n <= 1 || b.qoff[qi] + n > b.pool_cap) continue; // overflowed batches are discarded
+        const uint32_t n = min(b.qcount[qi], b.seg_cap);
+        if (n <= 1) continue;
         if (n > kSortCap) {
             if (threadIdx.x == 0) atomicAdd(&b.ctr->sort_overflow, 1u);
             continue;
         }
         uint32_t np = 1;
         while (np < n) np <<= 1;
-        int32_t* seg = b.out + b.qoff[qi];
+        int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
         for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) s[i] = i < n ? seg[i] : INT32_MAX;
         __syncthreads();
         for (uint32_t size = 2; size <= np; size <<= 1) {
@@ -525,10 +622,10 @@ __global__ __launch_bounds__(1024) void k_batch_sort(BatchDev b)
                     const uint32_t lo = 2 * i - (i & (stride - 1));
                     const uint32_t hi = lo + stride;
                     const bool up = (lo & size) == 0;
-                    const int32_t a = s[lo], c = s[hi];
-                    if ((a > c) == up) {
-                        s[lo] = c;
-                        s[hi] = a;
+                    const int32_t x = s[lo], y = s[hi];
+                    if ((x > y) == up) {
+                        s[lo] = y;
+                        s[hi] = x;
                     }
                 }
                 __syncthreads();
@@ -536,6 +633,17 @@ __global__ __launch_bounds__(1024) void k_batch_sort(BatchDev b)
         }
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) seg[i] = s[i];
         __syncthreads();
+    }
+}
+
+// Fetch-time: copy each query's segment to its place in the compacted array.
+__global__ void k_batch_compact(BatchDev b)
+{
+    for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
+        const uint32_t n = min(b.qcount[qi], b.seg_cap);
+        const int32_t* src = b.seg + (uint64_t)qi * b.seg_cap;
+        int32_t* dst = b.out + b.qoff[qi];
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
     }
 }
 
@@ -551,10 +659,10 @@ int grid_for(uint64_t n, int per_block, int max_blocks)
 
 int device_grid(int n_cu, int waves_per_cu) { return n_cu * waves_per_cu / kWaves; }
 
-void launch_morton_keys(const float* ux, const float* uy, uint32_t n, float minx, float miny, float sx, float sy,
-                        uint32_t* keys, uint32_t* idx, hipStream_t st)
+void launch_curve_keys(const float* ux, const float* uy, uint32_t n, float minx, float miny, float sx, float sy,
+                       uint32_t* keys, uint32_t* idx, hipStream_t st)
 {
-    k_morton<<<grid_for(n, 256, 4096), 256, 0, st>>>(ux, uy, n, minx, miny, sx, sy, keys, idx);
+    k_curve_keys<<<grid_for(n, 256, 4096), 256, 0, st>>>(ux, uy, n, minx, miny, sx, sy, keys, idx);
 }
 
 void launch_gather_users(const uint32_t* order, uint32_t n, uint32_t n_pad, const float* ux_in, const float* uy_in,
@@ -613,11 +721,15 @@ void launch_batch_leaf(const DsDev& ds, const BatchDev& b, int grid, hipStream_t
     k_batch_leaf<<<grid, kBlock, 0, st>>>(ds, b);
 }
 
-void launch_batch_finish(const BatchDev& b, int grid, hipStream_t st)
+void launch_batch_finish(const BatchDev& b, hipStream_t st)
 {
-    k_batch_scan<<<1, 1024, 0, st>>>(b);
-    k_batch_scatter<<<grid, 256, 0, st>>>(b);
     k_batch_sort<<<b.nq < 65535 ? (b.nq ? b.nq : 1) : 65535, 1024, 0, st>>>(b);
+    k_batch_scan<<<1, 1024, 0, st>>>(b);
+}
+
+void launch_batch_compact(const BatchDev& b, hipStream_t st)
+{
+    k_batch_compact<<<b.nq < 65535 ? (b.nq ? b.nq : 1) : 65535, 256, 0, st>>>(b);
 }
 
 } // namespace rk

@@ -201,6 +201,14 @@ class Batch:
             break
         return [out[int(offs[i]): int(offs[i + 1])].copy() for i in range(self.nq)]
 
+    def stats(self):
+        """(items per tree level, leaf first; results; binary64 re-checks) of the last run."""
+        items = np.zeros(16, np.uint64)
+        nl, res, rech = C.c_size_t(), C.c_uint64(), C.c_uint64()
+        self.ds.ctx.check(self._lib.rk_batch_stats(self.h, _ptr(items, C.c_uint64), 16, C.byref(nl), C.byref(res),
+                                                   C.byref(rech)))
+        return items[: nl.value].tolist(), res.value, rech.value
+
     def close(self):
         if self.h:
             self._lib.rk_batch_destroy(self.h)
