@@ -1,16 +1,18 @@
 // rk_batch.hip -- batched form: many queries with uneven candidate counts and
 // mixed k in one hipGraph.
 //
-// The graph is a fixed launch sequence over a device-side work list:
-//   prep    candidate records for every (query, candidate), binary64 -> fp32
-//   init    zero the counters, one root item per non-trivial query
-//   level p = top..1
+// The graph is a fixed launch sequence over device-side work lists:
+//   prep    zero the counters; candidate records for every (query, candidate),
+//           binary64 -> fp32
+//   root    one wave per query tests the <= 64 top tree nodes and lists
+//           the survivors
+//   level p = top .. 1
 //           items (query, node at level p); the node's 64 children on lanes;
-//           children not certainly covered by k candidates are pushed down
-//   leaf    items (query, tile); per-user count as in the plain form
-//   finish  per-query LDS bitonic sort of each result segment, offsets scan
-// Leaf survivors go straight into per-query result segments (one atomic per
-// tile with survivors); the fetch compacts the segments.
+//           children not certainly covered by k candidates are listed below
+//   leaf    items (query, tile); per-user count as in the plain form;
+//           survivors go straight into per-query result segments
+//   sort    per-query block radix sort of each result segment
+// The fetch compacts the segments (offsets scan + copy) outside the graph.
 // Every kernel is a grid-stride loop over a device-side item count, so the
 // graph is captured once and replayed unchanged for any data; nothing waits
 // on another wave, so every launch drains.
@@ -36,16 +38,15 @@ rk_status build_graph(rk_batch* b)
     rk_dataset* ds = b->ds;
     rk_ctx* c = ds->ctx;
     hipStream_t st = c->stream;
-    const uint32_t root = ds->view.n_levels; // parent level whose single node holds the top level
     const int grid = device_grid(c->n_cu, kWavesPerCU);
     RK_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    if (b->total_cand)
-        launch_prep_records(b->q.p, b->nq, b->cand.p, ds->fx.p, ds->fy.p, b->wb.p, b->rec.p, b->orig.p, b->total_cand,
-                            st);
-    launch_batch_init(b->view, root, st);
-    for (uint32_t p = root; p >= 1; --p) launch_batch_internal(ds->view, b->view, p, grid, st);
+    // the root kernel lists each query's surviving top nodes; the level
+    // kernels then descend level by level down to the tiles
+    launch_batch_prep(b->view, b->cand.p, ds->fx.p, ds->fy.p, b->wb.p, b->total_cand, st);
+    launch_batch_root(ds->view, b->view, grid, st);
+    for (uint32_t p = ds->view.n_levels - 1; p >= 1; --p) launch_batch_internal(ds->view, b->view, p, grid, st);
     launch_batch_leaf(ds->view, b->view, grid, st);
-    launch_batch_finish(b->view, st);
+    launch_batch_sort(b->view, st);
     hipError_t e = hipStreamEndCapture(st, &b->graph);
     if (e != hipSuccess) return fail_hip(c, e, "hipStreamEndCapture", __LINE__);
     RK_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -104,7 +105,6 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
     hipStream_t st = c->stream;
-    const uint32_t root = ds->view.n_levels;
     RK_HIP(c, b->q.reserve(nq));
     RK_HIP(c, b->wb.reserve(nq));
     RK_HIP(c, b->cand.reserve(all.size()));
@@ -114,12 +114,11 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
     RK_HIP(c, b->qoff.reserve(nq + 1));
     RK_HIP(c, b->ctr.reserve(1));
     RK_HIP(c, b->h_ctr.reserve(1));
-    // item lists: parent level p holds nodes of level p (p < root) or the root
-    // (the root level keeps all nq root items in bucket 0)
-    for (uint32_t p = 0; p <= root; ++p) {
-        uint64_t cap = (p == root) ? (uint64_t)nq : (uint64_t)nq * ds->view.nlev[p];
+    // work lists of tree levels 0 .. top
+    for (uint32_t p = 0; p < ds->view.n_levels; ++p) {
+        uint64_t cap = (uint64_t)nq * ds->view.nlev[p];
         cap = std::max<uint64_t>(1, std::min(cap, kItemBudget));
-        uint64_t bcap = (p == root) ? std::max<uint64_t>(1, nq) : (cap + kBuckets - 1) / kBuckets;
+        uint64_t bcap = (cap + kBuckets - 1) / kBuckets;
         // a bucket also takes one wave's whole staged flush
         bcap = std::max<uint64_t>(bcap, 256);
         RK_HIP(c, b->items[p].reserve(bcap * kBuckets));
@@ -193,6 +192,8 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
     const uint32_t nq = b->nq;
     std::vector<uint64_t> qoff(nq + 1, 0);
     if (b->exec) {
+        launch_batch_scan(b->view, c->stream);
+        RK_HIP(c, hipGetLastError());
         RK_HIP(c, hipMemcpy(qoff.data(), b->qoff.p, (nq + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
     }
     uint64_t total = 0;
@@ -219,7 +220,7 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
             if (n) std::memcpy(dst, ds->all_ids_sorted.data(), n * sizeof(int32_t));
         } else if (n) {
             std::memcpy(dst, dev.data() + qoff[i], n * sizeof(int32_t));
-            if (n > 8192) std::sort(dst, dst + n); // larger than the device LDS sort
+            if (n > 4096) std::sort(dst, dst + n); // larger than the device block sort
         }
     }
     return RK_OK;
@@ -230,7 +231,7 @@ rk_status rk_batch_stats(const rk_batch* b, uint64_t* items, size_t n_items_cap,
 {
     if (!b || !b->ran) return RK_ERR_INVALID_ARG;
     const BatchCounters& c = *b->h_ctr.p;
-    const size_t nl = b->ds->view.n_levels + 1;
+    const size_t nl = b->ds->view.n_levels;
     if (n_levels) *n_levels = nl;
     for (size_t l = 0; l < nl && l < n_items_cap && items; ++l) {
         uint64_t t = 0;

@@ -11,6 +11,7 @@
 // findKNearestSimpleKernel (rknn-naive-gpu/rknn_kernel.cu:119-169), and the
 // rknn-rt raygen/any-hit pair (rknn-rt/rt.cu:46-81,95-111).
 #include <hip/hip_runtime.h>
+#include <hipcub/block/block_radix_sort.hpp>
 #include <hipcub/hipcub.hpp>
 
 #include "rk_common.hpp"
@@ -22,7 +23,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kSortCap = 8192; // result ids sorted in LDS per query
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -372,48 +372,28 @@ __global__ __launch_bounds__(kBlock) void k_level_mbr(const float4* __restrict__
 
 // -------------------------------------------------------------- batched form
 
-__global__ void k_prep_records(const QDev* __restrict__ qs, uint32_t nq, const uint32_t* __restrict__ cand,
-                               const float* __restrict__ fx, const float* __restrict__ fy,
-                               const double2* __restrict__ wb, CandRec* __restrict__ rec, float2* __restrict__ orig,
-                               uint64_t total)
+// First kernel of a batch step: zero the counters, clear each query's
+// result count and build its candidate records (one block per query).
+__global__ void k_batch_prep(BatchDev b, const uint32_t* __restrict__ cand, const float* __restrict__ fx,
+                             const float* __restrict__ fy, const double2* __restrict__ wb, uint64_t total)
 {
-    // one block per query, threads over its candidates
-    for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
-        const QDev Q = qs[qi];
+    if (blockIdx.x == 0) {
+        uint32_t* c = reinterpret_cast<uint32_t*>(b.ctr);
+        for (uint32_t i = threadIdx.x; i < sizeof(BatchCounters) / 4; i += blockDim.x) c[i] = 0;
+    }
+    for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
+        const QDev Q = b.q[qi];
+        if (threadIdx.x == 0) b.qcount[qi] = 0;
         const double2 w = wb[qi];
         for (uint32_t j = threadIdx.x; j < Q.nc; j += blockDim.x) {
             const uint64_t g = Q.off + j;
             if (g >= total) break;
             const uint32_t f = cand[g];
             const float x = fx[f], y = fy[f];
-            rec[g] = make_record(Q.qx, Q.qy, x, y, w.x, w.y);
-            orig[g] = make_float2(x, y);
+            const_cast<CandRec*>(b.rec)[g] = make_record(Q.qx, Q.qy, x, y, w.x, w.y);
+            const_cast<float2*>(b.orig)[g] = make_float2(x, y);
         }
     }
-}
-
-__global__ void k_batch_init(BatchDev b, uint32_t root_level)
-{
-    __shared__ uint32_t s_n;
-    if (threadIdx.x == 0) {
-        for (int l = 0; l <= kMaxLevels; ++l)
-            for (int j = 0; j < kBuckets; ++j) b.ctr->items[l][j] = 0;
-        b.ctr->overflow = 0;
-        b.ctr->results = 0;
-        b.ctr->sort_overflow = 0;
-        for (int j = 0; j < kBuckets; ++j) b.ctr->rechecks[j] = 0;
-        s_n = 0;
-    }
-    __syncthreads();
-    for (uint32_t qi = threadIdx.x; qi < b.nq; qi += blockDim.x) {
-        b.qcount[qi] = 0;
-        if (!b.q[qi].trivial) {
-            const uint32_t pos = atomicAdd(&s_n, 1u);
-            b.items[root_level][pos] = ((uint64_t)qi << 32);
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) b.ctr->items[root_level][0] = s_n; // every root item sits in bucket 0
 }
 
 // Items at parent level p are (query, node); the node's <= 64 children at
@@ -475,18 +455,105 @@ __device__ void flush_items(uint64_t* stage, uint32_t& n, uint32_t* counters, ui
     n = 0;
 }
 
+// Lanes = the <= 64 children of `node` at a level with MBRs `cm`: the mask of
+// children that fewer than k candidates certainly cover.  Candidate records
+// are staged 64 at a time in wlds; `staged` says chunk 0 is already there.
+__device__ uint64_t surviving_children(const float4* __restrict__ cm, uint32_t n_child, uint32_t node,
+                                       const QDev& Q, const CandRec* __restrict__ rec, CandRec* wlds, bool staged)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t c = node * kFanout + lane;
+    const bool valid = c < n_child;
+    const float4 m = cm[valid ? c : 0];
+    const float lx = m.x - Q.qx, ly = m.y - Q.qy, hx = m.z - Q.qx, hy = m.w - Q.qy;
+    uint32_t full = 0;
+    for (uint32_t c0 = 0; c0 < Q.nc; c0 += 64) {
+        const uint32_t n = min(64u, Q.nc - c0);
+        if (c0 > 0 || !staged) {
+            __builtin_amdgcn_wave_barrier();
+            if (lane < n) wlds[lane] = rec[c0 + lane];
+            __builtin_amdgcn_wave_barrier();
+        }
+        bool live = true;
+        for (uint32_t i0 = 0; i0 < n && live; i0 += 16) {
+            if (i0 + 16 <= n) { // full block: unrolled so the LDS reads pipeline
+#pragma unroll
+                for (uint32_t i = 0; i < 16; ++i) {
+                    const CandRec r = wlds[i0 + i];
+                    const float t = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
+                    full += (t >= kCertainHi) ? 1u : 0u;
+                }
+            } else {
+                for (uint32_t i = i0; i < n; ++i) {
+                    const CandRec r = wlds[i];
+                    const float t = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
+                    full += (t >= kCertainHi) ? 1u : 0u;
+                }
+            }
+            live = __ballot(valid && full < Q.k) != 0;
+        }
+        if (!live) return 0;
+    }
+    return __ballot(valid && full < Q.k);
+}
+
+__device__ __forceinline__ void stage_chunk0(const QDev& Q, const CandRec* __restrict__ rec, CandRec* wlds)
+{
+    const uint32_t lane = lane_id();
+    __builtin_amdgcn_wave_barrier();
+    if (lane < min(64u, Q.nc)) wlds[lane] = rec[lane];
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void push_children(uint64_t surv, uint32_t qi, uint32_t node, uint64_t* stage,
+                                              uint32_t& staged, uint32_t* counters, uint64_t* next, uint32_t bcap,
+                                              uint32_t* overflow)
+{
+    const uint32_t ns = (uint32_t)__popcll(surv);
+    if (!ns) return;
+    if (staged + ns > kStage) flush_items(stage, staged, counters, next, bcap, overflow);
+    const uint32_t lane = lane_id();
+    if ((surv >> lane) & 1ull) stage[staged + rank_below(surv)] = ((uint64_t)qi << 32) | (node * kFanout + lane);
+    staged += ns;
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Root of every query: one wave per query tests the <= 64 top tree nodes
+// and lists the survivors for the level kernels.
+__global__ __launch_bounds__(kBlock) void k_batch_root(DsDev ds, BatchDev b)
+{
+    __shared__ CandRec s_rec[kWaves][64];
+    __shared__ uint64_t s_out[kWaves][kStage];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t stride = gridDim.x * kWaves;
+    const uint32_t top = ds.n_levels - 1; // level of the <= 64 top nodes
+    uint64_t* __restrict__ next = b.items[top];
+    const uint32_t bcap = b.item_cap[top];
+    CandRec* wlds = s_rec[wave];
+    uint64_t* stage = s_out[wave];
+    uint32_t staged = 0;
+    for (uint32_t qi = blockIdx.x * kWaves + wave; qi < b.nq; qi += stride) {
+        const QDev Q = uniform_q(b.q[qi]);
+        if (Q.trivial) continue;
+        const CandRec* __restrict__ rec = b.rec + Q.off;
+        stage_chunk0(Q, rec, wlds);
+        const uint64_t surv = surviving_children(ds.mbr[top], ds.nlev[top], 0, Q, rec, wlds, true);
+        push_children(surv, qi, 0, stage, staged, b.ctr->items[top], next, bcap, &b.ctr->overflow);
+    }
+    flush_items(stage, staged, b.ctr->items[top], next, bcap, &b.ctr->overflow);
+}
+
+// Items at parent level p: (query, node at level p); survivors among the
+// node's children are pushed to level p-1.
 __global__ __launch_bounds__(kBlock) void k_batch_internal(DsDev ds, BatchDev b, uint32_t p)
 {
     __shared__ CandRec s_rec[kWaves][64];
     __shared__ uint64_t s_out[kWaves][kStage];
     const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t lane = lane_id();
     const uint32_t stride = gridDim.x * kWaves;
     const ListView lv = list_view(b.items[p], b.ctr->items[p], b.item_cap[p]);
     const uint32_t n_items = lv.total;
     const uint32_t cl = p - 1;
-    const uint32_t n_child = ds.nlev[cl];
-    const float4* __restrict__ cm = ds.mbr[cl];
     uint64_t* __restrict__ next = b.items[cl];
     const uint32_t next_cap = b.item_cap[cl];
     CandRec* wlds = s_rec[wave];
@@ -499,36 +566,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_internal(DsDev ds, BatchDev b,
         const uint32_t qi = uniform((uint32_t)(item >> 32)), node = uniform((uint32_t)item);
         const QDev Q = uniform_q(b.q[qi]);
         const CandRec* __restrict__ rec = b.rec + Q.off;
-        const uint32_t c = node * kFanout + lane;
-        const bool valid = c < n_child;
-        const float4 m = cm[valid ? c : 0];
-        const float lx = m.x - Q.qx, ly = m.y - Q.qy, hx = m.z - Q.qx, hy = m.w - Q.qy;
-        uint32_t full = 0;
-        for (uint32_t c0 = 0; c0 < Q.nc; c0 += 64) {
-            const uint32_t n = min(64u, Q.nc - c0);
-            if (lane < n) wlds[lane] = rec[c0 + lane];
-            __builtin_amdgcn_wave_barrier();
-            bool live = true;
-            for (uint32_t i0 = 0; i0 < n && live; i0 += 16) {
-                const uint32_t i1 = min(n, i0 + 16);
-                for (uint32_t i = i0; i < i1; ++i) {
-                    const CandRec r = wlds[i];
-                    const float t = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
-                    full += (t >= kCertainHi) ? 1u : 0u;
-                }
-                live = __ballot(valid && full < Q.k) != 0;
-            }
-            __builtin_amdgcn_wave_barrier();
-            if (!live) break;
-        }
-        const uint64_t surv = __ballot(valid && full < Q.k);
-        const uint32_t ns = (uint32_t)__popcll(surv);
-        if (ns) {
-            if (staged + ns > kStage) flush_items(stage, staged, b.ctr->items[cl], next, next_cap, &b.ctr->overflow);
-            if ((surv >> lane) & 1ull) stage[staged + rank_below(surv)] = ((uint64_t)qi << 32) | c;
-            staged += ns;
-            __builtin_amdgcn_wave_barrier();
-        }
+        const uint64_t surv = surviving_children(ds.mbr[cl], ds.nlev[cl], node, Q, rec, wlds, false);
+        push_children(surv, qi, node, stage, staged, b.ctr->items[cl], next, next_cap, &b.ctr->overflow);
         item = nxt;
     }
     flush_items(stage, staged, b.ctr->items[cl], next, next_cap, &b.ctr->overflow);
@@ -599,39 +638,43 @@ __global__ __launch_bounds__(1024) void k_batch_scan(BatchDev b)
     }
 }
 
-// One block per query: bitonic sort of its segment in LDS.  Larger results
-// are flagged and sorted by the host when fetched.
-__global__ __launch_bounds__(1024) void k_batch_sort(BatchDev b)
+// One block per query: radix sort of its result segment (rocPRIM block
+// radix sort via hipCUB, keys held in registers).  Segments of up to 2048
+// ids use 8 keys per thread, up to 4096 use 16; larger ones are flagged and
+// sorted by the host when fetched.
+using SortSmall = hipcub::BlockRadixSort<int32_t, 256, 8>;
+using SortLarge = hipcub::BlockRadixSort<int32_t, 256, 16>;
+
+template <class Sorter, int kItems>
+__device__ void sort_segment(int32_t* seg, uint32_t n, typename Sorter::TempStorage& tmp)
 {
-    __shared__ int32_t s[kSortCap];
+    int32_t keys[kItems];
+    const uint32_t base = threadIdx.x * kItems;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) keys[i] = base + i < n ? seg[base + i] : INT32_MAX;
+    Sorter(tmp).Sort(keys);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i)
+        if (base + i < n) seg[base + i] = keys[i];
+}
+
+__global__ __launch_bounds__(256) void k_batch_sort(BatchDev b)
+{
+    __shared__ union {
+        typename SortSmall::TempStorage small;
+        typename SortLarge::TempStorage large;
+    } tmp;
     for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
         const uint32_t n = min(b.qcount[qi], b.seg_cap);
-        if (n <= 1) continue;
-        if (n > kSortCap) {
-            if (threadIdx.x == 0) atomicAdd(&b.ctr->sort_overflow, 1u);
-            continue;
-        }
-        uint32_t np = 1;
-        while (np < n) np <<= 1;
         int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
-        for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) s[i] = i < n ? seg[i] : INT32_MAX;
-        __syncthreads();
-        for (uint32_t size = 2; size <= np; size <<= 1) {
-            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-                for (uint32_t i = threadIdx.x; i < np / 2; i += blockDim.x) {
-                    const uint32_t lo = 2 * i - (i & (stride - 1));
-                    const uint32_t hi = lo + stride;
-                    const bool up = (lo & size) == 0;
-                    const int32_t x = s[lo], y = s[hi];
-                    if ((x > y) == up) {
-                        s[lo] = y;
-                        s[hi] = x;
-                    }
-                }
-                __syncthreads();
-            }
+        if (n <= 1) continue;
+        if (n <= 256 * 8) {
+            sort_segment<SortSmall, 8>(seg, n, tmp.small);
+        } else if (n <= 256 * 16) {
+            sort_segment<SortLarge, 16>(seg, n, tmp.large);
+        } else if (threadIdx.x == 0) {
+            atomicAdd(&b.ctr->sort_overflow, 1u);
         }
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) seg[i] = s[i];
         __syncthreads();
     }
 }
@@ -700,15 +743,15 @@ void launch_scatter_u32(const uint32_t* src_sorted, const uint32_t* perm, uint32
     k_scatter_u32<<<grid_for(n, 256, 4096), 256, 0, st>>>(src_sorted, perm, n, dst_orig);
 }
 
-void launch_prep_records(const QDev* q, uint32_t nq, const uint32_t* cand, const float* fx, const float* fy,
-                         const double2* wbound, CandRec* rec, float2* orig, uint64_t total, hipStream_t st)
+void launch_batch_prep(const BatchDev& b, const uint32_t* cand, const float* fx, const float* fy,
+                       const double2* wbound, uint64_t total, hipStream_t st)
 {
-    k_prep_records<<<nq < 4096 ? (nq ? nq : 1) : 4096, 256, 0, st>>>(q, nq, cand, fx, fy, wbound, rec, orig, total);
+    k_batch_prep<<<b.nq < 4096 ? (b.nq ? b.nq : 1) : 4096, 256, 0, st>>>(b, cand, fx, fy, wbound, total);
 }
 
-void launch_batch_init(const BatchDev& b, uint32_t root_level, hipStream_t st)
+void launch_batch_root(const DsDev& ds, const BatchDev& b, int grid, hipStream_t st)
 {
-    k_batch_init<<<1, 1024, 0, st>>>(b, root_level);
+    k_batch_root<<<grid, kBlock, 0, st>>>(ds, b);
 }
 
 void launch_batch_internal(const DsDev& ds, const BatchDev& b, uint32_t parent_level, int grid, hipStream_t st)
@@ -721,9 +764,13 @@ void launch_batch_leaf(const DsDev& ds, const BatchDev& b, int grid, hipStream_t
     k_batch_leaf<<<grid, kBlock, 0, st>>>(ds, b);
 }
 
-void launch_batch_finish(const BatchDev& b, hipStream_t st)
+void launch_batch_sort(const BatchDev& b, hipStream_t st)
 {
-    k_batch_sort<<<b.nq < 65535 ? (b.nq ? b.nq : 1) : 65535, 1024, 0, st>>>(b);
+    k_batch_sort<<<b.nq < 65535 ? (b.nq ? b.nq : 1) : 65535, 256, 0, st>>>(b);
+}
+
+void launch_batch_scan(const BatchDev& b, hipStream_t st)
+{
     k_batch_scan<<<1, 1024, 0, st>>>(b);
 }
 

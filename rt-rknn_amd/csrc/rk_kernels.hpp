@@ -62,14 +62,15 @@ void launch_scatter_u32(const uint32_t* src_sorted, const uint32_t* perm, uint32
                         hipStream_t st);
 
 // batched form
-void launch_prep_records(const QDev* q, uint32_t nq, const uint32_t* cand, const float* fx, const float* fy,
-                         const double2* wbound, CandRec* rec, float2* orig, uint64_t total, hipStream_t st);
-void launch_batch_init(const BatchDev& b, uint32_t root_level, hipStream_t st);
+void launch_batch_prep(const BatchDev& b, const uint32_t* cand, const float* fx, const float* fy,
+                       const double2* wbound, uint64_t total, hipStream_t st);
+void launch_batch_root(const DsDev& ds, const BatchDev& b, int grid, hipStream_t st);
 void launch_batch_internal(const DsDev& ds, const BatchDev& b, uint32_t parent_level, int grid,
                            hipStream_t st);
 void launch_batch_leaf(const DsDev& ds, const BatchDev& b, int grid, hipStream_t st);
-void launch_batch_finish(const BatchDev& b, hipStream_t st);
-void launch_batch_compact(const BatchDev& b, hipStream_t st);
+void launch_batch_sort(const BatchDev& b, hipStream_t st);
+void launch_batch_scan(const BatchDev& b, hipStream_t st);    // offsets of the compacted layout
+void launch_batch_compact(const BatchDev& b, hipStream_t st); // segments -> compacted layout
 
 int device_grid(int n_cu, int waves_per_cu);
 
