@@ -256,6 +256,11 @@ rk_status rk_dataset_create(rk_ctx* c, const float* ux, const float* uy, const i
         std::memcpy(ds->ubox, b, sizeof b);
         ds->all_ids_sorted.assign(uid, uid + nu);
         std::sort(ds->all_ids_sorted.begin(), ds->all_ids_sorted.end());
+        if (ds->all_ids_sorted.front() >= 0) { // sort keys need only the bits below the largest id
+            uint32_t bits = 1;
+            while (bits < 31 && ((uint32_t)ds->all_ids_sorted.back() >> bits) != 0) ++bits;
+            ds->id_bits = bits;
+        }
     }
     hipStream_t st = c->stream;
     if (nf) {

@@ -1,21 +1,12 @@
 // rk_batch.hip -- batched form: many queries with uneven candidate counts and
 // mixed k in one hipGraph.
 //
-// The graph is a fixed launch sequence over device-side work lists:
-//   prep    zero the counters; candidate records for every (query, candidate),
-//           binary64 -> fp32
-//   root    one wave per query tests the <= 64 top tree nodes and lists
-//           the survivors
-//   level p = top .. 1
-//           items (query, node at level p); the node's 64 children on lanes;
-//           children not certainly covered by k candidates are listed below
-//   leaf    items (query, tile); per-user count as in the plain form;
-//           survivors go straight into per-query result segments
-//   sort    per-query block radix sort of each result segment
-// The fetch compacts the segments (offsets scan + copy) outside the graph.
-// Every kernel is a grid-stride loop over a device-side item count, so the
-// graph is captured once and replayed unchanged for any data; nothing waits
-// on another wave, so every launch drains.
+// A step is two launches, one block per query.  k_batch_query: candidate
+// records (binary64 -> fp32), the breadth-first descent of the bounding-
+// rectangle tree through LDS frontiers, the per-tile dominance count, the
+// query's result segment; k_batch_sort: its radix sort.  Blocks share no
+// counter, so queries never wait on each other.  The fetch compacts the segments
+// (offsets scan + copy) outside the graph.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,24 +20,15 @@ using namespace rk;
 
 namespace {
 
-constexpr uint64_t kItemBudget = 1ull << 25; // per level
 constexpr uint64_t kPoolBudget = 1ull << 26; // result ids per batch (all segments)
-constexpr int kWavesPerCU = 32;             // persistent grids: 8 waves per SIMD
 
 rk_status build_graph(rk_batch* b)
 {
     rk_dataset* ds = b->ds;
     rk_ctx* c = ds->ctx;
     hipStream_t st = c->stream;
-    const int grid = device_grid(c->n_cu, kWavesPerCU);
     RK_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    // the root kernel lists each query's surviving top nodes; the level
-    // kernels then descend level by level down to the tiles
-    launch_batch_prep(b->view, b->cand.p, ds->fx.p, ds->fy.p, b->wb.p, b->total_cand, st);
-    launch_batch_root(ds->view, b->view, grid, st);
-    for (uint32_t p = ds->view.n_levels - 1; p >= 1; --p) launch_batch_internal(ds->view, b->view, p, grid, st);
-    launch_batch_leaf(ds->view, b->view, grid, st);
-    launch_batch_sort(b->view, st);
+    launch_batch_query(ds->view, b->view, st);
     hipError_t e = hipStreamEndCapture(st, &b->graph);
     if (e != hipSuccess) return fail_hip(c, e, "hipStreamEndCapture", __LINE__);
     RK_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -111,20 +93,9 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
     RK_HIP(c, b->rec.reserve(all.size()));
     RK_HIP(c, b->orig.reserve(all.size()));
     RK_HIP(c, b->qcount.reserve(nq));
+    RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
     RK_HIP(c, b->qoff.reserve(nq + 1));
-    RK_HIP(c, b->ctr.reserve(1));
-    RK_HIP(c, b->h_ctr.reserve(1));
-    // work lists of tree levels 0 .. top
-    for (uint32_t p = 0; p < ds->view.n_levels; ++p) {
-        uint64_t cap = (uint64_t)nq * ds->view.nlev[p];
-        cap = std::max<uint64_t>(1, std::min(cap, kItemBudget));
-        uint64_t bcap = (cap + kBuckets - 1) / kBuckets;
-        // a bucket also takes one wave's whole staged flush
-        bcap = std::max<uint64_t>(bcap, 256);
-        RK_HIP(c, b->items[p].reserve(bcap * kBuckets));
-        b->view.items[p] = b->items[p].p;
-        b->view.item_cap[p] = (uint32_t)bcap;
-    }
+    RK_HIP(c, b->h_qcount.reserve(nq));
     // result segment per query: the whole budget split evenly, never more than the users
     uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? kPoolBudget / nq : 1));
     seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
@@ -138,14 +109,19 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
     BatchDev& v = b->view;
     v.q = b->q.p;
     v.nq = b->nq;
+    v.seg_cap = (uint32_t)seg_cap;
+    v.cand = b->cand.p;
+    v.fx = ds->fx.p;
+    v.fy = ds->fy.p;
+    v.wbound = b->wb.p;
     v.rec = b->rec.p;
     v.orig = b->orig.p;
     v.seg = b->seg.p;
-    v.seg_cap = (uint32_t)seg_cap;
     v.qcount = b->qcount.p;
+    v.qstat = b->qstat.p;
     v.qoff = b->qoff.p;
     v.out = b->out.p;
-    v.ctr = b->ctr.p;
+    v.id_bits = ds->id_bits;
     if (ds->nu && nq) {
         rk_status s = build_graph(b.get());
         if (s != RK_OK) return s;
@@ -161,21 +137,22 @@ rk_status rk_batch_run(rk_batch* b, double* device_seconds)
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
     b->ran = true;
+    b->overflow = false;
     if (!b->exec) {
         if (device_seconds) *device_seconds = 0.0;
-        std::memset(b->h_ctr.p, 0, sizeof(BatchCounters));
         return RK_OK;
     }
     RK_HIP(c, hipEventRecord(c->ev0, c->stream));
     RK_HIP(c, hipGraphLaunch(b->exec, c->stream));
     RK_HIP(c, hipEventRecord(c->ev1, c->stream));
-    RK_HIP(c, hipMemcpyAsync(b->h_ctr.p, b->ctr.p, sizeof(BatchCounters), hipMemcpyDeviceToHost, c->stream));
+    RK_HIP(c, hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
     RK_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (device_seconds) *device_seconds = ms * 1e-3;
-    if (b->h_ctr.p->overflow)
-        return fail(c, RK_ERR_CAPACITY, "batch exceeds the item or result budget; split it into smaller batches");
+    for (uint32_t i = 0; i < b->nq; ++i) b->overflow |= b->h_qcount.p[i] > b->view.seg_cap;
+    if (b->overflow) return fail(c, RK_ERR_CAPACITY, "a query's result exceeds its segment; split the batch");
     return RK_OK;
 }
 
@@ -187,8 +164,7 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
     if (!b->ran) return fail(c, RK_ERR_INVALID_ARG, "rk_batch_run has not been called");
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
-    if (b->exec && b->h_ctr.p->overflow)
-        return fail(c, RK_ERR_CAPACITY, "the last run overflowed its item or result budget; split the batch");
+    if (b->overflow) return fail(c, RK_ERR_CAPACITY, "the last run overflowed a result segment; split the batch");
     const uint32_t nq = b->nq;
     std::vector<uint64_t> qoff(nq + 1, 0);
     if (b->exec) {
@@ -220,7 +196,7 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
             if (n) std::memcpy(dst, ds->all_ids_sorted.data(), n * sizeof(int32_t));
         } else if (n) {
             std::memcpy(dst, dev.data() + qoff[i], n * sizeof(int32_t));
-            if (n > 4096) std::sort(dst, dst + n); // larger than the device block sort
+            if (n > kDeviceSortMax) std::sort(dst, dst + n); // larger than the device block sort
         }
     }
     return RK_OK;
@@ -230,20 +206,26 @@ rk_status rk_batch_stats(const rk_batch* b, uint64_t* items, size_t n_items_cap,
                          uint64_t* rechecks)
 {
     if (!b || !b->ran) return RK_ERR_INVALID_ARG;
-    const BatchCounters& c = *b->h_ctr.p;
+    rk_ctx* c = b->ds->ctx;
     const size_t nl = b->ds->view.n_levels;
     if (n_levels) *n_levels = nl;
-    for (size_t l = 0; l < nl && l < n_items_cap && items; ++l) {
-        uint64_t t = 0;
-        for (int j = 0; j < kBuckets; ++j) t += std::min(c.items[l][j], b->view.item_cap[l]);
-        items[l] = t;
+    std::vector<uint32_t> st;
+    if (b->exec) {
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        RK_HIP(c, hipSetDevice(c->device));
+        st.resize((size_t)b->nq * kQueryStats);
+        RK_HIP(c, hipMemcpy(st.data(), b->qstat.p, st.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
     }
-    if (results) *results = c.results;
-    if (rechecks) {
-        uint64_t t = 0;
-        for (int j = 0; j < kBuckets; ++j) t += c.rechecks[j];
-        *rechecks = t;
+    uint64_t lv[kMaxLevels] = {}, rech = 0, res = 0;
+    for (uint32_t i = 0; b->exec && i < b->nq; ++i) {
+        const uint32_t* s = st.data() + (size_t)i * kQueryStats;
+        rech += s[0];
+        for (size_t l = 0; l < nl; ++l) lv[l] += s[1 + l];
+        res += b->h_qcount.p[i];
     }
+    for (size_t l = 0; l < nl && l < n_items_cap && items; ++l) items[l] = lv[l];
+    if (results) *results = res;
+    if (rechecks) *rechecks = rech;
     return RK_OK;
 }
 
@@ -307,8 +289,7 @@ rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq, int
         t->download_s = now_s() - t2;
         t->n_candidates = b->total_cand;
         uint64_t rech = 0;
-        for (int j = 0; j < kBuckets && b->h_ctr.p; ++j) rech += b->h_ctr.p->rechecks[j];
-        t->n_fp64_rechecks = rech;
+        if (rk_batch_stats(b, nullptr, 0, nullptr, nullptr, &rech) == RK_OK) t->n_fp64_rechecks = rech;
         t->total_s = now_s() - t0;
     }
     rk_batch_destroy(b);

@@ -107,6 +107,7 @@ struct rk_dataset {
     std::vector<int32_t> hfid;
     std::vector<int32_t> all_ids_sorted; // result of a query where every user qualifies
     double ubox[4] = {0, 0, -1, -1};     // users' bounding box (minx, miny, maxx, maxy)
+    uint32_t id_bits = 32;               // ids are non-negative and below 2^id_bits (32: any int32)
     std::unique_ptr<rk::FacilityIndex> index;
     rk::DsDev view{};
     // scratch of the plain / count-all forms
@@ -131,17 +132,17 @@ struct rk_batch {
     rk::DevBuf<double2> wb;
     rk::DevBuf<rk::CandRec> rec;
     rk::DevBuf<float2> orig;
-    rk::DevBuf<uint64_t> items[rk::kMaxLevels + 1];
     rk::DevBuf<int32_t> seg;     // per-query result segments
     rk::DevBuf<uint32_t> qcount;
+    rk::DevBuf<uint32_t> qstat;
     rk::DevBuf<uint64_t> qoff;
     rk::DevBuf<int32_t> out;
-    rk::DevBuf<rk::BatchCounters> ctr;
-    rk::HostBuf<rk::BatchCounters> h_ctr;
+    rk::HostBuf<uint32_t> h_qcount;
     rk::BatchDev view{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool ran = false;
+    bool overflow = false;       // the last run had a query beyond its result segment
     double filter_s = 0.0;
     ~rk_batch()
     {

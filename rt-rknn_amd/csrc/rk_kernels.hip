@@ -11,6 +11,7 @@
 // findKNearestSimpleKernel (rknn-naive-gpu/rknn_kernel.cu:119-169), and the
 // rknn-rt raygen/any-hit pair (rknn-rt/rt.cu:46-81,95-111).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <hipcub/block/block_radix_sort.hpp>
 #include <hipcub/hipcub.hpp>
 
@@ -25,6 +26,14 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Lane mask of a predicate (the wave64 ballot straight from the compare,
+// without materialising 0/1 in a vector register first).
+__device__ __forceinline__ uint64_t ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
+// 3/8 < t < 5/8 exactly: t - 1/2 is exact for t in [1/4, 1] (Sterbenz), and
+// outside that interval |fl(t - 1/2)| >= 1/4 by monotone rounding.
+__device__ __forceinline__ bool in_band(float t) { return fabsf(t - 0.5f) < 0.125f; }
 
 __device__ __forceinline__ uint32_t rank_below(uint64_t mask)
 {
@@ -57,22 +66,39 @@ struct LeafOut {
     uint32_t fp64;     // (user, candidate) tests resolved in binary64
 };
 
-// Wave-private LDS staging of one 64-candidate chunk.
+// Candidate records of one query live in a block-shared LDS array when there
+// are at most N of them (built or loaded once per block, then read as
+// broadcasts by every wave).  Queries with more candidates stream 64-record
+// chunks from global memory through a wave-private LDS slot (WaveLds); the
+// two layouts share the same LDS bytes.
 struct WaveLds {
     CandRec rec[64];
     float2 orig[64];
 };
 
+template <uint32_t N>
+struct SharedRecs {
+    CandRec rec[N];
+    float2 orig[N];
+};
+
 // One query against one tile of 64 users.  Phase A puts candidates on lanes
-// and classifies each against the tile's bounding rectangle: certainly
+// and classifies each against the tile's bounding rectangle m: certainly
 // covering the whole tile (counted once for every user), certainly missing
 // it, or crossing it.  Phase B puts users on lanes and evaluates only the
-// crossing candidates; a (user, candidate) test whose fp32 value lands in the
-// band is settled on the spot by the oracle's binary64 expression, so the
-// count is exact when the loop ends.
+// crossing candidates, four per pass so that their LDS reads overlap; a
+// (user, candidate) test whose fp32 value lands in the band is settled on
+// the spot by the oracle's binary64 expression, so the count is exact when
+// the loop ends.
+//   kShared: rec/orig are the block-shared LDS arrays (else global, staged
+//            64 at a time in L)
+//   kPre:    the users' coordinates are given (pux, puy); otherwise they are
+//            loaded on the first crossing candidate, so culled tiles cost no
+//            user traffic
+template <bool kShared, bool kPre>
 __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, uint32_t nc,
                              const CandRec* __restrict__ rec, const float2* __restrict__ orig, uint32_t tile,
-                             WaveLds* L)
+                             float4 m, float pux, float puy, WaveLds* L)
 {
     const uint32_t lane = lane_id();
     k = uniform(k);
@@ -80,10 +106,9 @@ __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, ui
     tile = uniform(tile);
     const uint32_t u = tile * kTile + lane;
     const bool valid = u < ds.nu;
-    float ux = 0.f, uy = 0.f, wx = 0.f, wy = 0.f; // loaded on the first crossing candidate
+    float ux = pux, uy = puy, wx = 0.f, wy = 0.f;
     double dq = 0.0;
     bool loaded = false; // wave-uniform
-    const float4 m = ds.mbr[0][tile];
     const float lx = m.x - qx, ly = m.y - qy, hx = m.z - qx, hy = m.w - qy;
 
     uint32_t base = 0; // wave-uniform: candidates covering the whole tile
@@ -97,70 +122,101 @@ __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, ui
         float2 o = make_float2(0.f, 0.f);
         if (have) {
             r = rec[j];
-            o = orig[j];
+            if (!kShared) o = orig[j];
         }
         const float tlo = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
         const float thi = fmaf(r.gx >= 0.f ? hx : lx, r.gx, fmaf(r.gy >= 0.f ? hy : ly, r.gy, -r.sh));
         const bool full = have && tlo >= kCertainHi;
         const bool cross = have && !full && thi > kCertainLo;
-        base += (uint32_t)__popcll(__ballot(full));
+        base += (uint32_t)__popcll(ballot(full));
         if (base >= k) {
             dead = true;
             break;
         }
-        uint64_t cm = __ballot(cross);
+        uint64_t cm = ballot(cross);
         if (cm) {
-            if (!loaded) { // culled tiles never touch their users (arrays padded to whole tiles)
-                ux = ds.ux[u];
-                uy = ds.uy[u];
+            if (!loaded) { // (arrays are padded to whole tiles)
+                if (!kPre) {
+                    ux = ds.ux[u];
+                    uy = ds.uy[u];
+                }
                 wx = ux - qx;
                 wy = uy - qy;
                 dq = d2_exact(ux, uy, qx, qy);
                 loaded = true;
             }
-            L->rec[lane] = r;
-            L->orig[lane] = o;
-            __builtin_amdgcn_wave_barrier();
+            const CandRec* cr = rec + c0;
+            const float2* co = orig + c0;
+            if (!kShared) {
+                L->rec[lane] = r;
+                L->orig[lane] = o;
+                __builtin_amdgcn_wave_barrier();
+                cr = L->rec;
+                co = L->orig;
+            }
             while (cm) {
-                const int b = __ffsll((unsigned long long)cm) - 1;
+                uint32_t b[4];
+                bool v[4]; // wave-uniform: slot g holds a candidate
+                b[0] = (uint32_t)__ffsll((unsigned long long)cm) - 1;
                 cm &= cm - 1;
-                const CandRec c = L->rec[b];
-                const float t = fmaf(wx, c.gx, fmaf(wy, c.gy, -c.sh));
-                cnt += (t >= kCertainHi) ? 1u : 0u;
-                const bool inband = (t > kCertainLo) && (t < kCertainHi);
-                const uint64_t bm = __ballot(inband);
-                if (bm) { // rare: settle the band in binary64
-                    const float2 f = L->orig[b];
-                    if (inband && d2_exact(ux, uy, f.x, f.y) < dq) ++cnt;
-                    fp64 += (uint32_t)__popcll(bm & __ballot(valid));
+                v[0] = true;
+#pragma unroll
+                for (int g = 1; g < 4; ++g) {
+                    v[g] = cm != 0;
+                    b[g] = v[g] ? (uint32_t)__ffsll((unsigned long long)cm) - 1 : b[0];
+                    cm &= cm - 1;
+                }
+                float t[4];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const CandRec c = cr[b[g]];
+                    t[g] = fmaf(wx, c.gx, fmaf(wy, c.gy, -c.sh));
+                }
+                bool band[4];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    cnt += (v[g] && t[g] >= kCertainHi) ? 1u : 0u;
+                    band[g] = v[g] && in_band(t[g]);
+                }
+                if (ballot(band[0] || band[1] || band[2] || band[3])) { // rare: settle in binary64
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const uint64_t bm = ballot(band[g]);
+                        if (bm) {
+                            const float2 f = co[b[g]];
+                            if (band[g] && d2_exact(ux, uy, f.x, f.y) < dq) ++cnt;
+                            fp64 += (uint32_t)__popcll(bm & ballot(valid));
+                        }
+                    }
                 }
             }
-            __builtin_amdgcn_wave_barrier();
+            if (!kShared) __builtin_amdgcn_wave_barrier();
         }
-        if (__ballot(valid && base + cnt < k) == 0) {
+        if (ballot(valid && base + cnt < k) == 0) {
             dead = true;
             break;
         }
     }
     LeafOut out;
-    out.keep = __ballot(!dead && valid && base + cnt < k);
+    out.keep = ballot(!dead && valid && base + cnt < k);
     out.fp64 = fp64;
     return out;
 }
 
 // ---------------------------------------------------------------- plain form
 
-__global__ __launch_bounds__(kBlock) void k_plain(DsDev ds, QDev q, const CandRec* __restrict__ rec,
-                                                  const float2* __restrict__ orig, int32_t* __restrict__ out,
-                                                  uint32_t* __restrict__ ctr)
+constexpr uint32_t kPlainRecLds = 512;
+
+template <bool kShared>
+__device__ void plain_tiles(const DsDev& ds, const QDev& q, const CandRec* __restrict__ rec,
+                            const float2* __restrict__ orig, WaveLds* L, int32_t* __restrict__ out,
+                            uint32_t* __restrict__ ctr, uint32_t& rech)
 {
-    __shared__ WaveLds s_w[kWaves];
-    const uint32_t wave = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
     const uint32_t stride = gridDim.x * kWaves;
-    uint32_t rech = 0;
-    for (uint32_t tile = blockIdx.x * kWaves + wave; tile < ds.n_tiles; tile += stride) {
-        const LeafOut r = leaf_tile(ds, q.qx, q.qy, q.k, q.nc, rec, orig, tile, &s_w[wave]);
+    for (uint32_t tile = blockIdx.x * kWaves + (threadIdx.x >> 6); tile < ds.n_tiles; tile += stride) {
+        const LeafOut r =
+            leaf_tile<kShared, false>(ds, q.qx, q.qy, q.k, q.nc, rec, orig, tile, ds.mbr[0][tile], 0.f, 0.f, L);
         rech += r.fp64;
         if (r.keep) {
             uint32_t pos = 0;
@@ -169,9 +225,30 @@ __global__ __launch_bounds__(kBlock) void k_plain(DsDev ds, QDev q, const CandRe
             if ((r.keep >> lane) & 1ull) out[pos + rank_below(r.keep)] = ds.uid[tile * kTile + lane];
         }
     }
-    if (lane == 0 && rech) atomicAdd(&ctr[1 + (blockIdx.x * kWaves + wave) % kBuckets], rech);
 }
 
+__global__ __launch_bounds__(kBlock) void k_plain(DsDev ds, QDev q, const CandRec* __restrict__ rec,
+                                                  const float2* __restrict__ orig, int32_t* __restrict__ out,
+                                                  uint32_t* __restrict__ ctr)
+{
+    __shared__ union {
+        SharedRecs<kPlainRecLds> s;
+        WaveLds w[kWaves];
+    } lds;
+    const uint32_t wave = threadIdx.x >> 6;
+    uint32_t rech = 0;
+    if (q.nc <= kPlainRecLds) {
+        for (uint32_t j = threadIdx.x; j < q.nc; j += kBlock) {
+            lds.s.rec[j] = rec[j];
+            lds.s.orig[j] = orig[j];
+        }
+        __syncthreads();
+        plain_tiles<true>(ds, q, lds.s.rec, lds.s.orig, nullptr, out, ctr, rech);
+    } else {
+        plain_tiles<false>(ds, q, rec, orig, &lds.w[wave], out, ctr, rech);
+    }
+    if (lane_id() == 0 && rech) atomicAdd(&ctr[1 + (blockIdx.x * kWaves + wave) % kBuckets], rech);
+}
 // ------------------------------------------------------------ count-all form
 
 // Every (user, candidate) pair is evaluated: no culling, no early exit.  A
@@ -236,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_count_all(DsDev ds, QDev q, const Ca
             const float thi = fmaf(r.gx >= 0.f ? hx : lx, r.gx, fmaf(r.gy >= 0.f ? hy : ly, r.gy, -r.sh));
             const bool clean = have && (tlo >= 1.1875f || thi <= -0.1875f);
             const bool slow = have && !clean;
-            const uint64_t mc = __ballot(clean), ms = __ballot(slow);
+            const uint64_t mc = ballot(clean), ms = ballot(slow);
             if (clean) wlds[rank_below(mc)] = r;
             if (slow) {
                 wlds[64 + rank_below(ms)] = r;
@@ -258,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_count_all(DsDev ds, QDev q, const Ca
                     const float t = fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh));
                     cnt[j] += (t >= kCertainHi) ? 1u : 0u;
                     const bool inband = (t > kCertainLo) && (t < kCertainHi);
-                    const uint64_t bm = __ballot(inband);
+                    const uint64_t bm = ballot(inband);
                     if (bm) { // rare: settle the band in binary64
                         const float2 f = worig[i];
                         if (inband && d2_exact(px[j], py[j], f.x, f.y) < dq[j]) ++cnt[j];
@@ -371,95 +448,95 @@ __global__ __launch_bounds__(kBlock) void k_level_mbr(const float4* __restrict__
 }
 
 // -------------------------------------------------------------- batched form
+//
+// One block of kQWaves waves per query.  The block builds the query's
+// candidate records, walks the bounding-rectangle tree breadth-first
+// through per-level frontiers in LDS, tests the surviving tiles and appends
+// kept users to the query's result segment, which it then radix-sorts (a
+// second launch sorts segments too large for the block's sort).  Every
+// counter is an LDS counter of the block: the batch
+// touches no shared global counter, so queries proceed independently.
+//
+// Frontiers are bounded: the deepest level with pending nodes is always
+// expanded first, at most kPhaseItems nodes at a time, so a level's frontier
+// is empty when its parent level is expanded and never holds more than
+// kPhaseItems * 64 = kFront nodes.  A phase with fewer nodes than waves
+// gives each node several waves, each counting a slice of the candidates;
+// the slices' counts meet in LDS (cover[]).  Without the split a phase of
+// one node (the root, say) runs on one wave of the eight while the others
+// wait at the barrier.
+//
+// Shared records are sorted by the signs of (gx, gy), so that within a run
+// the corner of a rectangle minimising t is the same for every candidate:
+// the node test is then two FMAs, a compare and an add per (child,
+// candidate).
 
-// First kernel of a batch step: zero the counters, clear each query's
-// result count and build its candidate records (one block per query).
-__global__ void k_batch_prep(BatchDev b, const uint32_t* __restrict__ cand, const float* __restrict__ fx,
-                             const float* __restrict__ fy, const double2* __restrict__ wb, uint64_t total)
-{
-    if (blockIdx.x == 0) {
-        uint32_t* c = reinterpret_cast<uint32_t*>(b.ctr);
-        for (uint32_t i = threadIdx.x; i < sizeof(BatchCounters) / 4; i += blockDim.x) c[i] = 0;
-    }
-    for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
-        const QDev Q = b.q[qi];
-        if (threadIdx.x == 0) b.qcount[qi] = 0;
-        const double2 w = wb[qi];
-        for (uint32_t j = threadIdx.x; j < Q.nc; j += blockDim.x) {
-            const uint64_t g = Q.off + j;
-            if (g >= total) break;
-            const uint32_t f = cand[g];
-            const float x = fx[f], y = fy[f];
-            const_cast<CandRec*>(b.rec)[g] = make_record(Q.qx, Q.qy, x, y, w.x, w.y);
-            const_cast<float2*>(b.orig)[g] = make_float2(x, y);
-        }
-    }
-}
+constexpr uint32_t kQWaves = kQueryBlock / 64;
+constexpr uint32_t kPhaseItems = kFront / kFanout;
+constexpr uint32_t kRecLds = 1024; // shared-record capacity of the batched form
 
-// Items at parent level p are (query, node); the node's <= 64 children at
-// level p-1 go on lanes.  A child is culled once k candidates certainly cover
-// its whole rectangle (the internal-node test of rknn-inf/inf.h:219-237,
-// with certified corner evaluation instead of a 4-corner FP64 test).
-// Survivors are staged in a wave-private LDS buffer and appended to level
-// p-1 with one atomic per flush instead of one per item.
-constexpr uint32_t kStage = 192;
-
-// Per-wave view of a bucketed work list: lane b holds bucket b's exclusive
-// and inclusive prefix of the append counts (clamped to the bucket size).
-struct ListView {
-    const uint64_t* base;
-    uint32_t bcap;
-    uint32_t excl, incl, total;
+struct QueryCtl {
+    uint32_t n[kMaxLevels];     // frontier fill per level
+    uint32_t c[kMaxLevels];     // frontier cursor per level
+    uint32_t items[kMaxLevels]; // nodes that entered each level's frontier
+    uint32_t nres;              // result ids of the query so far
+    uint32_t fp64;              // (user, candidate) tests settled in binary64
+    uint32_t qcnt[4];           // shared records per sign quadrant
+    uint32_t qcur[4];           // placement cursors per quadrant
+    uint32_t quad[5];           // quadrant q holds records [quad[q], quad[q+1])
+    uint32_t red[2 * kQWaves + 1]; // the segment sort's reductions and skew flag
+    uint32_t pad[2];
 };
+constexpr uint32_t kCtlWords = sizeof(QueryCtl) / 4;
+constexpr size_t kCtlBytes = (sizeof(QueryCtl) + 15) / 16 * 16;
+constexpr size_t kCoverBytes = kPhaseItems * 64 * sizeof(uint32_t);
+// candidate area: the shared records, or one streaming slot per wave
+constexpr size_t kCandBytes = sizeof(SharedRecs<kRecLds>) > kQWaves * sizeof(WaveLds)
+                                  ? sizeof(SharedRecs<kRecLds>)
+                                  : kQWaves * sizeof(WaveLds);
 
-__device__ ListView list_view(const uint64_t* base, const uint32_t* counts, uint32_t bcap)
+// Sign quadrant of a record: bit 0 gx < 0, bit 1 gy < 0 (the complement of
+// the corner choice gx >= 0 ? low : high of the node test).
+__device__ __forceinline__ uint32_t quadrant(const CandRec& r)
 {
-    const uint32_t lane = lane_id();
-    const uint32_t c = min(counts[lane], bcap);
-    uint32_t incl = c;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
-        if ((int)lane >= o) incl += v;
+    return (r.gx >= 0.f ? 0u : 1u) | (r.gy >= 0.f ? 0u : 2u);
+}
+
+// Per lane (child), count candidates [i0, i1) of one quadrant run whose t at
+// corner (cx, cy) is >= 5/8.  Stops once every valid lane has >= k.
+__device__ __forceinline__ void cover_run(const CandRec* __restrict__ cr, uint32_t i0, uint32_t i1, float cx,
+                                          float cy, uint32_t k, bool valid, uint32_t& full, bool& live)
+{
+    uint32_t i = i0;
+    while (i < i1 && live) {
+        if (i + 16 <= i1) { // unrolled so the LDS reads pipeline
+#pragma unroll
+            for (uint32_t d = 0; d < 16; ++d) {
+                const CandRec r = cr[i + d];
+                full += (fmaf(cx, r.gx, fmaf(cy, r.gy, -r.sh)) >= kCertainHi) ? 1u : 0u;
+            }
+            i += 16;
+        } else {
+            for (; i < i1; ++i) {
+                const CandRec r = cr[i];
+                full += (fmaf(cx, r.gx, fmaf(cy, r.gy, -r.sh)) >= kCertainHi) ? 1u : 0u;
+            }
+        }
+        live = ballot(valid && full < k) != 0;
     }
-    ListView v{base, bcap, incl - c, incl, 0};
-    v.total = uniform((uint32_t)__builtin_amdgcn_readlane((int)incl, 63));
-    return v;
 }
 
-// Item i of the list (all lanes call it with the same i < total).
-__device__ uint64_t list_item(const ListView& v, uint32_t i)
-{
-    const uint32_t bucket = uniform((uint32_t)__popcll(__ballot(v.incl <= i)));
-    const uint32_t start = (uint32_t)__builtin_amdgcn_readlane((int)v.excl, (int)bucket);
-    return uniform64(v.base[(uint64_t)bucket * v.bcap + (i - start)]);
-}
-
-__device__ void flush_items(uint64_t* stage, uint32_t& n, uint32_t* counters, uint64_t* next, uint32_t bcap,
-                            uint32_t* overflow)
-{
-    if (n == 0) return;
-    const uint32_t lane = lane_id();
-    const uint32_t bucket = (blockIdx.x * kWaves + (threadIdx.x >> 6)) % kBuckets;
-    uint32_t pos = 0;
-    if (lane == 0) pos = atomicAdd(&counters[bucket], n);
-    pos = uniform(pos);
-    __builtin_amdgcn_wave_barrier();
-    uint64_t* dst = next + (uint64_t)bucket * bcap;
-    for (uint32_t i = lane; i < n; i += 64) {
-        if (pos + i < bcap)
-            dst[pos + i] = stage[i];
-        else
-            *overflow = 1;
-    }
-    __builtin_amdgcn_wave_barrier();
-    n = 0;
-}
-
-// Lanes = the <= 64 children of `node` at a level with MBRs `cm`: the mask of
-// children that fewer than k candidates certainly cover.  Candidate records
-// are staged 64 at a time in wlds; `staged` says chunk 0 is already there.
-__device__ uint64_t surviving_children(const float4* __restrict__ cm, uint32_t n_child, uint32_t node,
-                                       const QDev& Q, const CandRec* __restrict__ rec, CandRec* wlds, bool staged)
+// Lanes = the <= 64 children of `node` at a level with MBRs `cm`: per child,
+// how many of candidates [lo, hi) certainly cover its whole rectangle (the
+// internal-node test of rknn-inf/inf.h:219-237, with certified corner
+// evaluation instead of a 4-corner binary64 test).  A count that reaches k
+// may stop early: the child is culled whatever the other candidates say.
+//   kShared: quadrant-sorted records in LDS (quad = run starts)
+//   else:    records in global memory, streamed through wlds
+template <bool kShared>
+__device__ uint32_t cover_count(const float4* __restrict__ cm, uint32_t n_child, uint32_t node, const QDev& Q,
+                                const CandRec* __restrict__ rec, const uint32_t* quad, uint32_t lo, uint32_t hi,
+                                CandRec* wlds)
 {
     const uint32_t lane = lane_id();
     const uint32_t c = node * kFanout + lane;
@@ -467,150 +544,440 @@ __device__ uint64_t surviving_children(const float4* __restrict__ cm, uint32_t n
     const float4 m = cm[valid ? c : 0];
     const float lx = m.x - Q.qx, ly = m.y - Q.qy, hx = m.z - Q.qx, hy = m.w - Q.qy;
     uint32_t full = 0;
-    for (uint32_t c0 = 0; c0 < Q.nc; c0 += 64) {
-        const uint32_t n = min(64u, Q.nc - c0);
-        if (c0 > 0 || !staged) {
-            __builtin_amdgcn_wave_barrier();
-            if (lane < n) wlds[lane] = rec[c0 + lane];
-            __builtin_amdgcn_wave_barrier();
+    bool live = true;
+    if (kShared) {
+        for (uint32_t qd = 0; qd < 4 && live; ++qd) {
+            const uint32_t a = max(lo, uniform(quad[qd])), e = min(hi, uniform(quad[qd + 1]));
+            if (a < e) cover_run(rec, a, e, (qd & 1) ? hx : lx, (qd & 2) ? hy : ly, Q.k, valid, full, live);
         }
-        bool live = true;
-        for (uint32_t i0 = 0; i0 < n && live; i0 += 16) {
-            if (i0 + 16 <= n) { // full block: unrolled so the LDS reads pipeline
-#pragma unroll
-                for (uint32_t i = 0; i < 16; ++i) {
-                    const CandRec r = wlds[i0 + i];
-                    const float t = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
-                    full += (t >= kCertainHi) ? 1u : 0u;
-                }
-            } else {
-                for (uint32_t i = i0; i < n; ++i) {
-                    const CandRec r = wlds[i];
-                    const float t = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
-                    full += (t >= kCertainHi) ? 1u : 0u;
-                }
-            }
-            live = __ballot(valid && full < Q.k) != 0;
-        }
-        if (!live) return 0;
+        return full;
     }
-    return __ballot(valid && full < Q.k);
+    for (uint32_t c0 = lo; c0 < hi && live; c0 += 64) {
+        const uint32_t n = min(64u, hi - c0);
+        __builtin_amdgcn_wave_barrier();
+        if (lane < n) wlds[lane] = rec[c0 + lane];
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t i0 = 0; i0 < n && live; i0 += 16) {
+            const uint32_t i1 = min(n, i0 + 16);
+            for (uint32_t i = i0; i < i1; ++i) {
+                const CandRec r = wlds[i];
+                const float t = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
+                full += (t >= kCertainHi) ? 1u : 0u;
+            }
+            live = ballot(valid && full < Q.k) != 0;
+        }
+    }
+    return full;
 }
 
-__device__ __forceinline__ void stage_chunk0(const QDev& Q, const CandRec* __restrict__ rec, CandRec* wlds)
-{
-    const uint32_t lane = lane_id();
-    __builtin_amdgcn_wave_barrier();
-    if (lane < min(64u, Q.nc)) wlds[lane] = rec[lane];
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ __forceinline__ void push_children(uint64_t surv, uint32_t qi, uint32_t node, uint64_t* stage,
-                                              uint32_t& staged, uint32_t* counters, uint64_t* next, uint32_t bcap,
-                                              uint32_t* overflow)
+// Append the surviving children of `node` to a frontier (one LDS atomic per wave).
+__device__ __forceinline__ void push_children(uint64_t surv, uint32_t node, uint32_t* front, uint32_t* fill)
 {
     const uint32_t ns = (uint32_t)__popcll(surv);
     if (!ns) return;
-    if (staged + ns > kStage) flush_items(stage, staged, counters, next, bcap, overflow);
     const uint32_t lane = lane_id();
-    if ((surv >> lane) & 1ull) stage[staged + rank_below(surv)] = ((uint64_t)qi << 32) | (node * kFanout + lane);
-    staged += ns;
-    __builtin_amdgcn_wave_barrier();
+    uint32_t pos = 0;
+    if (lane == 0) pos = atomicAdd(fill, ns);
+    pos = uniform(pos);
+    if ((surv >> lane) & 1ull) front[pos + rank_below(surv)] = node * kFanout + lane;
 }
 
-// Root of every query: one wave per query tests the <= 64 top tree nodes
-// and lists the survivors for the level kernels.
-__global__ __launch_bounds__(kBlock) void k_batch_root(DsDev ds, BatchDev b)
-{
-    __shared__ CandRec s_rec[kWaves][64];
-    __shared__ uint64_t s_out[kWaves][kStage];
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t stride = gridDim.x * kWaves;
-    const uint32_t top = ds.n_levels - 1; // level of the <= 64 top nodes
-    uint64_t* __restrict__ next = b.items[top];
-    const uint32_t bcap = b.item_cap[top];
-    CandRec* wlds = s_rec[wave];
-    uint64_t* stage = s_out[wave];
-    uint32_t staged = 0;
-    for (uint32_t qi = blockIdx.x * kWaves + wave; qi < b.nq; qi += stride) {
-        const QDev Q = uniform_q(b.q[qi]);
-        if (Q.trivial) continue;
-        const CandRec* __restrict__ rec = b.rec + Q.off;
-        stage_chunk0(Q, rec, wlds);
-        const uint64_t surv = surviving_children(ds.mbr[top], ds.nlev[top], 0, Q, rec, wlds, true);
-        push_children(surv, qi, 0, stage, staged, b.ctr->items[top], next, bcap, &b.ctr->overflow);
-    }
-    flush_items(stage, staged, b.ctr->items[top], next, bcap, &b.ctr->overflow);
-}
+// Frontier of level l in the block's LDS: kFront entries per level below the
+// top, 64 for the top level.
+__device__ __forceinline__ uint32_t* frontier(uint32_t* front, uint32_t l) { return front + l * kFront; }
 
-// Items at parent level p: (query, node at level p); survivors among the
-// node's children are pushed to level p-1.
-__global__ __launch_bounds__(kBlock) void k_batch_internal(DsDev ds, BatchDev b, uint32_t p)
+// The descent of one query (all threads of the block call it).  The first
+// phase expands a virtual root whose children are the top-level nodes.
+template <bool kShared>
+__device__ void descend(const DsDev& ds, const BatchDev& b, const QDev& Q, uint32_t qi, QueryCtl& ctl,
+                        uint32_t* front, uint32_t* cover, const CandRec* __restrict__ rec,
+                        const float2* __restrict__ orig, WaveLds* L)
 {
-    __shared__ CandRec s_rec[kWaves][64];
-    __shared__ uint64_t s_out[kWaves][kStage];
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t stride = gridDim.x * kWaves;
-    const ListView lv = list_view(b.items[p], b.ctr->items[p], b.item_cap[p]);
-    const uint32_t n_items = lv.total;
-    const uint32_t cl = p - 1;
-    uint64_t* __restrict__ next = b.items[cl];
-    const uint32_t next_cap = b.item_cap[cl];
-    CandRec* wlds = s_rec[wave];
-    uint64_t* stage = s_out[wave];
-    uint32_t staged = 0; // wave-uniform
-    uint32_t it = blockIdx.x * kWaves + wave;
-    uint64_t item = it < n_items ? list_item(lv, it) : 0;
-    for (; it < n_items; it += stride) {
-        const uint64_t nxt = it + stride < n_items ? list_item(lv, it + stride) : 0; // prefetch
-        const uint32_t qi = uniform((uint32_t)(item >> 32)), node = uniform((uint32_t)item);
-        const QDev Q = uniform_q(b.q[qi]);
-        const CandRec* __restrict__ rec = b.rec + Q.off;
-        const uint64_t surv = surviving_children(ds.mbr[cl], ds.nlev[cl], node, Q, rec, wlds, false);
-        push_children(surv, qi, node, stage, staged, b.ctr->items[cl], next, next_cap, &b.ctr->overflow);
-        item = nxt;
-    }
-    flush_items(stage, staged, b.ctr->items[cl], next, next_cap, &b.ctr->overflow);
-}
-
-// Leaf items (query, tile): the plain form's tile routine; survivors go to
-// the query's own result segment (one atomic per item with survivors).
-__global__ __launch_bounds__(kBlock) void k_batch_leaf(DsDev ds, BatchDev b)
-{
-    __shared__ WaveLds s_w[kWaves];
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t lane = lane_id();
-    const uint32_t stride = gridDim.x * kWaves;
-    const ListView lv = list_view(b.items[0], b.ctr->items[0], b.item_cap[0]);
-    const uint32_t n_items = lv.total;
-    uint32_t rech = 0;
-    uint32_t it = blockIdx.x * kWaves + wave;
-    uint64_t item = it < n_items ? list_item(lv, it) : 0;
-    for (; it < n_items; it += stride) {
-        const uint64_t nxt = it + stride < n_items ? list_item(lv, it + stride) : 0; // prefetch
-        const uint32_t qi = uniform((uint32_t)(item >> 32)), tile = uniform((uint32_t)item);
-        const QDev Q = uniform_q(b.q[qi]);
-        const LeafOut r = leaf_tile(ds, Q.qx, Q.qy, Q.k, Q.nc, b.rec + Q.off, b.orig + Q.off, tile, &s_w[wave]);
-        rech += r.fp64;
-        if (r.keep) {
-            uint32_t rank0 = 0;
-            if (lane == 0) rank0 = atomicAdd(&b.qcount[qi], (uint32_t)__popcll(r.keep));
-            rank0 = uniform(rank0);
-            if ((r.keep >> lane) & 1ull) {
-                const uint32_t rk = rank0 + rank_below(r.keep);
-                if (rk < b.seg_cap)
-                    b.seg[(uint64_t)qi * b.seg_cap + rk] = ds.uid[tile * kTile + lane];
-                else
-                    b.ctr->overflow = 1;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+    const uint32_t top = uniform(ds.n_levels - 1);
+    int32_t* __restrict__ seg = b.seg + (uint64_t)qi * b.seg_cap;
+    uint32_t fp64 = 0; // per wave
+    bool root = true;
+    for (;;) {
+        uint32_t l = top + 1, c = 0, n = 1; // the root phase
+        if (!root) {
+            l = kMaxLevels; // deepest level with pending nodes
+            for (uint32_t p = 0; p <= top; ++p)
+                if (ctl.c[p] < ctl.n[p]) {
+                    l = p;
+                    break;
+                }
+            l = uniform(l);
+            if (l == kMaxLevels) break;
+            c = uniform(ctl.c[l]);
+            n = uniform(ctl.n[l]);
+        }
+        const uint32_t* fl = frontier(front, l);
+        uint32_t end = n;
+        if (l == 0) {
+            // every pending tile; the next tile's rectangle and users are
+            // loaded while the current one is counted
+            uint32_t i = c + wave;
+            uint32_t tile = 0;
+            float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+            float ux = 0.f, uy = 0.f;
+            if (i < n) {
+                tile = uniform(fl[i]);
+                m = ds.mbr[0][tile];
+                ux = ds.ux[tile * kTile + lane];
+                uy = ds.uy[tile * kTile + lane];
+            }
+            for (; i < n; i += kQWaves) {
+                uint32_t tn = 0;
+                float4 mn = make_float4(0.f, 0.f, 0.f, 0.f);
+                float uxn = 0.f, uyn = 0.f;
+                if (i + kQWaves < n) {
+                    tn = uniform(fl[i + kQWaves]);
+                    mn = ds.mbr[0][tn];
+                    uxn = ds.ux[tn * kTile + lane];
+                    uyn = ds.uy[tn * kTile + lane];
+                }
+                const LeafOut r = leaf_tile<kShared, true>(ds, Q.qx, Q.qy, Q.k, Q.nc, rec, orig, tile, m, ux, uy, L);
+                fp64 += r.fp64;
+                if (r.keep) {
+                    uint32_t pos = 0;
+                    if (lane == 0) pos = atomicAdd(&ctl.nres, (uint32_t)__popcll(r.keep));
+                    pos = uniform(pos);
+                    if ((r.keep >> lane) & 1ull) {
+                        const uint32_t rk = pos + rank_below(r.keep);
+                        if (rk < b.seg_cap) seg[rk] = ds.uid[tile * kTile + lane];
+                    }
+                }
+                tile = tn;
+                m = mn;
+                ux = uxn;
+                uy = uyn;
+            }
+        } else {
+            // nodes [c, end) of level l; their children (level l-1) fill
+            // that level's frontier; S waves per node, one candidate slice each
+            end = min(n, c + kPhaseItems);
+            const uint32_t items = end - c;
+            const uint32_t S = max(1u, kQWaves / items);
+            const uint32_t per = ((Q.nc + S - 1) / S + 15) & ~15u;
+            const uint32_t cl = l - 1;
+            const float4* cmbr = ds.mbr[cl];
+            const uint32_t n_child = ds.nlev[cl];
+            uint32_t* dst = frontier(front, cl);
+            for (uint32_t w = wave; w < items * S; w += kQWaves) {
+                const uint32_t it = w / S, sl = w % S;
+                const uint32_t node = root ? 0u : uniform(fl[c + it]);
+                const uint32_t lo = min(Q.nc, sl * per), hi = min(Q.nc, lo + per);
+                const uint32_t full = cover_count<kShared>(cmbr, n_child, node, Q, rec, ctl.quad, lo, hi, L->rec);
+                if (S == 1)
+                    push_children(ballot(node * kFanout + lane < n_child && full < Q.k), node, dst, &ctl.n[cl]);
+                else if (lo < hi)
+                    atomicAdd(&cover[it * 64 + lane], full);
+            }
+            if (S > 1) {
+                __syncthreads();
+                for (uint32_t it = wave; it < items; it += kQWaves) {
+                    const uint32_t node = root ? 0u : uniform(fl[c + it]);
+                    const uint32_t full = cover[it * 64 + lane];
+                    cover[it * 64 + lane] = 0;
+                    push_children(ballot(node * kFanout + lane < n_child && full < Q.k), node, dst, &ctl.n[cl]);
+                }
             }
         }
-        item = nxt;
+        __syncthreads();
+        if (tid == 0 && !root) {
+            if (end == n) { // frontier drained
+                ctl.items[l] += n;
+                ctl.c[l] = 0;
+                ctl.n[l] = 0;
+            } else {
+                ctl.c[l] = end;
+            }
+        }
+        __syncthreads();
+        root = false;
     }
-    if (lane == 0 && rech) atomicAdd(&b.ctr->rechecks[(blockIdx.x * kWaves + wave) % kBuckets], rech);
+    if (lane == 0 && fp64) atomicAdd(&ctl.fp64, fp64);
 }
 
-// Exclusive scan of per-query result counts (one block) for the compacted
-// layout; also the batch total.
+// ------------------------------------------------------ result-segment sort
+//
+// A query's result ids are distinct, so an unstable bucket sort is exact:
+// keys are spread over kNB buckets by their position in [min, max] (a
+// monotone map, so bucket order is key order), placed with one LDS atomic
+// each, and every bucket (about n / kNB keys) is finished by an insertion
+// sort.  Cost O(n) with six block barriers, against six 4-bit passes of a
+// block radix sort.  A bucket holding more than kMaxBucket keys (ids bunched
+// very unevenly inside their range) aborts the attempt; the segment is then
+// left untouched for the radix-sort fallback.
+constexpr uint32_t kMaxBucket = 32;
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+    return v;
+}
+
+// Sorts seg[0, n), n <= kThreads * kItems, in place.  LDS: out[kThreads *
+// kItems], cnt[kNB], red[2 * kThreads / 64 + 1].  All threads call it.
+template <int kThreads, int kItems, uint32_t kNB>
+__device__ bool bucket_sort(int32_t* __restrict__ seg, uint32_t n, uint32_t* out, uint32_t* cnt, uint32_t* red)
+{
+    constexpr uint32_t kW = kThreads / 64;
+    constexpr uint32_t kPer = kNB / kThreads;
+    static_assert(kPer * kThreads == kNB, "buckets per thread");
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+    uint32_t key[kItems], pos[kItems];
+    uint32_t lo = 0xffffffffu, hi = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) { // keys mapped to unsigned order
+        const uint32_t j = tid + i * kThreads;
+        key[i] = j < n ? ((uint32_t)seg[j] ^ 0x80000000u) : 0u;
+        pos[i] = 0;
+        if (j < n) {
+            lo = min(lo, key[i]);
+            hi = max(hi, key[i]);
+        }
+    }
+    for (uint32_t b = tid; b < kNB; b += kThreads) cnt[b] = 0;
+    lo = wave_min_u32(lo);
+    hi = wave_max_u32(hi);
+    if (lane == 0) {
+        red[wave] = lo;
+        red[kW + wave] = hi;
+    }
+    if (tid == 0) red[2 * kW] = 0;
+    __syncthreads();
+    lo = red[0];
+    hi = red[kW];
+    for (uint32_t w = 1; w < kW; ++w) {
+        lo = min(lo, red[w]);
+        hi = max(hi, red[kW + w]);
+    }
+    const float scale = (float)kNB / ((float)(hi - lo) + 1.0f);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        if (tid + i * kThreads < n) {
+            const uint32_t b = min(kNB - 1, (uint32_t)((float)(key[i] - lo) * scale));
+            pos[i] = (b << 16) | atomicAdd(&cnt[b], 1u);
+        }
+    }
+    __syncthreads();
+    // exclusive scan of the counts: kPer consecutive buckets per thread
+    uint32_t local[kPer], sum = 0;
+#pragma unroll
+    for (uint32_t p = 0; p < kPer; ++p) {
+        local[p] = cnt[tid * kPer + p];
+        sum += local[p];
+    }
+    uint32_t inc = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)inc, o);
+        if ((int)lane >= o) inc += v;
+    }
+    if (lane == 63) red[wave] = inc;
+    __syncthreads();
+    uint32_t run = inc - sum;
+    for (uint32_t w = 0; w < wave; ++w) run += red[w];
+#pragma unroll
+    for (uint32_t p = 0; p < kPer; ++p) {
+        cnt[tid * kPer + p] = run; // bucket starts
+        run += local[p];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kItems; ++i)
+        if (tid + i * kThreads < n) out[cnt[pos[i] >> 16] + (pos[i] & 0xffffu)] = key[i];
+    __syncthreads();
+    for (uint32_t b = tid; b < kNB; b += kThreads) {
+        const uint32_t s0 = cnt[b], e0 = b + 1 < kNB ? cnt[b + 1] : n;
+        if (e0 - s0 > kMaxBucket) {
+            red[2 * kW] = 1;
+            continue;
+        }
+        for (uint32_t x = s0 + 1; x < e0; ++x) {
+            const uint32_t v = out[x];
+            uint32_t y = x;
+            while (y > s0 && out[y - 1] > v) {
+                out[y] = out[y - 1];
+                --y;
+            }
+            out[y] = v;
+        }
+    }
+    __syncthreads();
+    if (red[2 * kW]) return false;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+        const uint32_t j = tid + i * kThreads;
+        if (j < n) seg[j] = (int32_t)(out[j] ^ 0x80000000u);
+    }
+    return true;
+}
+
+// Segments of up to kQuerySortMax ids are sorted by the block that wrote
+// them; its candidate area (free once the descent ends) holds the keys and
+// the bucket counts.
+constexpr uint32_t kQuerySortMax = kQueryBlock * 8;
+constexpr uint32_t kQuerySortBuckets = 2048;
+static_assert((kQuerySortMax + kQuerySortBuckets) * sizeof(uint32_t) <= kCandBytes, "sort storage");
+
+// Radix sort fallback (rocPRIM's block radix sort via hipCUB, keys in
+// registers, only the bits below the largest id).
+template <class Sorter, int kItems>
+__device__ void sort_segment(int32_t* seg, uint32_t n, uint32_t id_bits, typename Sorter::TempStorage& tmp)
+{
+    int32_t keys[kItems];
+    const uint32_t base = threadIdx.x * kItems;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) keys[i] = base + i < n ? seg[base + i] : INT32_MAX;
+    Sorter(tmp).Sort(keys, 0, (int)id_bits);
+#pragma unroll
+    for (int i = 0; i < kItems; ++i)
+        if (base + i < n) seg[base + i] = keys[i];
+}
+
+// Eight waves per SIMD: four 8-wave blocks per CU, so a batch of 1024
+// queries is resident at once on 256 CUs.
+__global__ __launch_bounds__(kQueryBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+void k_batch_query(DsDev ds, BatchDev b)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    QueryCtl& ctl = *reinterpret_cast<QueryCtl*>(smem);
+    uint32_t* ctl_words = reinterpret_cast<uint32_t*>(smem);
+    uint32_t* cover = reinterpret_cast<uint32_t*>(smem + kCtlBytes);
+    unsigned char* cand = smem + kCtlBytes + kCoverBytes;
+    SharedRecs<kRecLds>& S = *reinterpret_cast<SharedRecs<kRecLds>*>(cand);
+    WaveLds* wl = reinterpret_cast<WaveLds*>(cand);
+    uint32_t* front = reinterpret_cast<uint32_t*>(cand + kCandBytes);
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+
+    for (uint32_t i = tid; i < kPhaseItems * 64; i += kQueryBlock) cover[i] = 0;
+    for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
+        const QDev Q = uniform_q(b.q[qi]);
+        uint32_t* stat = b.qstat + (uint64_t)qi * kQueryStats;
+        if (Q.trivial) { // k > |C|: every user qualifies; the host fills the result
+            if (tid < kQueryStats) stat[tid] = 0;
+            if (tid == 0) b.qcount[qi] = 0;
+            continue;
+        }
+        if (tid < kCtlWords) ctl_words[tid] = 0;
+        __syncthreads();
+        // candidate records (binary64 -> fp32): into LDS sorted by sign
+        // quadrant when they fit, else into the batch's global record array
+        const bool shared = Q.nc <= kRecLds;
+        const double2 w = b.wbound[qi];
+        if (shared) {
+            CandRec r[2];
+            float2 o[2];
+            uint32_t qd[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint32_t j = tid + h * kQueryBlock;
+                r[h] = {0.f, 0.f, 0.f, 0.f};
+                o[h] = make_float2(0.f, 0.f);
+                qd[h] = 4;
+                if (j < Q.nc) {
+                    const uint32_t f = b.cand[Q.off + j];
+                    const float x = b.fx[f], y = b.fy[f];
+                    r[h] = make_record(Q.qx, Q.qy, x, y, w.x, w.y);
+                    o[h] = make_float2(x, y);
+                    qd[h] = quadrant(r[h]);
+                }
+            }
+            uint32_t rank[2] = {0, 0};
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                for (uint32_t g = 0; g < 4; ++g) { // wave-aggregated quadrant counts
+                    const uint64_t mk = ballot(qd[h] == g);
+                    if (qd[h] == g) rank[h] = rank_below(mk);
+                    uint32_t pos = 0;
+                    if (lane == 0 && mk) pos = atomicAdd(&ctl.qcnt[g], (uint32_t)__popcll(mk));
+                    pos = uniform(pos);
+                    if (qd[h] == g) rank[h] += pos;
+                }
+            __syncthreads();
+            uint32_t start[5];
+            start[0] = 0;
+            for (int g = 0; g < 4; ++g) start[g + 1] = start[g] + ctl.qcnt[g];
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+                if (qd[h] < 4) {
+                    const uint32_t slot = start[qd[h]] + rank[h];
+                    S.rec[slot] = r[h];
+                    S.orig[slot] = o[h];
+                }
+            if (tid < 5) ctl.quad[tid] = start[tid];
+        } else {
+            CandRec* grec = b.rec + Q.off;
+            float2* gorig = b.orig + Q.off;
+            for (uint32_t j = tid; j < Q.nc; j += kQueryBlock) {
+                const uint32_t f = b.cand[Q.off + j];
+                const float x = b.fx[f], y = b.fy[f];
+                grec[j] = make_record(Q.qx, Q.qy, x, y, w.x, w.y);
+                gorig[j] = make_float2(x, y);
+            }
+        }
+        __syncthreads();
+        if (shared)
+            descend<true>(ds, b, Q, qi, ctl, front, cover, S.rec, S.orig, &wl[wave]);
+        else
+            descend<false>(ds, b, Q, qi, ctl, front, cover, b.rec + Q.off, b.orig + Q.off, &wl[wave]);
+        __syncthreads();
+        const uint32_t nres = ctl.nres;
+        if (tid == 0) {
+            b.qcount[qi] = nres;
+            stat[0] = ctl.fp64;
+        }
+        if (tid < kMaxLevels) stat[1 + tid] = ctl.items[tid];
+        const uint32_t ns = min(nres, b.seg_cap);
+        bool sorted = ns <= 1;
+        if (ns > 1 && ns <= kQuerySortMax) {
+            uint32_t* keys = reinterpret_cast<uint32_t*>(cand);
+            sorted = bucket_sort<kQueryBlock, kQuerySortMax / kQueryBlock, kQuerySortBuckets>(
+                b.seg + (uint64_t)qi * b.seg_cap, ns, keys, keys + kQuerySortMax, ctl.red);
+        }
+        if (tid == 0) stat[kSortSlot] = sorted ? 0u : 1u;
+        __syncthreads(); // before the next query resets the state
+    }
+}
+
+// Segments the traversal kernel left unsorted (larger than kQuerySortMax,
+// or bunched ids): bucket sort with up to kDeviceSortMax keys, else the
+// radix sort.  Larger segments are sorted by the host when fetched.
+constexpr int kSortBlock = 512;
+constexpr uint32_t kSortBuckets = 4096;
+static_assert(kDeviceSortMax == kSortBlock * 16, "device sort capacity");
+using SortLarge = hipcub::BlockRadixSort<int32_t, kSortBlock, 16>;
+
+__global__ __launch_bounds__(kSortBlock) void k_batch_sort(BatchDev b)
+{
+    __shared__ union {
+        struct {
+            uint32_t keys[kDeviceSortMax];
+            uint32_t cnt[kSortBuckets];
+            uint32_t red[2 * kSortBlock / 64 + 1];
+        } bk;
+        typename SortLarge::TempStorage radix;
+    } lds;
+    for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
+        if (b.qstat[(uint64_t)qi * kQueryStats + kSortSlot] == 0) continue;
+        const uint32_t n = min(b.qcount[qi], b.seg_cap);
+        if (n > kDeviceSortMax) continue;
+        int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
+        if (!bucket_sort<kSortBlock, 16, kSortBuckets>(seg, n, lds.bk.keys, lds.bk.cnt, lds.bk.red))
+            sort_segment<SortLarge, 16>(seg, n, b.id_bits, lds.radix);
+        __syncthreads();
+    }
+}
+
+// Fetch-time: exclusive scan of the per-query result counts (one block) for
+// the compacted layout.
 __global__ __launch_bounds__(1024) void k_batch_scan(BatchDev b)
 {
     __shared__ uint64_t s_part[1024];
@@ -632,51 +999,7 @@ __global__ __launch_bounds__(1024) void k_batch_scan(BatchDev b)
         b.qoff[i] = run;
         run += min(b.qcount[i], b.seg_cap);
     }
-    if (t == 1023) {
-        b.qoff[b.nq] = s_part[1023];
-        b.ctr->results = (uint32_t)min<uint64_t>(s_part[1023], 0xffffffffull);
-    }
-}
-
-// One block per query: radix sort of its result segment (rocPRIM block
-// radix sort via hipCUB, keys held in registers).  Segments of up to 2048
-// ids use 8 keys per thread, up to 4096 use 16; larger ones are flagged and
-// sorted by the host when fetched.
-using SortSmall = hipcub::BlockRadixSort<int32_t, 256, 8>;
-using SortLarge = hipcub::BlockRadixSort<int32_t, 256, 16>;
-
-template <class Sorter, int kItems>
-__device__ void sort_segment(int32_t* seg, uint32_t n, typename Sorter::TempStorage& tmp)
-{
-    int32_t keys[kItems];
-    const uint32_t base = threadIdx.x * kItems;
-#pragma unroll
-    for (int i = 0; i < kItems; ++i) keys[i] = base + i < n ? seg[base + i] : INT32_MAX;
-    Sorter(tmp).Sort(keys);
-#pragma unroll
-    for (int i = 0; i < kItems; ++i)
-        if (base + i < n) seg[base + i] = keys[i];
-}
-
-__global__ __launch_bounds__(256) void k_batch_sort(BatchDev b)
-{
-    __shared__ union {
-        typename SortSmall::TempStorage small;
-        typename SortLarge::TempStorage large;
-    } tmp;
-    for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
-        const uint32_t n = min(b.qcount[qi], b.seg_cap);
-        int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
-        if (n <= 1) continue;
-        if (n <= 256 * 8) {
-            sort_segment<SortSmall, 8>(seg, n, tmp.small);
-        } else if (n <= 256 * 16) {
-            sort_segment<SortLarge, 16>(seg, n, tmp.large);
-        } else if (threadIdx.x == 0) {
-            atomicAdd(&b.ctr->sort_overflow, 1u);
-        }
-        __syncthreads();
-    }
+    if (t == 1023) b.qoff[b.nq] = s_part[1023];
 }
 
 // Fetch-time: copy each query's segment to its place in the compacted array.
@@ -689,7 +1012,6 @@ __global__ void k_batch_compact(BatchDev b)
         for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
     }
 }
-
 int grid_for(uint64_t n, int per_block, int max_blocks)
 {
     uint64_t g = (n + per_block - 1) / per_block;
@@ -743,30 +1065,16 @@ void launch_scatter_u32(const uint32_t* src_sorted, const uint32_t* perm, uint32
     k_scatter_u32<<<grid_for(n, 256, 4096), 256, 0, st>>>(src_sorted, perm, n, dst_orig);
 }
 
-void launch_batch_prep(const BatchDev& b, const uint32_t* cand, const float* fx, const float* fy,
-                       const double2* wbound, uint64_t total, hipStream_t st)
+size_t batch_query_lds_bytes(uint32_t n_levels)
 {
-    k_batch_prep<<<b.nq < 4096 ? (b.nq ? b.nq : 1) : 4096, 256, 0, st>>>(b, cand, fx, fy, wbound, total);
+    return kCtlBytes + kCoverBytes + kCandBytes + ((size_t)(n_levels - 1) * kFront + 64) * sizeof(uint32_t);
 }
 
-void launch_batch_root(const DsDev& ds, const BatchDev& b, int grid, hipStream_t st)
+void launch_batch_query(const DsDev& ds, const BatchDev& b, hipStream_t st)
 {
-    k_batch_root<<<grid, kBlock, 0, st>>>(ds, b);
-}
-
-void launch_batch_internal(const DsDev& ds, const BatchDev& b, uint32_t parent_level, int grid, hipStream_t st)
-{
-    k_batch_internal<<<grid, kBlock, 0, st>>>(ds, b, parent_level);
-}
-
-void launch_batch_leaf(const DsDev& ds, const BatchDev& b, int grid, hipStream_t st)
-{
-    k_batch_leaf<<<grid, kBlock, 0, st>>>(ds, b);
-}
-
-void launch_batch_sort(const BatchDev& b, hipStream_t st)
-{
-    k_batch_sort<<<b.nq < 65535 ? (b.nq ? b.nq : 1) : 65535, 256, 0, st>>>(b);
+    const uint32_t grid = b.nq < 65535 ? (b.nq ? b.nq : 1) : 65535;
+    k_batch_query<<<grid, kQueryBlock, batch_query_lds_bytes(ds.n_levels), st>>>(ds, b);
+    k_batch_sort<<<grid < 1024 ? grid : 1024, kSortBlock, 0, st>>>(b);
 }
 
 void launch_batch_scan(const BatchDev& b, hipStream_t st)

@@ -8,35 +8,37 @@
 
 namespace rk {
 
-// Batch-wide device state written by the kernels.
-// Each level's work list is split into kBuckets regions with their own
-// append counters, so concurrent appends spread over kBuckets addresses.
+// Slots of the plain form's per-wave statistics counters: one counter hit by
+// every wave would serialise thousands of device atomics.
 constexpr int kBuckets = 64;
 
-struct BatchCounters {
-    uint32_t items[kMaxLevels + 1][kBuckets]; // items appended per parent level and bucket
-    uint32_t overflow;              // nonzero: an item list or a result segment ran out of room
-    uint32_t results;               // total result ids (written by the scan)
-    uint32_t sort_overflow;         // queries whose result did not fit the LDS sort
-    uint32_t pad;
-    // (user, candidate) tests settled in binary64, one slot per (wave % kBuckets):
-    // a single counter hit by every wave serialises thousands of device atomics
-    uint32_t rechecks[kBuckets];
-};
+// Batched form: one block of kQueryBlock threads per query; tree frontiers
+// of kFront nodes per level in LDS; result segments of up to kDeviceSortMax
+// ids are sorted on the device, larger ones by the host when fetched.
+constexpr int kQueryBlock = 512;
+constexpr uint32_t kFront = 512;
+constexpr uint32_t kDeviceSortMax = 8192;
+// per-query statistics: [0] binary64 tests, [1 + l] nodes entering level l,
+// [kSortSlot] nonzero when the traversal kernel left the segment unsorted
+constexpr uint32_t kSortSlot = 1 + kMaxLevels;
+constexpr uint32_t kQueryStats = 2 + kMaxLevels;
 
 struct BatchDev {
     const QDev* q;          // nq queries
     uint32_t nq;
-    const CandRec* rec;     // all candidate records, query-major
-    const float2* orig;     // all candidate facility coordinates (for the re-check)
-    uint64_t* items[kMaxLevels + 1]; // (query << 32 | node) per parent level, kBuckets regions
-    uint32_t item_cap[kMaxLevels + 1]; // capacity of one bucket region
+    uint32_t seg_cap;       // result ids one query's segment holds
+    const uint32_t* cand;   // candidate facility indices, query-major (QDev::off)
+    const float* fx;        // facility coordinates of the dataset
+    const float* fy;
+    const double2* wbound;  // per-query |u - q| axis bounds of the record scaling
+    CandRec* rec;           // candidate records of queries with > 64 candidates
+    float2* orig;           // their facility coordinates (for the binary64 re-check)
     int32_t* seg;           // nq segments of seg_cap ids: query i's results at seg[i * seg_cap]
-    uint32_t seg_cap;
-    uint32_t* qcount;       // results per query
+    uint32_t* qcount;       // results per query (may exceed seg_cap: overflow)
+    uint32_t* qstat;        // nq x kQueryStats statistics
     uint64_t* qoff;         // nq+1 exclusive offsets of the compacted layout
     int32_t* out;           // compacted results (written when fetched)
-    BatchCounters* ctr;
+    uint32_t id_bits;       // user ids are non-negative and below 2^id_bits (32: any int32)
 };
 
 // dataset construction
@@ -61,14 +63,11 @@ void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2*
 void launch_scatter_u32(const uint32_t* src_sorted, const uint32_t* perm, uint32_t n, uint32_t* dst_orig,
                         hipStream_t st);
 
-// batched form
-void launch_batch_prep(const BatchDev& b, const uint32_t* cand, const float* fx, const float* fy,
-                       const double2* wbound, uint64_t total, hipStream_t st);
-void launch_batch_root(const DsDev& ds, const BatchDev& b, int grid, hipStream_t st);
-void launch_batch_internal(const DsDev& ds, const BatchDev& b, uint32_t parent_level, int grid,
-                           hipStream_t st);
-void launch_batch_leaf(const DsDev& ds, const BatchDev& b, int grid, hipStream_t st);
-void launch_batch_sort(const BatchDev& b, hipStream_t st);
+// batched form: the whole step is the traversal launch (which also sorts
+// segments of up to 4096 ids) and the large-segment sort; capture them in a
+// graph
+size_t batch_query_lds_bytes(uint32_t n_levels);
+void launch_batch_query(const DsDev& ds, const BatchDev& b, hipStream_t st);
 void launch_batch_scan(const BatchDev& b, hipStream_t st);    // offsets of the compacted layout
 void launch_batch_compact(const BatchDev& b, hipStream_t st); // segments -> compacted layout
 

@@ -172,3 +172,43 @@ def test_medium_random_vs_oracle(rk, ctx, config):
                 assert np.array_equal(O.rknn(ux, uy, uid, fx, fy, int(q), k), exp)
     finally:
         ds.close()
+
+
+def test_batch_sort_paths(rk, ctx):
+    """Result segments of every size class and id layout against the oracle:
+    sorted by the traversal kernel (<= 4096 ids), by the large-segment kernel
+    (<= 8192, bucket sort or its radix fallback when ids bunch unevenly in
+    their range) and by the host beyond that; random, spatially coherent with
+    far outliers, and negative ids."""
+    r = np.random.default_rng(11)
+    nu, nf = 300_000, 200
+    ux = (r.random(nu) * 1000).astype(np.float32)
+    uy = (r.random(nu) * 1000).astype(np.float32)
+    fx = (r.random(nf) * 1000).astype(np.float32)
+    fy = (r.random(nf) * 1000).astype(np.float32)
+    fid = np.arange(nf, dtype=np.int32)
+    xrank = np.empty(nu, np.int64)
+    xrank[np.argsort(ux, kind="stable")] = np.arange(nu)
+    layouts = {
+        "random": r.permutation(nu),
+        "coherent+outliers": np.where(r.random(nu) < 0.01, xrank + (1 << 28), xrank),
+        "negative": -xrank - 1,
+    }
+    qs = r.choice(nf, 8, replace=False)
+    ks = [1, 2, 3, 5, 8, 3, 5, 1]
+    O.set_threads(16)
+    sizes = []
+    for name, ids in layouts.items():
+        uid = ids.astype(np.int32)
+        ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+        try:
+            print("sort paths: ids", name, flush=True)
+            got = ds.query_batch([(int(q), k) for q, k in zip(qs, ks)])
+            for q, k, g in zip(qs, ks, got):
+                exp = O.rknn(ux, uy, uid, fx, fy, int(q), k)
+                sizes.append(len(exp))
+                assert np.array_equal(g, exp), (name, int(q), k)
+        finally:
+            ds.close()
+    sizes = np.array(sizes)
+    assert (sizes <= 4096).any() and ((sizes > 4096) & (sizes <= 8192)).any() and (sizes > 8192).any()
