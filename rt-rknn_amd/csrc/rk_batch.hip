@@ -37,36 +37,41 @@ rk_status build_graph(rk_batch* b)
 
 } // namespace
 
-extern "C" {
+namespace rk {
 
-rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, rk_batch** out)
+rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
 {
-    if (!ds || !out || (nq && !qs)) return RK_ERR_INVALID_ARG;
-    *out = nullptr;
+    rk_dataset* ds = b->ds;
     rk_ctx* c = ds->ctx;
     if (nq > (1u << 24)) return fail(c, RK_ERR_INVALID_ARG, "too many queries in one batch");
-    std::unique_ptr<rk_batch> b(new rk_batch());
-    b->ds = ds;
     b->nq = (uint32_t)nq;
-    b->hq.resize(nq);
+    b->ran = false;
+    b->hq.assign(nq, QDev{});
     std::vector<uint32_t> all;
     std::vector<double2> wb(nq);
-    std::vector<uint32_t> cand, cl;
-    const double tf0 = now_s();
-    double filt = 0.0;
+    std::vector<uint32_t> cl;
+    // influence-zone filter for queries without a candidate list, on the
+    // host's cores
+    std::vector<uint32_t> fq, fk, fslot(nq, UINT32_MAX);
     for (size_t i = 0; i < nq; ++i) {
         const rk_query_desc& d = qs[i];
         if (d.q >= ds->nf) return fail(c, RK_ERR_INVALID_ARG, "query index q must be < number of facilities");
         if (d.k == 0) return fail(c, RK_ERR_INVALID_ARG, "k must be a positive integer");
-        rk_status s;
         if (d.cand == nullptr && d.nc == 0) {
-            const double f0 = now_s();
-            ds->index->filter(d.q, d.k, cand);
-            filt += now_s() - f0;
-            s = clean_candidates(ds, d.q, cand.data(), cand.size(), cl);
-        } else {
-            s = clean_candidates(ds, d.q, d.cand, d.nc, cl);
+            fslot[i] = (uint32_t)fq.size();
+            fq.push_back(d.q);
+            fk.push_back(d.k);
         }
+    }
+    std::vector<std::vector<uint32_t>> fc;
+    const double f0 = now_s();
+    if (!fq.empty()) filter_many(*ds->index, fq.data(), fk.data(), fq.size(), fc);
+    const double filt = now_s() - f0;
+    for (size_t i = 0; i < nq; ++i) {
+        const rk_query_desc& d = qs[i];
+        const rk_status s = fslot[i] != UINT32_MAX
+                                ? clean_candidates(ds, d.q, fc[fslot[i]].data(), fc[fslot[i]].size(), cl)
+                                : clean_candidates(ds, d.q, d.cand, d.nc, cl);
         if (s != RK_OK) return s;
         QDev& h = b->hq[i];
         h.qx = ds->hfx[d.q];
@@ -80,7 +85,6 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
         wb[i] = make_double2(wx, wy);
         all.insert(all.end(), cl.begin(), cl.end());
     }
-    (void)tf0;
     b->filter_s = filt;
     b->total_cand = all.size();
 
@@ -122,8 +126,25 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
     v.qoff = b->qoff.p;
     v.out = b->out.p;
     v.id_bits = ds->id_bits;
+    return RK_OK;
+}
+
+
+} // namespace rk
+
+extern "C" {
+
+rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, rk_batch** out)
+{
+    if (!ds || !out || (nq && !qs)) return RK_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::unique_ptr<rk_batch> b(new rk_batch());
+    b->ds = ds;
+    rk_status s = batch_fill(b.get(), qs, nq);
+    if (s != RK_OK) return s;
     if (ds->nu && nq) {
-        rk_status s = build_graph(b.get());
+        std::lock_guard<std::recursive_mutex> lk(ds->ctx->mu);
+        s = build_graph(b.get());
         if (s != RK_OK) return s;
     }
     *out = b.release();
@@ -138,12 +159,17 @@ rk_status rk_batch_run(rk_batch* b, double* device_seconds)
     RK_HIP(c, hipSetDevice(c->device));
     b->ran = true;
     b->overflow = false;
-    if (!b->exec) {
+    const bool direct = !b->exec && b->direct && b->ds->nu && b->nq;
+    if (!b->exec && !direct) {
         if (device_seconds) *device_seconds = 0.0;
         return RK_OK;
     }
     RK_HIP(c, hipEventRecord(c->ev0, c->stream));
-    RK_HIP(c, hipGraphLaunch(b->exec, c->stream));
+    if (direct)
+        launch_batch_query(b->ds->view, b->view, c->stream);
+    else
+        RK_HIP(c, hipGraphLaunch(b->exec, c->stream));
+    RK_HIP(c, hipGetLastError());
     RK_HIP(c, hipEventRecord(c->ev1, c->stream));
     RK_HIP(c, hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost,
                              c->stream));
@@ -167,10 +193,18 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
     if (b->overflow) return fail(c, RK_ERR_CAPACITY, "the last run overflowed a result segment; split the batch");
     const uint32_t nq = b->nq;
     std::vector<uint64_t> qoff(nq + 1, 0);
-    if (b->exec) {
+    const bool dev_ran = b->exec || (b->direct && ds->nu && nq);
+    if (dev_ran) {
         launch_batch_scan(b->view, c->stream);
         RK_HIP(c, hipGetLastError());
-        RK_HIP(c, hipMemcpy(qoff.data(), b->qoff.p, (nq + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        // on the context's (non-blocking) stream: a null-stream copy would not
+        // wait for the scan
+        RK_HIP(c, hipMemcpyAsync(qoff.data(), b->qoff.p, (nq + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+        RK_HIP(c, hipStreamSynchronize(c->stream));
+        for (uint32_t i = 0; i < nq; ++i)
+            if (qoff[i + 1] - qoff[i] != std::min(b->h_qcount.p[i], b->view.seg_cap))
+                return fail(c, RK_ERR_INTERNAL, "result offsets disagree with the per-query counts");
     }
     uint64_t total = 0;
     for (uint32_t i = 0; i < nq; ++i) {
@@ -180,7 +214,7 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
     offsets[nq] = total;
     if (total > cap) return fail(c, RK_ERR_CAPACITY, "output buffer too small; offsets[nq] holds the size needed");
     std::vector<int32_t> dev;
-    if (b->exec && qoff[nq]) {
+    if (dev_ran && qoff[nq]) {
         RK_HIP(c, b->out.reserve(qoff[nq]));
         b->view.out = b->out.p; // compaction is outside the graph, so the pointer may change
         launch_batch_compact(b->view, c->stream);
@@ -210,14 +244,17 @@ rk_status rk_batch_stats(const rk_batch* b, uint64_t* items, size_t n_items_cap,
     const size_t nl = b->ds->view.n_levels;
     if (n_levels) *n_levels = nl;
     std::vector<uint32_t> st;
-    if (b->exec) {
+    const bool dev_ran = b->exec || (b->direct && b->ds->nu && b->nq);
+    if (dev_ran) {
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         RK_HIP(c, hipSetDevice(c->device));
         st.resize((size_t)b->nq * kQueryStats);
-        RK_HIP(c, hipMemcpy(st.data(), b->qstat.p, st.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        RK_HIP(c, hipMemcpyAsync(st.data(), b->qstat.p, st.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+        RK_HIP(c, hipStreamSynchronize(c->stream));
     }
     uint64_t lv[kMaxLevels] = {}, rech = 0, res = 0;
-    for (uint32_t i = 0; b->exec && i < b->nq; ++i) {
+    for (uint32_t i = 0; dev_ran && i < b->nq; ++i) {
         const uint32_t* s = st.data() + (size_t)i * kQueryStats;
         rech += s[0];
         for (size_t l = 0; l < nl; ++l) lv[l] += s[1 + l];

@@ -29,10 +29,13 @@
 #include "rk_filter.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <limits>
 #include <numeric>
 #include <queue>
+#include <thread>
 
 namespace rk {
 
@@ -291,6 +294,35 @@ void FacilityIndex::filter(uint32_t q, uint32_t k, std::vector<uint32_t>& out, F
         st->candidates = out.size();
         st->radius = R;
     }
+}
+
+unsigned filter_threads(size_t n, unsigned threads)
+{
+    if (threads == 0) {
+        const char* env = std::getenv("RK_FILTER_THREADS");
+        threads = env ? (unsigned)std::max(1, std::atoi(env)) : std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    }
+    // a thread per 8 queries at least: below that the spawn costs more than it saves
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(threads, n / 8));
+}
+
+void filter_many(const FacilityIndex& idx, const uint32_t* q, const uint32_t* k, size_t n,
+                 std::vector<std::vector<uint32_t>>& out, unsigned threads)
+{
+    out.resize(n);
+    const unsigned t = filter_threads(n, threads);
+    if (t <= 1) {
+        for (size_t i = 0; i < n; ++i) idx.filter(q[i], k[i], out[i]);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        for (size_t i; (i = next.fetch_add(1)) < n;) idx.filter(q[i], k[i], out[i]);
+    };
+    std::vector<std::thread> pool;
+    for (unsigned j = 1; j < t; ++j) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
 }
 
 } // namespace rk

@@ -48,4 +48,13 @@ private:
     double ex_[kSectors + 1], ey_[kSectors + 1]; // sector boundary directions (ex_[S] = ex_[0])
 };
 
+// Filters queries (q[i], k[i]) into out[i] on up to `threads` host threads
+// (0: the machine's cores, at most 16, or RK_FILTER_THREADS).  Each query is
+// filtered exactly as by FacilityIndex::filter.
+void filter_many(const FacilityIndex& idx, const uint32_t* q, const uint32_t* k, size_t n,
+                 std::vector<std::vector<uint32_t>>& out, unsigned threads = 0);
+
+// Threads filter_many uses for n queries when asked for `threads` (0: default).
+unsigned filter_threads(size_t n, unsigned threads = 0);
+
 } // namespace rk

@@ -142,6 +142,7 @@ struct rk_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool ran = false;
+    bool direct = false;         // run launches the kernels without a graph (refilled batches)
     bool overflow = false;       // the last run had a query beyond its result segment
     double filter_s = 0.0;
     ~rk_batch()
@@ -157,6 +158,9 @@ rk_status fail_hip(rk_ctx* c, hipError_t e, const char* what, int line);
 // candidate list cleaning shared by every form: validates indices, drops q
 // and facilities at q's exact position (never strictly closer)
 rk_status clean_candidates(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc, std::vector<uint32_t>& out);
+// (re)fills a batch with queries; device buffers only grow, so a batch
+// refilled step after step allocates nothing once warm
+rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq);
 } // namespace rk
 
 #define RK_HIP(ctx, call)                                                                                      \

@@ -873,11 +873,12 @@ void k_batch_query(DsDev ds, BatchDev b)
         const bool shared = Q.nc <= kRecLds;
         const double2 w = b.wbound[qi];
         if (shared) {
-            CandRec r[2];
-            float2 o[2];
-            uint32_t qd[2];
+            constexpr int kPer = (kRecLds + kQueryBlock - 1) / kQueryBlock; // records per thread
+            CandRec r[kPer];
+            float2 o[kPer];
+            uint32_t qd[kPer];
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int h = 0; h < kPer; ++h) {
                 const uint32_t j = tid + h * kQueryBlock;
                 r[h] = {0.f, 0.f, 0.f, 0.f};
                 o[h] = make_float2(0.f, 0.f);
@@ -890,9 +891,10 @@ void k_batch_query(DsDev ds, BatchDev b)
                     qd[h] = quadrant(r[h]);
                 }
             }
-            uint32_t rank[2] = {0, 0};
+            uint32_t rank[kPer];
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+            for (int h = 0; h < kPer; ++h) {
+                rank[h] = 0;
                 for (uint32_t g = 0; g < 4; ++g) { // wave-aggregated quadrant counts
                     const uint64_t mk = ballot(qd[h] == g);
                     if (qd[h] == g) rank[h] = rank_below(mk);
@@ -901,12 +903,13 @@ void k_batch_query(DsDev ds, BatchDev b)
                     pos = uniform(pos);
                     if (qd[h] == g) rank[h] += pos;
                 }
+            }
             __syncthreads();
             uint32_t start[5];
             start[0] = 0;
             for (int g = 0; g < 4; ++g) start[g + 1] = start[g] + ctl.qcnt[g];
 #pragma unroll
-            for (int h = 0; h < 2; ++h)
+            for (int h = 0; h < kPer; ++h)
                 if (qd[h] < 4) {
                     const uint32_t slot = start[qd[h]] + rank[h];
                     S.rec[slot] = r[h];

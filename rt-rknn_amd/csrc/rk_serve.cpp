@@ -3,11 +3,11 @@
 // No reference counterpart: every reference program answers one query per
 // process (rknn-inf-gpu/inf.cpp:631-658).  Here requests (dataset, q, k)
 // arrive from any thread.  Two stages run concurrently:
-//   * the host stage filters newly admitted requests (influence-zone
-//     candidates) and prepares the next device step while the current one
-//     runs, and
-//   * the device stage runs one batched step (rk_batch_*) over every request
-//     prepared so far for a dataset.
+//   * the host stage (one filter worker per host core, up to 16) filters
+//     newly admitted requests (influence-zone candidates) while the current
+//     device step runs, and
+//   * the device stage runs one batched step over every request filtered so
+//     far for a dataset, refilling that dataset's batch object in place.
 // Admission is by memory: each request reserves an estimate of the device
 // bytes it needs while in flight; a request waits until the reservation fits
 // the budget, and one that could never fit is rejected at submission.
@@ -57,8 +57,10 @@ struct rk_server {
     uint64_t next_ticket = 1;
     uint64_t reserved = 0;
     bool stop = false;
-    std::thread host_thread, dev_thread;
+    std::vector<std::thread> host_threads; // filter workers
+    std::thread dev_thread;
     std::condition_variable cv_dev;
+    std::map<rk_dataset*, std::unique_ptr<rk_batch>> batches; // device stage only
 };
 
 namespace {
@@ -89,6 +91,43 @@ void host_stage(rk_server* s)
         lk.lock();
         s->ready.push_back(std::move(r));
         s->cv_dev.notify_one();
+    }
+}
+
+// One device step over qd.  Each dataset keeps one batch object whose
+// device buffers only grow: refilled every step and launched without a
+// graph, it allocates nothing once warm.  A step over its result budget is
+// split by rk_query_batch.
+rk_status run_step(rk_server* s, rk_dataset* ds, const std::vector<rk_query_desc>& qd, std::vector<int32_t>& ids,
+                   std::vector<uint64_t>& off)
+{
+    std::unique_ptr<rk_batch>& b = s->batches[ds];
+    if (!b) {
+        b.reset(new rk_batch());
+        b->ds = ds;
+        b->direct = true;
+    }
+    rk_status st = rk::batch_fill(b.get(), qd.data(), qd.size());
+    if (st == RK_OK) st = rk_batch_run(b.get(), nullptr);
+    if (st == RK_OK) {
+        uint64_t total = 0; // the result size is known once the step ran
+        for (size_t i = 0; i < qd.size(); ++i) {
+            const rk::QDev& h = b->hq[i];
+            total += h.trivial ? (h.k > h.nc ? ds->nu : 0) : std::min(b->h_qcount.p[i], b->view.seg_cap);
+        }
+        ids.resize(std::max<uint64_t>(1, total));
+        return rk_batch_fetch(b.get(), ids.data(), ids.size(), off.data());
+    }
+    if (st != RK_ERR_CAPACITY) return st;
+    size_t cap = 1 << 20;
+    while (true) {
+        ids.resize(cap);
+        st = rk_query_batch(ds, qd.data(), qd.size(), ids.data(), ids.size(), off.data(), nullptr);
+        if (st == RK_ERR_CAPACITY && off[qd.size()] > cap) {
+            cap = off[qd.size()];
+            continue;
+        }
+        return st;
     }
 }
 
@@ -125,18 +164,8 @@ void dev_stage(rk_server* s)
             qd[i].nc = step[i].cand.size();
         }
         std::vector<uint64_t> off(step.size() + 1);
-        size_t cap = 1 << 20;
         std::vector<int32_t> ids;
-        rk_status st;
-        while (true) {
-            ids.resize(cap);
-            st = rk_query_batch(ds, qd.data(), qd.size(), ids.data(), ids.size(), off.data(), nullptr);
-            if (st == RK_ERR_CAPACITY && off[step.size()] > cap) {
-                cap = off[step.size()];
-                continue;
-            }
-            break;
-        }
+        rk_status st = run_step(s, ds, qd, ids, off);
         const double t_end = rk::now_s();
         lk.lock();
         for (size_t i = 0; i < step.size(); ++i) {
@@ -169,7 +198,8 @@ rk_status rk_serve_start(rk_ctx* ctx, const rk_serve_config* cfg, rk_server** ou
         if (cfg->memory_budget) s->cfg.memory_budget = cfg->memory_budget;
         if (cfg->max_wait_s >= 0) s->cfg.max_wait_s = cfg->max_wait_s;
     }
-    s->host_thread = std::thread(host_stage, s);
+    const unsigned nf = rk::filter_threads(1u << 20);
+    for (unsigned i = 0; i < nf; ++i) s->host_threads.emplace_back(host_stage, s);
     s->dev_thread = std::thread(dev_stage, s);
     *out = s;
     return RK_OK;
@@ -229,7 +259,8 @@ rk_status rk_serve_stop(rk_server* s)
     }
     s->cv.notify_all();
     s->cv_dev.notify_all();
-    if (s->host_thread.joinable()) s->host_thread.join();
+    for (auto& t : s->host_threads)
+        if (t.joinable()) t.join();
     if (s->dev_thread.joinable()) s->dev_thread.join();
     delete s;
     return RK_OK;
