@@ -4,13 +4,17 @@ counting launch, sync and readback per query), bit-exact sets.
 
 A step = one pass of the batched form over one batch of NQ queries whose
 candidate lists (from the host influence-zone filter) are already resident
-in HBM; the users are Morton-sorted and resident.  `value` counts queries/s
+in HBM; the users are Hilbert-sorted and resident.  `value` counts queries/s
 over exactly --steps timed steps.  The JSON line also carries:
   roofline      dense counting kernel (count-all: every user x every candidate,
                 4 fp32 flops per test) timed with HIP events on its stream
   cpu_baseline  the FP64 brute-force oracle (rknn-naive semantics) on a
                 bounded sample of the same workload, one host thread
+                (cpu_baseline_mt: the same on every host core)
   plain_*       the one-launch-per-query form on the same queries
+  mixed_k       the same step with k drawn uniformly from 1..50 per query
+  end_to_end    filter (host cores) + batched step + result download
+  serve         the serving loop under Poisson arrivals (rknn-serve-bench)
 """
 import argparse
 import json
@@ -48,6 +52,9 @@ def main():
     ap.add_argument("--roofline-queries", type=int, default=8)
     ap.add_argument("--cpu-queries", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-qps", default="50000,100000,200000")
+    ap.add_argument("--serve-seconds", type=float, default=0.5)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -94,6 +101,41 @@ def main():
     value = args.queries * args.steps / wall
     log(f"batched: {ms_step:.3f} ms/step (device {np.mean(dev) * 1e3:.3f} ms), {value:,.0f} queries/s")
 
+    # ---- mixed k (1..50 per query, the batched workload's k mix), same step
+    rng = np.random.default_rng(seed)
+    ks = rng.integers(1, 51, size=args.queries)
+    mcands = [ds.filter(q, int(k)) for q, k in zip(qs, ks)]
+    mb = ds.prepare_batch([(q, int(k), c) for q, k, c in zip(qs, ks, mcands)])
+    for _ in range(args.warmup):
+        mb.run()
+    mdev = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        mdev.append(mb.run())
+    mwall = time.perf_counter() - t0
+    mixed = {"qps": round(args.queries * args.steps / mwall, 1), "ms_per_step": round(mwall / args.steps * 1e3, 4),
+             "device_ms": round(float(np.mean(mdev)) * 1e3, 4),
+             "candidates_mean": round(float(np.mean([len(c) for c in mcands])), 2)}
+    mixed_res = mb.fetch()
+    mb.close()
+    for i in range(min(32, args.queries)):  # spot check against the plain form
+        if not np.array_equal(ds.query_plain(qs[i], int(ks[i]), mcands[i]), mixed_res[i]):
+            log("ERROR: mixed-k batched and plain result sets differ")
+            sys.exit(1)
+    log(f"mixed k 1..50: {mixed['ms_per_step']:.3f} ms/step (device {mixed['device_ms']:.3f} ms), "
+        f"{mixed['qps']:,.0f} queries/s, |C| mean {mixed['candidates_mean']}")
+
+    # ---- end to end: host filter on every core + batched step + download
+    t0 = time.perf_counter()
+    e2e_res = ds.query_batch([(q, args.k) for q in qs])
+    e2e_s = time.perf_counter() - t0
+    e2e_ok = all(np.array_equal(a, b) for a, b in zip(e2e_res, batched))
+    end_to_end = {"qps": round(args.queries / e2e_s, 1), "ms": round(e2e_s * 1e3, 3), "matches": bool(e2e_ok)}
+    log(f"end to end (filter + step + download): {args.queries / e2e_s:,.0f} queries/s; same sets: {e2e_ok}")
+    if not e2e_ok:
+        log("ERROR: end-to-end results differ from the batched step")
+        sys.exit(1)
+
     # ---- plain form: one counting launch + sync + readback per query
     n_plain = min(args.plain, args.queries)
     lat = []
@@ -132,6 +174,7 @@ def main():
 
     # ---- CPU baseline: FP64 brute-force oracle, bounded sample, 1 thread
     cpu = None
+    cpu_mt = None
     if not args.no_cpu:
         import oracle
 
@@ -151,6 +194,38 @@ def main():
         log(f"cpu oracle: {n_cpu / cpu_s:.3f} queries/s; GPU sets equal oracle: {ok}")
         if not ok:
             sys.exit(1)
+        # the same on every host core (one query per thread), larger sample
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        n_mt = min(args.queries, 2 * threads)
+        t0 = time.perf_counter()
+        ok_mt = True
+        for i in range(n_cpu, n_cpu + n_mt):
+            exp = oracle.rknn(ux, uy, uid, fx, fy, qs[i % args.queries], args.k)
+            ok_mt &= np.array_equal(exp, batched[i % args.queries])
+        mt_s = time.perf_counter() - t0
+        cpu_mt = {"value": round(n_mt / mt_s, 4), "unit": "queries/s", "cores": threads, "kind": "port",
+                  "sample": f"{n_mt} queries, users split over {threads} threads per query", "matches_gpu": bool(ok_mt)}
+        log(f"cpu oracle on {threads} threads: {n_mt / mt_s:.3f} queries/s; equal: {ok_mt}")
+        if not ok_mt:
+            sys.exit(1)
+
+    # ---- serving loop under Poisson arrivals (native load generator)
+    serve = None
+    sb = os.path.join(ROOT, "rt-rknn_amd", "bin", "rknn-serve-bench")
+    if not args.no_serve and os.path.exists(sb):
+        batch.close()  # free the step's buffers for the server's own
+        cmd = [sb, "-c", str(args.config), "-u", str(args.users), "-f", str(args.facilities), "-k", str(args.k),
+               "--fixed-k", "-s", str(args.serve_seconds), "--qps", args.serve_qps]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            log("serve bench failed:", r.stderr[-2000:])
+            sys.exit(1)
+        serve = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+        for pt in serve:
+            log(f"serve k={args.k}: offered {pt['offered_qps']:,.0f}/s -> {pt['achieved_qps']:,.0f}/s, "
+                f"p50 {pt['p50_ms']:.3f} ms, p99 {pt['p99_ms']:.3f} ms")
+        batch = None
 
     out = {
         "metric": "RkNN queries/s @10M users,k=10 batched vs plain; p99 latency; bit-exact sets",
@@ -163,7 +238,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 generator, config %d)" % args.config,
         "config": {"workload": f"batched RkNN, {args.users} users, {args.facilities} facilities, "
                                f"{args.queries} queries per step, k={args.k}",
@@ -177,9 +252,14 @@ def main():
         "filter_qps": round(args.queries / filter_s, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_mt": cpu_mt,
+        "mixed_k": mixed,
+        "end_to_end": end_to_end,
+        "serve": serve,
     }
     print(json.dumps(out), flush=True)
-    batch.close()
+    if batch is not None:
+        batch.close()
     ds.close()
     ctx.close()
 
