@@ -11,6 +11,8 @@
  *   3  off-by-one keep rule (count <= k keeps the user)
  *   4  dropped term         (dy*dy missing from D)
  *   5  wrong exclusion      (skips facility q+1 instead of q)
+ *   6  early prune          (oracle_tree.c: a node is pruned at k-1 covering
+ *                            candidates)
  */
 #include "oracle.h"
 
@@ -26,6 +28,7 @@
 static int g_threads = 1;
 
 void oracle_set_threads(int n) { g_threads = n < 1 ? 1 : (n > 256 ? 256 : n); }
+int oracle_get_threads(void) { return g_threads; }
 
 /* Squared distance exactly as the oracle defines it (rknn-naive/rknn.cpp:45-47
  * computes the same differences; the oracle keeps the square in binary64). */

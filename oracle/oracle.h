@@ -28,6 +28,10 @@
  *   oracle_rknn_naive_int : literal rknn-naive (float distance, sort by
  *     (dist,x,y,id)); pinned against the reference rknn-naive binary's
  *     counts on integer data with ties.
+ *   oracle_tree_rknn : the reference's tree-culled verification
+ *     (rknn-inf/inf.h:194-249); pinned by set equality with oracle_rknn on
+ *     random, tie-laden and clustered data, and by the reference rknn-inf
+ *     binary's counts where that binary is exact.
  */
 #ifndef RKNN_ORACLE_H
 #define RKNN_ORACLE_H
@@ -65,6 +69,26 @@ int64_t oracle_rknn_naive_int(const int32_t* ux, const int32_t* uy, const int32_
 
 /* Number of worker threads used by the functions above (1 = serial). */
 void oracle_set_threads(int n);
+int oracle_get_threads(void);
+
+/* ---- cpu-inf: the reference's tree-culled verification (oracle_tree.c) ----
+ * Users in a bulk-loaded R-tree (16 entries per node); a node is pruned when
+ * >= k candidates certainly contain its rectangle on their side; leaf users
+ * are counted with the oracle predicate.  Same sets as oracle_rknn on the
+ * same candidate list.  Pinned against oracle_rknn (tests/test_oracle.py). */
+typedef struct oracle_tree oracle_tree;
+oracle_tree* oracle_tree_build(const float* ux, const float* uy, const int32_t* uid, uint64_t nu);
+void oracle_tree_free(oracle_tree* t);
+/* Sorted ids of R(q,k) into out (nu entries, or NULL to count only);
+ * cand == NULL means C = F \ {f_q}.  Returns |R| or -1. */
+int64_t oracle_tree_rknn(const oracle_tree* t, const float* fx, const float* fy, uint64_t nf, uint32_t q,
+                         uint32_t k, const uint32_t* cand, uint64_t nc, int32_t* out);
+/* counts[i] = |R(q[i], k[i])| with candidates cands[off[i] .. off[i+1]) (or
+ * every facility when cands == NULL), queries spread over the worker
+ * threads, one query per thread at a time. */
+int oracle_tree_rknn_many(const oracle_tree* t, const float* fx, const float* fy, uint64_t nf, uint64_t nq,
+                          const uint32_t* q, const uint32_t* k, const uint64_t* off, const uint32_t* cands,
+                          int64_t* counts);
 
 #ifdef __cplusplus
 }

@@ -283,3 +283,71 @@ def test_mutants_fail(mutant):
     if not caught and O.ref_binary("rknn-naive") and O.ref_binary("rknn-inf"):
         caught = bool(run_reference(mutant, with_inf=False)[0])
     assert caught, f"planted mutant {mutant} passed every pin"
+
+
+# ------------------------------------------------- tree-culled verification
+
+def run_tree(mutant=0):
+    """oracle_tree_rknn (the reference's tree-culled verification, restated)
+    against the brute-force oracle: hand cases, tie-laden dyadic data with
+    users on bisectors, clustered data, with and without candidate lists."""
+    L = O.lib(mutant)
+    import ctypes as C
+
+    def tree_rknn(ux, uy, uid, fx, fy, q, k, cand=None):
+        ux, uy = np.ascontiguousarray(ux, np.float32), np.ascontiguousarray(uy, np.float32)
+        uid = np.ascontiguousarray(uid, np.int32)
+        fx, fy = np.ascontiguousarray(fx, np.float32), np.ascontiguousarray(fy, np.float32)
+        h = L.oracle_tree_build(O._p(ux, C.c_float), O._p(uy, C.c_float), O._p(uid, C.c_int32), len(ux))
+        out = np.empty(max(1, len(ux)), np.int32)
+        c = None if cand is None else np.ascontiguousarray(cand, np.uint32)
+        n = L.oracle_tree_rknn(h, O._p(fx, C.c_float), O._p(fy, C.c_float), len(fx), q, k,
+                               None if c is None else O._p(c, C.c_uint32), 0 if c is None else len(c),
+                               O._p(out, C.c_int32))
+        L.oracle_tree_free(h)
+        return out[:n].copy()
+
+    bad = []
+    for i, (U, F, q, k, exp) in enumerate(hand_cases()):
+        U, F = np.array(U, np.float32), np.array(F, np.float32)
+        got = tree_rknn(U[:, 0], U[:, 1], _ids(len(U), 100), F[:, 0], F[:, 1], q, k)
+        if list(got) != exp:
+            bad.append(("hand", i))
+    from rknn_amd import generate, generate_queries, default_seed  # generators only (host code)
+    for cfg, nu, nf in [(1, 3000, 300), (3, 20000, 800), (4, 20000, 800)]:
+        seed = default_seed(cfg)
+        ux, uy, uid, fx, fy, fid = generate(cfg, nu, nf, seed=seed, nq_ties=8, seed_ties=seed)
+        for j, q in enumerate(generate_queries(nf, 8, seed)):
+            k = (1, 3, 7, 25)[j % 4]
+            exp = O.rknn(ux, uy, uid, fx, fy, int(q), k)
+            if not np.array_equal(tree_rknn(ux, uy, uid, fx, fy, int(q), k), exp):
+                bad.append(("all", cfg, int(q), k))
+            sub = np.arange(0, nf, 3, dtype=np.uint32)  # an arbitrary candidate list
+            exp = O.rknn(ux, uy, uid, fx, fy, int(q), k, cand=sub)
+            if not np.array_equal(tree_rknn(ux, uy, uid, fx, fy, int(q), k, cand=sub), exp):
+                bad.append(("cand", cfg, int(q), k))
+    return bad
+
+
+def test_tree_matches_bruteforce():
+    assert run_tree() == []
+
+
+def test_tree_many_matches_single():
+    r = np.random.default_rng(4)
+    ux, uy, uid, fx, fy, fid = tiefree(9, nu=5000, nf=120)
+    t = O.Tree(ux, uy, uid)
+    qs = r.integers(0, len(fx), 24)
+    ks = r.integers(1, 20, 24)
+    cands = [np.arange(len(fx), dtype=np.uint32) for _ in qs]
+    exp = [len(t.rknn(fx, fy, int(q), int(k))) for q, k in zip(qs, ks)]
+    for threads in (1, 4):
+        O.set_threads(threads)
+        assert list(t.counts_many(fx, fy, qs, ks, cands)) == exp
+        assert list(t.counts_many(fx, fy, qs, ks)) == exp
+    O.set_threads(1)
+    t.close()
+
+
+def test_tree_mutant_fails():
+    assert run_tree(mutant=6) != []
