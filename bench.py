@@ -56,6 +56,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-qps", default="50000,100000,200000")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
+    ap.add_argument("--table", action="store_true",
+                    help="instead of the headline: one JSON line per config 1-3 (plain and batched forms, the CPU "
+                         "tree restatement and, where built, the reference's own rknn-inf)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -64,6 +67,9 @@ def main():
         sys.exit(2)
 
     import rknn_amd as R
+
+    if args.table:
+        return table(R)
 
     t0 = time.time()
     seed = R.default_seed(args.config)
@@ -280,6 +286,78 @@ def main():
         batch.close()
     ds.close()
     ctx.close()
+
+
+# configs 1-3: (users, facilities, queries, k values)
+TABLE = {1: (10_000, 1_000, 10, [5]), 2: (1_000_000, 10_000, 100, [10]), 3: (4_000_000, 50_000, 200, [1, 5, 10, 25])}
+
+
+def table(R):
+    """One JSON line per config 1-3: the plain form (one launch per query, with its host filter), the batched
+    form over the config's queries, the CPU tree restatement on one thread, and the reference's own rknn-inf
+    binary (oracle/_ref, built from its sources here) on a few queries of configs 1-2, with its printed times."""
+    import tempfile
+
+    import oracle
+
+    for cfg, (nu, nf, nq, kset) in TABLE.items():
+        seed = R.default_seed(cfg)
+        ux, uy, uid, fx, fy, fid = R.generate(cfg, nu, nf, seed=seed, nq_ties=nq, seed_ties=seed)
+        qs = [int(q) for q in R.generate_queries(nf, nq, seed)]
+        ks = [kset[i % len(kset)] for i in range(nq)]
+        ctx = R.Context(0)
+        ds = R.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+        cands = [ds.filter(q, k) for q, k in zip(qs, ks)]
+        for q, k, c in zip(qs[:4], ks, cands):  # warm
+            ds.query_plain(q, k, c)
+        lat, plain = [], []
+        for q, k, c in zip(qs, ks, cands):
+            t0 = time.perf_counter()
+            plain.append(ds.query_plain(q, k, c))
+            lat.append(time.perf_counter() - t0)
+        b = ds.prepare_batch([(q, k, c) for q, k, c in zip(qs, ks, cands)])
+        for _ in range(3):
+            b.run()
+        dev = [b.run() for _ in range(10)]
+        batched = b.fetch()
+        b.close()
+        same = all(np.array_equal(a, c) for a, c in zip(plain, batched))
+        tree = oracle.Tree(ux, uy, uid)
+        oracle.set_threads(1)
+        t0 = time.perf_counter()
+        cnt = tree.counts_many(fx, fy, qs, ks, cands)
+        cpu_s = time.perf_counter() - t0
+        same &= all(int(n) == len(r) for n, r in zip(cnt, plain))
+        tree.close()
+        row = {"config": cfg, "users": nu, "facilities": nf, "queries": nq, "k": sorted(set(ks)),
+               "candidates_mean": round(float(np.mean([len(c) for c in cands])), 2),
+               "plain_qps": round(nq / sum(lat), 1),
+               "plain_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 4),
+               "plain_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 4),
+               "batched_qps": round(nq / float(np.mean(dev)), 1), "batched_device_ms": round(float(np.mean(dev)) * 1e3, 4),
+               "cpu_tree_qps_1thread": round(nq / cpu_s, 1), "sets_equal": bool(same)}
+        ref = oracle.ref_binary("rknn-inf")
+        if ref and cfg in (1, 2):
+            with tempfile.TemporaryDirectory() as td:
+                path = os.path.join(td, "data.txt")
+                R.write_text(path, ux, uy, uid, fx, fy, fid)
+                tot, found, build = [], [], []
+                for q, k in list(zip(qs, ks))[:3]:
+                    out = subprocess.run([ref, "-if", path, "-k", str(k), "-q", str(q)], capture_output=True,
+                                         text=True, timeout=600).stdout
+                    for line in out.splitlines():
+                        if line.startswith("Total time:"):
+                            tot.append(float(line.split()[2].split("[")[0]))
+                        elif line.startswith("Found "):
+                            found.append(int(line.split()[1]))
+                        elif line.startswith("R*-tree is built in"):
+                            build.append(float(line.split()[4].split("[")[0]))
+                row["ref_rknn_inf"] = {"queries": len(tot), "total_s_mean": round(float(np.mean(tot)), 6) if tot else None,
+                                       "tree_build_s": round(float(np.mean(build)), 3) if build else None,
+                                       "found": found, "ours": [len(r) for r in plain[: len(found)]]}
+        ds.close()
+        ctx.close()
+        print(json.dumps(row), flush=True)
 
 
 if __name__ == "__main__":
