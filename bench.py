@@ -103,7 +103,8 @@ def main():
     wall = time.perf_counter() - t0
     batched = batch.fetch()
     items, n_res, n_rech = batch.stats()
-    log(f"batched work per step: items per level (leaf first) {items}, results {n_res}, fp64 re-checks {n_rech}")
+    log(f"batched work per step: items per level (leaf first) {items}, results {n_res}, fp64 re-checks {n_rech}; "
+        f"autotuned block: {batch.waves_per_query()} waves per query")
     ms_step = wall / args.steps * 1e3
     value = args.queries * args.steps / wall
     log(f"batched: {ms_step:.3f} ms/step (device {np.mean(dev) * 1e3:.3f} ms), {value:,.0f} queries/s")

@@ -135,18 +135,25 @@ typedef struct rk_query_desc {
 rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq,
                          int32_t* out_ids, size_t cap, uint64_t* offsets, rk_timing* t);
 
-/* Split form for resident repeat runs: prepare uploads the candidate lists
- * and captures the hipGraph; run replays it (results stay in HBM); fetch
- * copies them out. */
+/* Split form for resident repeat runs: prepare filters (when cand is NULL),
+ * uploads the candidate lists, picks the traversal's block shape (autotuned
+ * per workload class: the first batch of a class times each shape on its
+ * own data; RK_AUTOTUNE=0 disables, =8 or =4 forces that many waves per
+ * query, RK_TUNE_FILE names the JSON cache) and
+ * captures the hipGraph; run replays it (results stay in HBM); fetch copies
+ * them out. */
 rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, rk_batch** out);
 rk_status rk_batch_run(rk_batch* b, double* device_seconds);
 rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets);
 rk_status rk_batch_destroy(rk_batch* b);
-/* Work counters of the last run: items[l] = (query, node) items processed at
- * tree level l (0 = leaf tiles; n_levels+1 entries, at most 9), results
- * appended, users re-counted in binary64. */
+/* Work counters of the last run: items[l] = tree nodes of level l that
+ * entered a query's frontier (0 = leaf tiles tested; n_levels entries, at
+ * most 8), result ids, (user, candidate) tests settled in binary64. */
 rk_status rk_batch_stats(const rk_batch* b, uint64_t* items, size_t n_items_cap, size_t* n_levels,
                          uint64_t* results, uint64_t* rechecks);
+/* The block shape the autotuner chose: waves per query (8 or 4).  No
+ * reference counterpart. */
+rk_status rk_batch_config(const rk_batch* b, uint32_t* waves_per_query);
 
 /* ---- serving: continuous batching over resident datasets. ---- */
 typedef struct rk_serve_config {

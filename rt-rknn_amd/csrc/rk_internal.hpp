@@ -143,6 +143,7 @@ struct rk_batch {
     hipGraphExec_t exec = nullptr;
     bool ran = false;
     bool direct = false;         // run launches the kernels without a graph (refilled batches)
+    int shape = 0;               // traversal block shape (rk::kShape*), set by the autotuner
     bool overflow = false;       // the last run had a query beyond its result segment
     double filter_s = 0.0;
     ~rk_batch()
@@ -161,6 +162,9 @@ rk_status clean_candidates(rk_dataset* ds, uint32_t q, const uint32_t* cand, siz
 // (re)fills a batch with queries; device buffers only grow, so a batch
 // refilled step after step allocates nothing once warm
 rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq);
+// picks the traversal block shape for a filled batch (autotuner); caller
+// holds the context lock
+rk_status batch_tune(rk_batch* b);
 } // namespace rk
 
 #define RK_HIP(ctx, call)                                                                                      \

@@ -449,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_level_mbr(const float4* __restrict__
 
 // -------------------------------------------------------------- batched form
 //
-// One block of kQWaves waves per query.  The block builds the query's
+// One block of kW (8 or 4) waves per query.  The block builds the query's
 // candidate records, walks the bounding-rectangle tree breadth-first
 // through per-level frontiers in LDS, tests the surviving tiles and appends
 // kept users to the query's result segment, which it then radix-sorts (a
@@ -471,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void k_level_mbr(const float4* __restrict__
 // the node test is then two FMAs, a compare and an add per (child,
 // candidate).
 
-constexpr uint32_t kQWaves = kQueryBlock / 64;
+constexpr uint32_t kQWaves = kQueryBlock / 64; // waves of the largest block shape
 constexpr uint32_t kPhaseItems = kFront / kFanout;
 constexpr uint32_t kRecLds = 1024; // shared-record capacity of the batched form
 
@@ -588,7 +588,7 @@ __device__ __forceinline__ uint32_t* frontier(uint32_t* front, uint32_t l) { ret
 
 // The descent of one query (all threads of the block call it).  The first
 // phase expands a virtual root whose children are the top-level nodes.
-template <bool kShared>
+template <uint32_t kW, bool kShared>
 __device__ void descend(const DsDev& ds, const BatchDev& b, const QDev& Q, uint32_t qi, QueryCtl& ctl,
                         uint32_t* front, uint32_t* cover, const CandRec* __restrict__ rec,
                         const float2* __restrict__ orig, WaveLds* L)
@@ -627,12 +627,12 @@ __device__ void descend(const DsDev& ds, const BatchDev& b, const QDev& Q, uint3
                 ux = ds.ux[tile * kTile + lane];
                 uy = ds.uy[tile * kTile + lane];
             }
-            for (; i < n; i += kQWaves) {
+            for (; i < n; i += kW) {
                 uint32_t tn = 0;
                 float4 mn = make_float4(0.f, 0.f, 0.f, 0.f);
                 float uxn = 0.f, uyn = 0.f;
-                if (i + kQWaves < n) {
-                    tn = uniform(fl[i + kQWaves]);
+                if (i + kW < n) {
+                    tn = uniform(fl[i + kW]);
                     mn = ds.mbr[0][tn];
                     uxn = ds.ux[tn * kTile + lane];
                     uyn = ds.uy[tn * kTile + lane];
@@ -658,13 +658,13 @@ __device__ void descend(const DsDev& ds, const BatchDev& b, const QDev& Q, uint3
             // that level's frontier; S waves per node, one candidate slice each
             end = min(n, c + kPhaseItems);
             const uint32_t items = end - c;
-            const uint32_t S = max(1u, kQWaves / items);
+            const uint32_t S = max(1u, kW / items);
             const uint32_t per = ((Q.nc + S - 1) / S + 15) & ~15u;
             const uint32_t cl = l - 1;
             const float4* cmbr = ds.mbr[cl];
             const uint32_t n_child = ds.nlev[cl];
             uint32_t* dst = frontier(front, cl);
-            for (uint32_t w = wave; w < items * S; w += kQWaves) {
+            for (uint32_t w = wave; w < items * S; w += kW) {
                 const uint32_t it = w / S, sl = w % S;
                 const uint32_t node = root ? 0u : uniform(fl[c + it]);
                 const uint32_t lo = min(Q.nc, sl * per), hi = min(Q.nc, lo + per);
@@ -676,7 +676,7 @@ __device__ void descend(const DsDev& ds, const BatchDev& b, const QDev& Q, uint3
             }
             if (S > 1) {
                 __syncthreads();
-                for (uint32_t it = wave; it < items; it += kQWaves) {
+                for (uint32_t it = wave; it < items; it += kW) {
                     const uint32_t node = root ? 0u : uniform(fl[c + it]);
                     const uint32_t full = cover[it * 64 + lane];
                     cover[it * 64 + lane] = 0;
@@ -820,9 +820,9 @@ __device__ bool bucket_sort(int32_t* __restrict__ seg, uint32_t n, uint32_t* out
     return true;
 }
 
-// Segments of up to kQuerySortMax ids are sorted by the block that wrote
-// them; its candidate area (free once the descent ends) holds the keys and
-// the bucket counts.
+// Segments of up to 8 ids per thread (kQuerySortMax for the largest block)
+// are sorted by the block that wrote them; its candidate area (free once the
+// descent ends) holds the keys and the bucket counts.
 constexpr uint32_t kQuerySortMax = kQueryBlock * 8;
 constexpr uint32_t kQuerySortBuckets = 2048;
 static_assert((kQuerySortMax + kQuerySortBuckets) * sizeof(uint32_t) <= kCandBytes, "sort storage");
@@ -842,11 +842,16 @@ __device__ void sort_segment(int32_t* seg, uint32_t n, uint32_t id_bits, typenam
         if (base + i < n) seg[base + i] = keys[i];
 }
 
-// Eight waves per SIMD: four 8-wave blocks per CU, so a batch of 1024
-// queries is resident at once on 256 CUs.
-__global__ __launch_bounds__(kQueryBlock) __attribute__((amdgpu_waves_per_eu(8, 8)))
+// kW waves per query.  With 8-wave blocks at eight waves per SIMD (<= 64
+// VGPRs), a batch of 1024 queries is resident at once on 256 CUs; 4-wave
+// blocks may use twice the registers.  The autotuner picks the shape per
+// workload.
+template <uint32_t kW>
+__global__ __launch_bounds__(kW * 64) __attribute__((amdgpu_waves_per_eu(kW, 8)))
 void k_batch_query(DsDev ds, BatchDev b)
 {
+    constexpr uint32_t kThreads = kW * 64;
+    constexpr uint32_t kSortMax = kThreads * 8; // segment ids sorted in the block
     extern __shared__ __align__(16) unsigned char smem[];
     QueryCtl& ctl = *reinterpret_cast<QueryCtl*>(smem);
     uint32_t* ctl_words = reinterpret_cast<uint32_t*>(smem);
@@ -857,7 +862,7 @@ void k_batch_query(DsDev ds, BatchDev b)
     uint32_t* front = reinterpret_cast<uint32_t*>(cand + kCandBytes);
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
 
-    for (uint32_t i = tid; i < kPhaseItems * 64; i += kQueryBlock) cover[i] = 0;
+    for (uint32_t i = tid; i < kPhaseItems * 64; i += kThreads) cover[i] = 0;
     for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
         const QDev Q = uniform_q(b.q[qi]);
         uint32_t* stat = b.qstat + (uint64_t)qi * kQueryStats;
@@ -873,13 +878,13 @@ void k_batch_query(DsDev ds, BatchDev b)
         const bool shared = Q.nc <= kRecLds;
         const double2 w = b.wbound[qi];
         if (shared) {
-            constexpr int kPer = (kRecLds + kQueryBlock - 1) / kQueryBlock; // records per thread
+            constexpr int kPer = (kRecLds + kThreads - 1) / kThreads; // records per thread
             CandRec r[kPer];
             float2 o[kPer];
             uint32_t qd[kPer];
 #pragma unroll
             for (int h = 0; h < kPer; ++h) {
-                const uint32_t j = tid + h * kQueryBlock;
+                const uint32_t j = tid + h * kThreads;
                 r[h] = {0.f, 0.f, 0.f, 0.f};
                 o[h] = make_float2(0.f, 0.f);
                 qd[h] = 4;
@@ -919,7 +924,7 @@ void k_batch_query(DsDev ds, BatchDev b)
         } else {
             CandRec* grec = b.rec + Q.off;
             float2* gorig = b.orig + Q.off;
-            for (uint32_t j = tid; j < Q.nc; j += kQueryBlock) {
+            for (uint32_t j = tid; j < Q.nc; j += kThreads) {
                 const uint32_t f = b.cand[Q.off + j];
                 const float x = b.fx[f], y = b.fy[f];
                 grec[j] = make_record(Q.qx, Q.qy, x, y, w.x, w.y);
@@ -928,9 +933,9 @@ void k_batch_query(DsDev ds, BatchDev b)
         }
         __syncthreads();
         if (shared)
-            descend<true>(ds, b, Q, qi, ctl, front, cover, S.rec, S.orig, &wl[wave]);
+            descend<kW, true>(ds, b, Q, qi, ctl, front, cover, S.rec, S.orig, &wl[wave]);
         else
-            descend<false>(ds, b, Q, qi, ctl, front, cover, b.rec + Q.off, b.orig + Q.off, &wl[wave]);
+            descend<kW, false>(ds, b, Q, qi, ctl, front, cover, b.rec + Q.off, b.orig + Q.off, &wl[wave]);
         __syncthreads();
         const uint32_t nres = ctl.nres;
         if (tid == 0) {
@@ -940,10 +945,10 @@ void k_batch_query(DsDev ds, BatchDev b)
         if (tid < kMaxLevels) stat[1 + tid] = ctl.items[tid];
         const uint32_t ns = min(nres, b.seg_cap);
         bool sorted = ns <= 1;
-        if (ns > 1 && ns <= kQuerySortMax) {
+        if (ns > 1 && ns <= kSortMax) {
             uint32_t* keys = reinterpret_cast<uint32_t*>(cand);
-            sorted = bucket_sort<kQueryBlock, kQuerySortMax / kQueryBlock, kQuerySortBuckets>(
-                b.seg + (uint64_t)qi * b.seg_cap, ns, keys, keys + kQuerySortMax, ctl.red);
+            sorted = bucket_sort<kThreads, 8, kQuerySortBuckets>(b.seg + (uint64_t)qi * b.seg_cap, ns, keys,
+                                                                  keys + kQuerySortMax, ctl.red);
         }
         if (tid == 0) stat[kSortSlot] = sorted ? 0u : 1u;
         __syncthreads(); // before the next query resets the state
@@ -1073,10 +1078,13 @@ size_t batch_query_lds_bytes(uint32_t n_levels)
     return kCtlBytes + kCoverBytes + kCandBytes + ((size_t)(n_levels - 1) * kFront + 64) * sizeof(uint32_t);
 }
 
-void launch_batch_query(const DsDev& ds, const BatchDev& b, hipStream_t st)
+void launch_batch_query(const DsDev& ds, const BatchDev& b, int shape, hipStream_t st)
 {
     const uint32_t grid = b.nq < 65535 ? (b.nq ? b.nq : 1) : 65535;
-    k_batch_query<<<grid, kQueryBlock, batch_query_lds_bytes(ds.n_levels), st>>>(ds, b);
+    if (shape == kShape4Waves)
+        k_batch_query<4><<<grid, 256, batch_query_lds_bytes(ds.n_levels), st>>>(ds, b);
+    else
+        k_batch_query<8><<<grid, 512, batch_query_lds_bytes(ds.n_levels), st>>>(ds, b);
     k_batch_sort<<<grid < 1024 ? grid : 1024, kSortBlock, 0, st>>>(b);
 }
 

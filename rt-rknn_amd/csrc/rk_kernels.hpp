@@ -12,7 +12,7 @@ namespace rk {
 // every wave would serialise thousands of device atomics.
 constexpr int kBuckets = 64;
 
-// Batched form: one block of kQueryBlock threads per query; tree frontiers
+// Batched form: one block of up to kQueryBlock threads per query; tree frontiers
 // of kFront nodes per level in LDS; result segments of up to kDeviceSortMax
 // ids are sorted on the device, larger ones by the host when fetched.
 constexpr int kQueryBlock = 512;
@@ -67,7 +67,11 @@ void launch_scatter_u32(const uint32_t* src_sorted, const uint32_t* perm, uint32
 // segments of up to 4096 ids) and the large-segment sort; capture them in a
 // graph
 size_t batch_query_lds_bytes(uint32_t n_levels);
-void launch_batch_query(const DsDev& ds, const BatchDev& b, hipStream_t st);
+// block shapes of the traversal kernel (waves per query), chosen by the autotuner
+constexpr int kShape8Waves = 0;
+constexpr int kShape4Waves = 1;
+constexpr int kNumShapes = 2;
+void launch_batch_query(const DsDev& ds, const BatchDev& b, int shape, hipStream_t st);
 void launch_batch_scan(const BatchDev& b, hipStream_t st);    // offsets of the compacted layout
 void launch_batch_compact(const BatchDev& b, hipStream_t st); // segments -> compacted layout
 

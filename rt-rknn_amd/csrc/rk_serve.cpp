@@ -108,6 +108,10 @@ rk_status run_step(rk_server* s, rk_dataset* ds, const std::vector<rk_query_desc
         b->direct = true;
     }
     rk_status st = rk::batch_fill(b.get(), qd.data(), qd.size());
+    if (st == RK_OK) {
+        std::lock_guard<std::recursive_mutex> lk(ds->ctx->mu);
+        st = rk::batch_tune(b.get());
+    }
     if (st == RK_OK) st = rk_batch_run(b.get(), nullptr);
     if (st == RK_OK) {
         uint64_t total = 0; // the result size is known once the step ran

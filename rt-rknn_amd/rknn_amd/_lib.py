@@ -84,6 +84,7 @@ SIGNATURES = {
     "rk_batch_fetch": (ST, [P, I32P, C.c_size_t, U64P]),
     "rk_batch_destroy": (ST, [P]),
     "rk_batch_stats": (ST, [P, U64P, C.c_size_t, SZP, U64P, U64P]),
+    "rk_batch_config": (ST, [P, U32P]),
     "rk_serve_start": (ST, [P, C.POINTER(rk_serve_config), C.POINTER(P)]),
     "rk_serve_submit": (ST, [P, P, C.c_uint32, C.c_uint32, U64P]),
     "rk_serve_poll": (ST, [P, C.c_uint64, I32P, C.c_size_t, SZP, C.POINTER(C.c_double)]),

@@ -209,6 +209,12 @@ class Batch:
                                                    C.byref(rech)))
         return items[: nl.value].tolist(), res.value, rech.value
 
+    def waves_per_query(self):
+        """Block shape the autotuner chose for the traversal kernel."""
+        w = C.c_uint32()
+        self.ds.ctx.check(self._lib.rk_batch_config(self.h, C.byref(w)))
+        return w.value
+
     def close(self):
         if self.h:
             self._lib.rk_batch_destroy(self.h)

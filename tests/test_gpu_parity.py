@@ -212,3 +212,41 @@ def test_batch_sort_paths(rk, ctx):
             ds.close()
     sizes = np.array(sizes)
     assert (sizes <= 4096).any() and ((sizes > 4096) & (sizes <= 8192)).any() and (sizes > 8192).any()
+
+
+def test_autotuned_shapes_agree(rk, ctx, tmp_path, monkeypatch):
+    """Every block shape of the traversal kernel returns the oracle's sets;
+    the autotuner times them on the batch, records its choice in the JSON
+    cache and reuses it."""
+    seed = rk.default_seed(4)
+    nu, nf, nq = 200_000, 5_000, 96
+    ux, uy, uid, fx, fy, fid = rk.generate(4, nu, nf, seed=seed, nq_ties=nq, seed_ties=seed)
+    qs = [int(q) for q in rk.generate_queries(nf, nq, seed)]
+    ks = [1 + (7 * i) % 50 for i in range(nq)]
+    ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    O.set_threads(16)
+    try:
+        exp = [O.rknn(ux, uy, uid, fx, fy, q, k, cand=ds.filter(q, k)) for q, k in zip(qs[:12], ks)]
+        for shape in ("8", "4"):
+            monkeypatch.setenv("RK_AUTOTUNE", shape)
+            b = ds.prepare_batch([(q, k) for q, k in zip(qs, ks)])
+            assert b.waves_per_query() == int(shape)
+            b.run()
+            got = b.fetch()
+            b.close()
+            for g, e in zip(got, exp):
+                assert np.array_equal(g, e), shape
+        monkeypatch.delenv("RK_AUTOTUNE")
+        cache = tmp_path / "autotune.json"
+        monkeypatch.setenv("RK_TUNE_FILE", str(cache))
+        b = ds.prepare_batch([(q, k) for q, k in zip(qs, ks)])
+        w = b.waves_per_query()
+        assert w in (4, 8)
+        b.run()
+        for g, e in zip(b.fetch(), exp):
+            assert np.array_equal(g, e)
+        b.close()
+        text = cache.read_text()
+        assert '"waves_per_query": %d' % w in text and "/q128/" in text
+    finally:
+        ds.close()
