@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -95,6 +96,25 @@ rk_status upload_records(rk_dataset* ds, uint32_t q, const std::vector<uint32_t>
         RK_HIP(c, hipMemcpyAsync(ds->orig.p, ds->h_orig.p, n * sizeof(float2), hipMemcpyHostToDevice, c->stream));
     }
     return RK_OK;
+}
+
+// Count-all kernel class: device and candidates (power of two).
+std::string count_key(rk_dataset* ds, size_t nc)
+{
+    uint64_t p = 1;
+    while (p < nc && p < (1ull << 30)) p <<= 1;
+    return device_arch(ds->ctx) + "/count/c" + std::to_string(p);
+}
+
+// RK_COUNT_KERNEL=valu|mfma forces a variant; otherwise the autotuner's
+// choice for the class, or -1 when the class has not been timed yet.
+int count_variant(rk_dataset* ds, size_t nc)
+{
+    if (const char* e = std::getenv("RK_COUNT_KERNEL")) {
+        if (std::strcmp(e, "valu") == 0) return kCountValu;
+        if (std::strcmp(e, "mfma") == 0) return kCountMfma;
+    }
+    return tuned_choice(count_key(ds, nc));
 }
 
 QDev make_qdev(rk_dataset* ds, uint32_t q, uint32_t k, size_t nc)
@@ -449,9 +469,31 @@ rk_status rk_count_all(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t 
     RK_HIP(c, ds->h_ctr.reserve(kPlainCounters));
     RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, kPlainCounters * sizeof(uint32_t), c->stream));
     QDev qd = make_qdev(ds, q, 1, cl.size());
+    const int grid = device_grid(c->n_cu, 32);
+    int variant = count_variant(ds, cl.size());
+    if (variant < 0) { // first call of this class: time both kernels on this query, keep the faster
+        float best = 0.f;
+        for (int v = 0; v < 2; ++v) {
+            float ms = 0.f;
+            for (int r = 0; r < 2; ++r) {
+                RK_HIP(c, hipEventRecord(c->ev0, c->stream));
+                launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, grid, v,
+                                 c->stream);
+                RK_HIP(c, hipGetLastError());
+                RK_HIP(c, hipEventRecord(c->ev1, c->stream));
+                RK_HIP(c, hipEventSynchronize(c->ev1));
+                RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            }
+            if (v == 0 || ms < best) {
+                best = ms;
+                variant = v;
+            }
+        }
+        tuned_store(count_key(ds, cl.size()), variant, variant == kCountMfma ? "matrix core" : "valu");
+        RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, kPlainCounters * sizeof(uint32_t), c->stream));
+    }
     RK_HIP(c, hipEventRecord(c->ev0, c->stream));
-    launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, device_grid(c->n_cu, 32),
-                     c->stream);
+    launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, grid, variant, c->stream);
     RK_HIP(c, hipGetLastError());
     RK_HIP(c, hipEventRecord(c->ev1, c->stream));
     launch_scatter_u32(ds->counts.p, ds->perm.p, (uint32_t)ds->nu, ds->counts_orig.p, c->stream);

@@ -9,9 +9,6 @@
 // (offsets scan + copy) outside the graph.
 #include <hip/hip_runtime.h>
 
-#include <sys/stat.h>
-#include <unistd.h>
-
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -43,36 +40,6 @@ rk_status build_graph(rk_batch* b)
     return RK_OK;
 }
 
-// ------------------------------------------------------------- autotuner
-//
-// The traversal kernel comes in block shapes (waves per query).  Which is
-// faster depends on the batch: how many queries share the chip, how many
-// candidates each has, k.  The first batch of a workload class times every
-// shape on its own data (results do not depend on the shape) and keeps the
-// fastest; the choice is remembered per class in the process and in a JSON
-// file (RK_TUNE_FILE, default $HOME/.cache/rknn_amd/autotune.json), so later
-// processes skip the timing.  RK_AUTOTUNE=0 keeps the default shape;
-// RK_AUTOTUNE=8 or 4 forces that many waves per query.
-
-struct TuneCache {
-    std::mutex mu;
-    bool loaded = false;
-    std::map<std::string, int> shape;
-};
-
-TuneCache& tune_cache()
-{
-    static TuneCache c;
-    return c;
-}
-
-std::string tune_path()
-{
-    if (const char* p = std::getenv("RK_TUNE_FILE")) return p;
-    const char* h = std::getenv("HOME");
-    return std::string(h ? h : "/tmp") + "/.cache/rknn_amd/autotune.json";
-}
-
 uint32_t pow2_ceil(uint64_t v)
 {
     uint32_t p = 1;
@@ -80,15 +47,10 @@ uint32_t pow2_ceil(uint64_t v)
     return p;
 }
 
-// Workload class: device, tree depth, queries (power of two), mean
+// Workload class of a batch: device, tree depth, queries (power of two), mean
 // candidates and mean k (powers of two).
 std::string tune_key(const rk_batch* b)
 {
-    rk_ctx* c = b->ds->ctx;
-    hipDeviceProp_t prop{};
-    std::string arch = hipGetDeviceProperties(&prop, c->device) == hipSuccess ? prop.gcnArchName : "unknown";
-    const size_t colon = arch.find(':');
-    if (colon != std::string::npos) arch.resize(colon);
     uint64_t nc = 0, k = 0, n = 0;
     for (const QDev& q : b->hq)
         if (!q.trivial) {
@@ -97,47 +59,9 @@ std::string tune_key(const rk_batch* b)
             ++n;
         }
     n = n ? n : 1;
-    return arch + "/L" + std::to_string(b->ds->view.n_levels) + "/q" + std::to_string(pow2_ceil(b->nq)) + "/c" +
-           std::to_string(pow2_ceil(nc / n)) + "/k" + std::to_string(pow2_ceil(k / n));
-}
-
-void load_tune_file(TuneCache& tc)
-{
-    tc.loaded = true;
-    FILE* f = std::fopen(tune_path().c_str(), "r");
-    if (!f) return;
-    char line[512];
-    while (std::fgets(line, sizeof line, f)) { // lines of the form  "key": {"shape": N, ...
-        const char* q0 = std::strchr(line, '"');
-        const char* q1 = q0 ? std::strchr(q0 + 1, '"') : nullptr;
-        const char* sh = q1 ? std::strstr(q1, "\"shape\":") : nullptr;
-        if (!sh) continue;
-        const int v = std::atoi(sh + 8);
-        if (v >= 0 && v < kNumShapes) tc.shape[std::string(q0 + 1, q1)] = v;
-    }
-    std::fclose(f);
-}
-
-void save_tune_file(const TuneCache& tc)
-{
-    const std::string path = tune_path();
-    const size_t slash = path.rfind('/');
-    if (slash != std::string::npos) { // best effort: the directory may not exist yet
-        std::string dir = path.substr(0, slash);
-        for (size_t i = 1; i <= dir.size(); ++i)
-            if (i == dir.size() || dir[i] == '/') mkdir(dir.substr(0, i).c_str(), 0755);
-    }
-    const std::string tmp = path + ".tmp" + std::to_string(getpid());
-    FILE* f = std::fopen(tmp.c_str(), "w");
-    if (!f) return;
-    std::fprintf(f, "{\n");
-    size_t i = 0;
-    for (const auto& kv : tc.shape)
-        std::fprintf(f, "  \"%s\": {\"shape\": %d, \"waves_per_query\": %d}%s\n", kv.first.c_str(), kv.second,
-                     kv.second == kShape4Waves ? 4 : 8, ++i < tc.shape.size() ? "," : "");
-    std::fprintf(f, "}\n");
-    std::fclose(f);
-    std::rename(tmp.c_str(), path.c_str());
+    return device_arch(b->ds->ctx) + "/batch/L" + std::to_string(b->ds->view.n_levels) + "/q" +
+           std::to_string(pow2_ceil(b->nq)) + "/c" + std::to_string(pow2_ceil(nc / n)) + "/k" +
+           std::to_string(pow2_ceil(k / n));
 }
 
 } // namespace
@@ -153,16 +77,11 @@ rk_status batch_tune(rk_batch* b)
     const char* env = std::getenv("RK_AUTOTUNE"); // "0": default shape, "8" / "4": that shape
     if (env && env[0] == '4' && env[1] == 0) b->shape = kShape4Waves;
     if ((env && (env[0] == '0' || env[0] == '4' || env[0] == '8')) || !b->ds->nu || b->nq == 0) return RK_OK;
-    TuneCache& tc = tune_cache();
     const std::string key = tune_key(b);
-    {
-        std::lock_guard<std::mutex> lk(tc.mu);
-        if (!tc.loaded) load_tune_file(tc);
-        auto it = tc.shape.find(key);
-        if (it != tc.shape.end()) {
-            b->shape = it->second;
-            return RK_OK;
-        }
+    const int known = tuned_choice(key);
+    if (known >= 0 && known < kNumShapes) {
+        b->shape = known;
+        return RK_OK;
     }
     if (b->nq < 64) return RK_OK;
     rk_ctx* c = b->ds->ctx;
@@ -185,13 +104,9 @@ rk_status batch_tune(rk_batch* b)
             b->shape = shape;
         }
     }
-    std::lock_guard<std::mutex> lk(tc.mu);
-    tc.shape[key] = b->shape;
-    save_tune_file(tc);
+    tuned_store(key, b->shape, b->shape == kShape4Waves ? "4 waves per query" : "8 waves per query");
     return RK_OK;
 }
-
-
 
 rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
 {

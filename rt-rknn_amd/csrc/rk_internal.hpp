@@ -92,6 +92,7 @@ struct rk_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::recursive_mutex mu; // serialises device work on this context
     std::string err;
+    std::string arch;        // gcnArchName without feature suffixes (autotuner classes)
 };
 
 struct rk_dataset {
@@ -165,6 +166,11 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq);
 // picks the traversal block shape for a filled batch (autotuner); caller
 // holds the context lock
 rk_status batch_tune(rk_batch* b);
+// autotuner memory (rk_tune.cpp): the variant chosen for a workload class,
+// -1 if none yet
+std::string device_arch(rk_ctx* c);
+int tuned_choice(const std::string& key);
+void tuned_store(const std::string& key, int choice, const char* label);
 } // namespace rk
 
 #define RK_HIP(ctx, call)                                                                                      \

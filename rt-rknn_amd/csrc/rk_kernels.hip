@@ -354,6 +354,140 @@ __global__ __launch_bounds__(kBlock) void k_count_all(DsDev ds, QDev q, const Ca
     if (lane == 0 && rech) atomicAdd(&rech_out[(blockIdx.x * kWaves + wave) % kBuckets], rech);
 }
 
+// Matrix-core variant.  The clean candidates' t values come from
+// v_mfma_f32_16x16x4_f32: 16 candidates as rows (gx, gy, -sh, 0) times 16
+// users as columns (wx, wy, 1, 0) give the 16 x 16 block of t.  Lane l
+// supplies A[l % 16][l / 16] and B[l / 16][l % 16] and receives
+// D[4 (l / 16) + i][l % 16], i = 0..3: four candidates of user l % 16.  The
+// matrix core's error on these 3-term sums is within the 4.01u the record
+// certificate assumes (it rounds like a chain of fp32 FMAs), so clamp(t) is
+// exactly 0 or 1 for a clean candidate as in the VALU variant; the VALU
+// then only clamps and adds (2 instructions per test instead of 3), and the
+// four lanes holding the same user are summed once per group at the end.
+// Non-clean candidates take the same banded path as the VALU variant.
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+constexpr int kMfmaTiles = 2; // tiles (64 users) per wave
+
+__global__ __launch_bounds__(kBlock) void k_count_all_mfma(DsDev ds, QDev q, const CandRec* __restrict__ rec,
+                                                           const float2* __restrict__ orig,
+                                                           uint32_t* __restrict__ counts,
+                                                           uint32_t* __restrict__ rech_out)
+{
+    __shared__ CandRec s_rec[kWaves][128];
+    __shared__ float2 s_orig[kWaves][64];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t lane = lane_id();
+    const uint32_t kk = lane >> 4;   // MFMA operand component of this lane
+    const uint32_t col = lane & 15u; // MFMA row / column of this lane
+    const uint32_t stride = gridDim.x * kWaves;
+    const uint32_t n_groups = (ds.n_tiles + kMfmaTiles - 1) / kMfmaTiles;
+    uint32_t rech = 0;
+    CandRec* wlds = s_rec[wave];
+    float2* worig = s_orig[wave];
+    const mfma_f4 zero = {0.f, 0.f, 0.f, 0.f};
+    for (uint32_t g = blockIdx.x * kWaves + wave; g < n_groups; g += stride) {
+        const uint32_t t0 = g * kMfmaTiles;
+        const uint32_t nt = min((uint32_t)kMfmaTiles, ds.n_tiles - t0);
+        float4 m = ds.mbr[0][t0];
+        float px[kMfmaTiles], py[kMfmaTiles], wx[kMfmaTiles], wy[kMfmaTiles];
+        double dq[kMfmaTiles];
+        uint32_t cnt[kMfmaTiles];
+        float bop[kMfmaTiles][4], acc[kMfmaTiles][4];
+#pragma unroll
+        for (int j = 0; j < kMfmaTiles; ++j) {
+            px[j] = py[j] = wx[j] = wy[j] = 0.f;
+            dq[j] = 0.0;
+            cnt[j] = 0;
+            if ((uint32_t)j < nt) {
+                const uint32_t u = (t0 + j) * kTile + lane;
+                px[j] = ds.ux[u];
+                py[j] = ds.uy[u];
+                wx[j] = px[j] - q.qx;
+                wy[j] = py[j] - q.qy;
+                dq[j] = d2_exact(px[j], py[j], q.qx, q.qy);
+                if (j) {
+                    const float4 b = ds.mbr[0][t0 + j];
+                    m.x = fminf(m.x, b.x);
+                    m.y = fminf(m.y, b.y);
+                    m.z = fmaxf(m.z, b.z);
+                    m.w = fmaxf(m.w, b.w);
+                }
+            }
+            // B operands: users 16 s .. 16 s + 15 of tile j as columns
+#pragma unroll
+            for (int sg = 0; sg < 4; ++sg) {
+                const int src = sg * 16 + (int)col;
+                const float xw = __shfl(wx[j], src), yw = __shfl(wy[j], src);
+                bop[j][sg] = kk == 0 ? xw : (kk == 1 ? yw : (kk == 2 ? 1.0f : 0.0f));
+                acc[j][sg] = 0.f;
+            }
+        }
+        const float lx = m.x - q.qx, ly = m.y - q.qy, hx = m.z - q.qx, hy = m.w - q.qy;
+        for (uint32_t c0 = 0; c0 < q.nc; c0 += 64) {
+            const uint32_t jc = c0 + lane;
+            const bool have = jc < q.nc;
+            CandRec r = {0.f, 0.f, 0.f, 0.f};
+            if (have) r = rec[jc];
+            const float tlo = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
+            const float thi = fmaf(r.gx >= 0.f ? hx : lx, r.gx, fmaf(r.gy >= 0.f ? hy : ly, r.gy, -r.sh));
+            const bool clean = have && (tlo >= 1.1875f || thi <= -0.1875f);
+            const bool slow = have && !clean;
+            const uint64_t mc = ballot(clean), ms = ballot(slow);
+            if (clean) wlds[rank_below(mc)] = r;
+            if (slow) {
+                wlds[64 + rank_below(ms)] = r;
+                worig[rank_below(ms)] = orig[jc];
+            }
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t n_clean = (uint32_t)__popcll(mc), n_slow = (uint32_t)__popcll(ms);
+            for (uint32_t b0 = 0; b0 < n_clean; b0 += 16) {
+                const uint32_t ci = b0 + col;
+                CandRec c = {0.f, 0.f, 1.f, 0.f}; // padding row: t = -1 for every user
+                if (ci < n_clean) c = wlds[ci];
+                const float a = kk == 0 ? c.gx : (kk == 1 ? c.gy : (kk == 2 ? -c.sh : 0.0f));
+#pragma unroll
+                for (int j = 0; j < kMfmaTiles; ++j)
+#pragma unroll
+                    for (int sg = 0; sg < 4; ++sg) {
+                        const mfma_f4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bop[j][sg], zero, 0, 0, 0);
+                        acc[j][sg] += (__builtin_amdgcn_fmed3f(d[0], 0.0f, 1.0f) + __builtin_amdgcn_fmed3f(d[1], 0.0f, 1.0f)) +
+                                      (__builtin_amdgcn_fmed3f(d[2], 0.0f, 1.0f) + __builtin_amdgcn_fmed3f(d[3], 0.0f, 1.0f));
+                    }
+            }
+            for (uint32_t i = 0; i < n_slow; ++i) {
+                const CandRec c = wlds[64 + i];
+#pragma unroll
+                for (int j = 0; j < kMfmaTiles; ++j) {
+                    const float t = fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh));
+                    cnt[j] += (t >= kCertainHi) ? 1u : 0u;
+                    const bool inband = in_band(t);
+                    const uint64_t bm = ballot(inband);
+                    if (bm) { // rare: settle the band in binary64
+                        const float2 f = worig[i];
+                        if (inband && d2_exact(px[j], py[j], f.x, f.y) < dq[j]) ++cnt[j];
+                        rech += (uint32_t)__popcll(bm);
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+#pragma unroll
+        for (int j = 0; j < kMfmaTiles; ++j) {
+            if ((uint32_t)j >= nt) break;
+            float mine = 0.f;
+#pragma unroll
+            for (int sg = 0; sg < 4; ++sg) { // the four lanes of a user (same col) hold disjoint candidates
+                float v = acc[j][sg];
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if ((int)kk == sg) mine = v; // this lane's own user: 16 kk + col = lane
+            }
+            counts[(t0 + j) * kTile + lane] = (uint32_t)mine + cnt[j];
+        }
+    }
+    if (lane == 0 && rech) atomicAdd(&rech_out[(blockIdx.x * kWaves + wave) % kBuckets], rech);
+}
+
 __global__ void k_scatter_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm, uint32_t n,
                               uint32_t* __restrict__ dst)
 {
@@ -1061,8 +1195,13 @@ void launch_plain(const DsDev& ds, QDev q, const CandRec* rec, const float2* ori
 }
 
 void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, uint32_t* counts,
-                      uint32_t* rechecks, int grid, hipStream_t st)
+                      uint32_t* rechecks, int grid, int variant, hipStream_t st)
 {
+    if (variant == kCountMfma) {
+        const uint32_t groups = (ds.n_tiles + kMfmaTiles - 1) / kMfmaTiles;
+        k_count_all_mfma<<<grid_for(groups, kWaves, grid), kBlock, 0, st>>>(ds, q, rec, orig, counts, rechecks);
+        return;
+    }
     const uint32_t groups = (ds.n_tiles + kCountUsers - 1) / kCountUsers;
     k_count_all<<<grid_for(groups, kWaves, grid), kBlock, 0, st>>>(ds, q, rec, orig, counts, rechecks);
 }

@@ -58,8 +58,12 @@ void launch_plain(const DsDev& ds, QDev q, const CandRec* rec, const float2* ori
 
 // count-all form: exact per-user count (sorted order), no early exit;
 // rechecks has kBuckets slots
+// variants: kCountValu (3 VALU per test) or kCountMfma (matrix core for the
+// clean candidates, 2 VALU per test)
+constexpr int kCountValu = 0;
+constexpr int kCountMfma = 1;
 void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, uint32_t* counts,
-                      uint32_t* rechecks, int grid, hipStream_t st);
+                      uint32_t* rechecks, int grid, int variant, hipStream_t st);
 void launch_scatter_u32(const uint32_t* src_sorted, const uint32_t* perm, uint32_t n, uint32_t* dst_orig,
                         hipStream_t st);
 

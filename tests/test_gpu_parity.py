@@ -76,7 +76,12 @@ def test_ties_golden_every_form(gties):
         assert np.array_equal(g, exp)
 
 
-def test_count_all_matches_oracle(g1, gties):
+@pytest.mark.parametrize("kernel", ["valu", "mfma", "tuned"])
+def test_count_all_matches_oracle(g1, gties, kernel, monkeypatch):
+    """Dense per-user counts, both count-all kernels (VALU and matrix core)
+    and the autotuned choice, against the oracle's counts."""
+    if kernel != "tuned":
+        monkeypatch.setenv("RK_COUNT_KERNEL", kernel)
     for d, ds, qk, _ in (g1, gties):
         for q, k in qk[:6]:
             cand = ds.filter(q, k)
@@ -247,6 +252,27 @@ def test_autotuned_shapes_agree(rk, ctx, tmp_path, monkeypatch):
             assert np.array_equal(g, e)
         b.close()
         text = cache.read_text()
-        assert '"waves_per_query": %d' % w in text and "/q128/" in text
+        assert '"variant": "%d waves per query"' % w in text and "/batch/" in text and "/q128/" in text
+    finally:
+        ds.close()
+
+
+@pytest.mark.parametrize("kernel", ["valu", "mfma"])
+def test_count_all_large_vs_oracle(rk, ctx, kernel, monkeypatch):
+    """Count-all on clustered data with exact bisector ties (config 3) and
+    thousands of candidates: every per-user count equals the oracle's."""
+    monkeypatch.setenv("RK_COUNT_KERNEL", kernel)
+    seed = rk.default_seed(3)
+    nu, nf = 60_000, 3_000
+    ux, uy, uid, fx, fy, fid = rk.generate(3, nu, nf, seed=seed, nq_ties=4, seed_ties=seed)
+    ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    O.set_threads(16)
+    try:
+        for q in rk.generate_queries(nf, 4, seed)[:3]:
+            q = int(q)
+            for cand in (ds.filter(q, 25), np.arange(nf, dtype=np.uint32)):
+                exp = O.counts(ux, uy, fx, fy, q, cand=cand)
+                got, t = ds.count_all(q, cand, timing=True)
+                assert np.array_equal(got, exp), (kernel, q, len(cand))
     finally:
         ds.close()
