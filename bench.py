@@ -166,10 +166,10 @@ def main():
         sys.exit(1)
 
     # ---- roofline: dense counting kernel (count-all), HIP events on its stream;
-    # both variants (VALU: 3 instructions per test; matrix core: the clean
-    # candidates' t from v_mfma_f32_16x16x4_f32, 2 VALU per test)
+    # both variants (4 or 8 users per lane; 3 VALU instructions per clean test)
     variants = {}
-    for kern in ("valu", "mfma"):
+    kernel_names = {"valu4": "k_count_all<4>", "valu8": "k_count_all<8>"}
+    for kern in kernel_names:
         os.environ["RK_COUNT_KERNEL"] = kern
         kt, flops = [], 0.0
         for i in range(min(args.roofline_queries, args.queries)):
@@ -178,13 +178,13 @@ def main():
             flops += 4.0 * args.users * t.n_candidates
         variants[kern] = (flops / sum(kt) / 1e12, float(np.mean(kt)), flops / len(kt))
         log(f"count-all kernel ({kern}): {variants[kern][0]:.2f} TFLOP/s over {len(kt)} launches, "
-            f"avg {variants[kern][1] * 1e3:.3f} ms")
+            f"avg {variants[kern][1] * 1e3:.4f} ms")
     del os.environ["RK_COUNT_KERNEL"]
     best = max(variants, key=lambda v: variants[v][0])
     achieved, avg_s, flops_launch = variants[best]
-    roofline = {"bound": "mfma" if best == "mfma" else "valu", "achieved": round(achieved, 3),
+    roofline = {"bound": "valu", "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": None, "kernel": "k_count_all_mfma" if best == "mfma" else "k_count_all",
+                "traffic": None, "kernel": kernel_names[best],
                 "avg_launch_ms": round(avg_s * 1e3, 4), "flops_per_launch": flops_launch,
                 "variants_tflops": {k: round(v[0], 3) for k, v in variants.items()}}
 

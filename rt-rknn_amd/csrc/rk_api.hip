@@ -101,20 +101,23 @@ rk_status upload_records(rk_dataset* ds, uint32_t q, const std::vector<uint32_t>
 // Count-all kernel class: device and candidates (power of two).
 std::string count_key(rk_dataset* ds, size_t nc)
 {
-    uint64_t p = 1;
+    uint64_t p = 1, u = 1;
     while (p < nc && p < (1ull << 30)) p <<= 1;
-    return device_arch(ds->ctx) + "/count/c" + std::to_string(p);
+    while (u < ds->nu && u < (1ull << 40)) u <<= 1;
+    return device_arch(ds->ctx) + "/count/u" + std::to_string(u) + "/c" + std::to_string(p);
 }
 
-// RK_COUNT_KERNEL=valu|mfma forces a variant; otherwise the autotuner's
-// choice for the class, or -1 when the class has not been timed yet.
+// RK_COUNT_KERNEL=valu4|valu8|mfma forces a variant; otherwise the
+// autotuner's choice for the class, or -1 when the class has not been timed
+// yet.
 int count_variant(rk_dataset* ds, size_t nc)
 {
     if (const char* e = std::getenv("RK_COUNT_KERNEL")) {
-        if (std::strcmp(e, "valu") == 0) return kCountValu;
-        if (std::strcmp(e, "mfma") == 0) return kCountMfma;
+        for (int v = 0; v < kCountVariants; ++v)
+            if (std::strcmp(e, count_variant_name(v)) == 0) return v;
     }
-    return tuned_choice(count_key(ds, nc));
+    const int v = tuned_choice(count_key(ds, nc));
+    return v < kCountVariants ? v : -1;
 }
 
 QDev make_qdev(rk_dataset* ds, uint32_t q, uint32_t k, size_t nc)
@@ -465,38 +468,44 @@ rk_status rk_count_all(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t 
     if (s != RK_OK) return s;
     RK_HIP(c, ds->counts.reserve(ds->n_pad));
     RK_HIP(c, ds->counts_orig.reserve(ds->nu));
+    // [0] unused, [1 .. kBuckets] binary64 tests
     RK_HIP(c, ds->ctr.reserve(kPlainCounters));
     RK_HIP(c, ds->h_ctr.reserve(kPlainCounters));
     RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, kPlainCounters * sizeof(uint32_t), c->stream));
     QDev qd = make_qdev(ds, q, 1, cl.size());
-    const int grid = device_grid(c->n_cu, 32);
+    // one timed launch of variant v
+    auto run = [&](int v, float* ms) -> rk_status {
+        RK_HIP(c, hipEventRecord(c->ev0, c->stream));
+        launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, v, c->stream);
+        RK_HIP(c, hipGetLastError());
+        RK_HIP(c, hipEventRecord(c->ev1, c->stream));
+        if (ms) {
+            RK_HIP(c, hipEventSynchronize(c->ev1));
+            RK_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+        }
+        return RK_OK;
+    };
     int variant = count_variant(ds, cl.size());
-    if (variant < 0) { // first call of this class: time both kernels on this query, keep the faster
+    if (variant < 0) { // first call of this class: time every kernel on this query, keep the fastest
         float best = 0.f;
-        for (int v = 0; v < 2; ++v) {
+        for (int v = 0; v < kCountVariants; ++v) {
             float ms = 0.f;
             for (int r = 0; r < 2; ++r) {
-                RK_HIP(c, hipEventRecord(c->ev0, c->stream));
-                launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, grid, v,
-                                 c->stream);
-                RK_HIP(c, hipGetLastError());
-                RK_HIP(c, hipEventRecord(c->ev1, c->stream));
-                RK_HIP(c, hipEventSynchronize(c->ev1));
-                RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+                s = run(v, &ms);
+                if (s != RK_OK) return s;
             }
             if (v == 0 || ms < best) {
                 best = ms;
                 variant = v;
             }
         }
-        tuned_store(count_key(ds, cl.size()), variant, variant == kCountMfma ? "matrix core" : "valu");
+        tuned_store(count_key(ds, cl.size()), variant, count_variant_name(variant));
         RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, kPlainCounters * sizeof(uint32_t), c->stream));
     }
-    RK_HIP(c, hipEventRecord(c->ev0, c->stream));
-    launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, grid, variant, c->stream);
-    RK_HIP(c, hipGetLastError());
-    RK_HIP(c, hipEventRecord(c->ev1, c->stream));
-    launch_scatter_u32(ds->counts.p, ds->perm.p, (uint32_t)ds->nu, ds->counts_orig.p, c->stream);
+    s = run(variant, nullptr);
+    if (s != RK_OK) return s;
+    launch_scatter_counts(ds->counts.p, count_narrow(qd.nc), ds->perm.p, (uint32_t)ds->nu, ds->counts_orig.p,
+                          c->stream);
     RK_HIP(c, hipGetLastError());
     RK_HIP(c, hipMemcpyAsync(counts, ds->counts_orig.p, ds->nu * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, kPlainCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,

@@ -250,249 +250,183 @@ __global__ __launch_bounds__(kBlock) void k_plain(DsDev ds, QDev q, const CandRe
     if (lane_id() == 0 && rech) atomicAdd(&ctr[1 + (blockIdx.x * kWaves + wave) % kBuckets], rech);
 }
 // ------------------------------------------------------------ count-all form
-
-// Every (user, candidate) pair is evaluated: no culling, no early exit.  A
-// wave owns 4 consecutive tiles (256 users, 4 per lane) so that each
-// broadcast LDS read of a record feeds 4 users and the loop is VALU-bound.
+//
+// Every (user, candidate) pair is evaluated: no culling, no early exit (the
+// rknn-rt any-hit counter with k = infinity).  Each wave counts one group of
+// U adjacent tiles and the grid has a block for every kWaves groups, so the
+// hardware dispatcher balances the load: groups that cost more (near q
+// every bisector crosses them and they take the banded path) hold their
+// slot longer while other slots take more blocks, and the kernel's tail is
+// at most one group long.  (A persistent grid with a static split left the
+// waves that drew those groups running alone; work counters in global
+// memory cost a serialised device atomic, about 100 ns, per group.)  The
+// query's records are staged once per block in LDS (up to kCountLdsRecs of
+// them; longer lists are read from global).
+//
 // Per 64-candidate chunk the candidates are split against the group's
-// rectangle: "clean" ones, whose computed t is >= 19/16 (or <= -3/16) at the
-// rectangle's extreme corner, give every user t >= 1 (or <= 0) because the
-// fp32 error is < 1/16 per evaluation, so the saturated value is exactly 1
-// or 0 and a float sum counts them exactly (2 FMAs with clamp + 1 add per
-// test); the others take the banded path with the binary64 re-check.
-constexpr int kCountUsers = 4;
+// joint rectangle: "clean" ones, whose computed t is >= 19/16 (or <= -3/16)
+// at the rectangle's extreme corner, give every user of the group t >= 1 (or
+// <= 0), because the fp32 error is < 1/16 per evaluation; the others take
+// the banded path, users on lanes, with the binary64 re-check.
+//
+// Counts are written in sorted user order, as 16-bit values when the query
+// has fewer than 2^16 candidates (10 instead of 12 bytes of HBM traffic per
+// user), else 32-bit.
 
-__global__ __launch_bounds__(kBlock) void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec,
-                                                      const float2* __restrict__ orig,
-                                                      uint32_t* __restrict__ counts, uint32_t* __restrict__ rech_out)
+constexpr uint32_t kCountLdsRecs = 1024;
+
+// The candidate split of one chunk (see above): clean when certain for the
+// whole rectangle (lx, ly, hx, hy relative to q).
+__device__ __forceinline__ bool clean_for(const CandRec& r, float lx, float ly, float hx, float hy)
 {
-    __shared__ CandRec s_rec[kWaves][128];
-    __shared__ float2 s_orig[kWaves][64];
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t lane = lane_id();
-    const uint32_t stride = gridDim.x * kWaves;
-    const uint32_t n_groups = (ds.n_tiles + kCountUsers - 1) / kCountUsers;
-    uint32_t rech = 0;
-    CandRec* wlds = s_rec[wave];
-    float2* worig = s_orig[wave];
-    for (uint32_t g = blockIdx.x * kWaves + wave; g < n_groups; g += stride) {
-        const uint32_t t0 = g * kCountUsers;
-        const uint32_t nt = min((uint32_t)kCountUsers, ds.n_tiles - t0);
-        float4 m = ds.mbr[0][t0];
-        float px[kCountUsers], py[kCountUsers], wx[kCountUsers], wy[kCountUsers], acc[kCountUsers];
-        double dq[kCountUsers];
-        uint32_t cnt[kCountUsers];
+    const float tlo = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
+    const float thi = fmaf(r.gx >= 0.f ? hx : lx, r.gx, fmaf(r.gy >= 0.f ? hy : ly, r.gy, -r.sh));
+    return tlo >= 1.1875f || thi <= -0.1875f;
+}
+
+// Banded path for n_slow candidates s[0 .. n_slow) against U tiles of users
+// (lane = user), counting into cnt; band values are settled in binary64.
+template <int U>
+__device__ __forceinline__ void count_slow(const CandRec* s, const float2* so, uint32_t n_slow, const float* px,
+                                           const float* py, float qx, float qy, uint32_t* cnt, uint32_t& rech)
+{
+    for (uint32_t i = 0; i < n_slow; ++i) {
+        const CandRec c = s[i];
 #pragma unroll
-        for (int j = 0; j < kCountUsers; ++j) {
-            px[j] = py[j] = wx[j] = wy[j] = acc[j] = 0.f;
-            dq[j] = 0.0;
-            cnt[j] = 0;
-            if ((uint32_t)j < nt) {
-                const uint32_t u = (t0 + j) * kTile + lane;
-                px[j] = ds.ux[u];
-                py[j] = ds.uy[u];
-                wx[j] = px[j] - q.qx;
-                wy[j] = py[j] - q.qy;
-                dq[j] = d2_exact(px[j], py[j], q.qx, q.qy);
-                if (j) {
-                    const float4 b = ds.mbr[0][t0 + j];
-                    m.x = fminf(m.x, b.x);
-                    m.y = fminf(m.y, b.y);
-                    m.z = fmaxf(m.z, b.z);
-                    m.w = fmaxf(m.w, b.w);
-                }
+        for (int j = 0; j < U; ++j) {
+            const float t = fmaf(px[j] - qx, c.gx, fmaf(py[j] - qy, c.gy, -c.sh));
+            cnt[j] += (t >= kCertainHi) ? 1u : 0u;
+            const bool inband = in_band(t);
+            const uint64_t bm = ballot(inband);
+            if (bm) { // rare: settle the band in binary64
+                const float2 f = so[i];
+                if (inband && d2_exact(px[j], py[j], f.x, f.y) < d2_exact(px[j], py[j], qx, qy)) ++cnt[j];
+                rech += (uint32_t)__popcll(bm);
             }
         }
-        const float lx = m.x - q.qx, ly = m.y - q.qy, hx = m.z - q.qx, hy = m.w - q.qy;
-        for (uint32_t c0 = 0; c0 < q.nc; c0 += 64) {
-            const uint32_t jc = c0 + lane;
-            const bool have = jc < q.nc;
-            CandRec r = {0.f, 0.f, 0.f, 0.f};
-            if (have) r = rec[jc];
-            const float tlo = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
-            const float thi = fmaf(r.gx >= 0.f ? hx : lx, r.gx, fmaf(r.gy >= 0.f ? hy : ly, r.gy, -r.sh));
-            const bool clean = have && (tlo >= 1.1875f || thi <= -0.1875f);
-            const bool slow = have && !clean;
-            const uint64_t mc = ballot(clean), ms = ballot(slow);
-            if (clean) wlds[rank_below(mc)] = r;
-            if (slow) {
-                wlds[64 + rank_below(ms)] = r;
-                worig[rank_below(ms)] = orig[jc];
+    }
+}
+
+// U tiles per group, user j of the group's tile j on every lane (U users per
+// lane), so that each broadcast LDS read of a record feeds U tests.  A clean
+// test is 3 instructions: t = fma(wx, gx, fma(wy, gy, -sh)) with the second
+// FMA's clamp bit giving exactly 0 or 1, and one add (all three run at the
+// double FP32 rate of gfx950).
+// The users (lane = user, one per tile) and joint rectangle of group g.
+template <int U>
+__device__ __forceinline__ void load_group(const DsDev& ds, uint32_t g, float* px, float* py, float4& m)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t t0 = g * U;
+    const uint32_t nt = min((uint32_t)U, ds.n_tiles - t0);
+    m = ds.mbr[0][t0];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        px[j] = py[j] = 0.f;
+        if ((uint32_t)j < nt) {
+            px[j] = ds.ux[(t0 + j) * kTile + lane];
+            py[j] = ds.uy[(t0 + j) * kTile + lane];
+            if (j) {
+                const float4 b = ds.mbr[0][t0 + j];
+                m.x = fminf(m.x, b.x);
+                m.y = fminf(m.y, b.y);
+                m.z = fmaxf(m.z, b.z);
+                m.w = fmaxf(m.w, b.w);
             }
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t n_clean = (uint32_t)__popcll(mc), n_slow = (uint32_t)__popcll(ms);
+        }
+    }
+}
+
+template <int U, typename OutT>
+__device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint32_t nc, bool lds_recs,
+                                            const CandRec* __restrict__ srec, const CandRec* __restrict__ rec,
+                                            const float2* __restrict__ orig, OutT* __restrict__ counts,
+                                            CandRec* wl, float2* wo, uint32_t g, const float* px, const float* py,
+                                            float4 m, uint32_t& rech)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t t0 = g * U;
+    const uint32_t nt = min((uint32_t)U, ds.n_tiles - t0);
+    float wx[U], wy[U], acc[U];
+    uint32_t cnt[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        wx[j] = px[j] - q.qx;
+        wy[j] = py[j] - q.qy;
+        acc[j] = 0.f;
+        cnt[j] = 0;
+    }
+    const float lx = m.x - q.qx, ly = m.y - q.qy, hx = m.z - q.qx, hy = m.w - q.qy;
+    for (uint32_t c0 = 0; c0 < nc; c0 += 64) {
+        const uint32_t jc = c0 + lane;
+        const bool have = jc < nc;
+        CandRec r = {0.f, 0.f, 0.f, 0.f};
+        if (have) r = lds_recs ? srec[jc] : rec[jc];
+        const bool clean = have && clean_for(r, lx, ly, hx, hy);
+        const bool slow = have && !clean;
+        const uint64_t mc = ballot(clean), ms = ballot(slow);
+        if (clean) wl[rank_below(mc)] = r;
+        if (slow) {
+            wl[64 + rank_below(ms)] = r;
+            wo[rank_below(ms)] = orig[jc];
+        }
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t n_clean = (uint32_t)__popcll(mc), n_slow = (uint32_t)__popcll(ms);
 #pragma unroll 4
-            for (uint32_t i = 0; i < n_clean; ++i) {
-                const CandRec c = wlds[i];
+        for (uint32_t i = 0; i < n_clean; ++i) {
+            const CandRec c = wl[i];
 #pragma unroll
-                for (int j = 0; j < kCountUsers; ++j) // med3(t,0,1) folds into the FMA's clamp bit
-                    acc[j] += __builtin_amdgcn_fmed3f(fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh)), 0.0f, 1.0f);
-            }
-            for (uint32_t i = 0; i < n_slow; ++i) {
-                const CandRec c = wlds[64 + i];
-#pragma unroll
-                for (int j = 0; j < kCountUsers; ++j) {
-                    const float t = fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh));
-                    cnt[j] += (t >= kCertainHi) ? 1u : 0u;
-                    const bool inband = (t > kCertainLo) && (t < kCertainHi);
-                    const uint64_t bm = ballot(inband);
-                    if (bm) { // rare: settle the band in binary64
-                        const float2 f = worig[i];
-                        if (inband && d2_exact(px[j], py[j], f.x, f.y) < dq[j]) ++cnt[j];
-                        rech += (uint32_t)__popcll(bm);
-                    }
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
+            for (int j = 0; j < U; ++j) // med3(t,0,1) folds into the FMA's clamp bit
+                acc[j] += __builtin_amdgcn_fmed3f(fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh)), 0.0f, 1.0f);
         }
-#pragma unroll
-        for (int j = 0; j < kCountUsers; ++j) {
-            if ((uint32_t)j >= nt) break;
-            counts[(t0 + j) * kTile + lane] = (uint32_t)acc[j] + cnt[j];
-        }
+        if (n_slow) count_slow<U>(wl + 64, wo, n_slow, px, py, q.qx, q.qy, cnt, rech);
+        __builtin_amdgcn_wave_barrier();
     }
-    if (lane == 0 && rech) atomicAdd(&rech_out[(blockIdx.x * kWaves + wave) % kBuckets], rech);
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+        if ((uint32_t)j < nt) counts[(t0 + j) * kTile + lane] = (OutT)((uint32_t)acc[j] + cnt[j]);
 }
 
-// Matrix-core variant.  The clean candidates' t values come from
-// v_mfma_f32_16x16x4_f32: 16 candidates as rows (gx, gy, -sh, 0) times 16
-// users as columns (wx, wy, 1, 0) give the 16 x 16 block of t.  Lane l
-// supplies A[l % 16][l / 16] and B[l / 16][l % 16] and receives
-// D[4 (l / 16) + i][l % 16], i = 0..3: four candidates of user l % 16.  The
-// matrix core's error on these 3-term sums is within the 4.01u the record
-// certificate assumes (it rounds like a chain of fp32 FMAs), so clamp(t) is
-// exactly 0 or 1 for a clean candidate as in the VALU variant; the VALU
-// then only clamps and adds (2 instructions per test instead of 3), and the
-// four lanes holding the same user are summed once per group at the end.
-// Non-clean candidates take the same banded path as the VALU variant.
-typedef float mfma_f4 __attribute__((ext_vector_type(4)));
-constexpr int kMfmaTiles = 2; // tiles (64 users) per wave
-
-__global__ __launch_bounds__(kBlock) void k_count_all_mfma(DsDev ds, QDev q, const CandRec* __restrict__ rec,
-                                                           const float2* __restrict__ orig,
-                                                           uint32_t* __restrict__ counts,
-                                                           uint32_t* __restrict__ rech_out)
+// The first group's users are requested before the records are staged, so
+// the two loads overlap.  (Loading a second group ahead, two groups per
+// wave, measured slower: the tail doubles and occupancy drops.)
+template <int U, typename OutT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(U == 4 ? 8 : 1)))
+void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec, const float2* __restrict__ orig,
+                 OutT* __restrict__ counts, uint32_t* __restrict__ rech_out)
 {
-    __shared__ CandRec s_rec[kWaves][128];
-    __shared__ float2 s_orig[kWaves][64];
+    extern __shared__ CandRec s_recs[];       // the query's records (nc <= kCountLdsRecs)
+    __shared__ CandRec s_w[kWaves][128];      // per chunk: clean ones at [0, 64), the others at [64, 128)
+    __shared__ float2 s_o[kWaves][64];        // facility coordinates of the others
     const uint32_t wave = threadIdx.x >> 6;
-    const uint32_t lane = lane_id();
-    const uint32_t kk = lane >> 4;   // MFMA operand component of this lane
-    const uint32_t col = lane & 15u; // MFMA row / column of this lane
+    const uint32_t nc = uniform(q.nc);
+    const bool lds_recs = nc <= kCountLdsRecs;
+    const uint32_t n_groups = (ds.n_tiles + U - 1) / U;
     const uint32_t stride = gridDim.x * kWaves;
-    const uint32_t n_groups = (ds.n_tiles + kMfmaTiles - 1) / kMfmaTiles;
-    uint32_t rech = 0;
-    CandRec* wlds = s_rec[wave];
-    float2* worig = s_orig[wave];
-    const mfma_f4 zero = {0.f, 0.f, 0.f, 0.f};
-    for (uint32_t g = blockIdx.x * kWaves + wave; g < n_groups; g += stride) {
-        const uint32_t t0 = g * kMfmaTiles;
-        const uint32_t nt = min((uint32_t)kMfmaTiles, ds.n_tiles - t0);
-        float4 m = ds.mbr[0][t0];
-        float px[kMfmaTiles], py[kMfmaTiles], wx[kMfmaTiles], wy[kMfmaTiles];
-        double dq[kMfmaTiles];
-        uint32_t cnt[kMfmaTiles];
-        float bop[kMfmaTiles][4], acc[kMfmaTiles][4];
-#pragma unroll
-        for (int j = 0; j < kMfmaTiles; ++j) {
-            px[j] = py[j] = wx[j] = wy[j] = 0.f;
-            dq[j] = 0.0;
-            cnt[j] = 0;
-            if ((uint32_t)j < nt) {
-                const uint32_t u = (t0 + j) * kTile + lane;
-                px[j] = ds.ux[u];
-                py[j] = ds.uy[u];
-                wx[j] = px[j] - q.qx;
-                wy[j] = py[j] - q.qy;
-                dq[j] = d2_exact(px[j], py[j], q.qx, q.qy);
-                if (j) {
-                    const float4 b = ds.mbr[0][t0 + j];
-                    m.x = fminf(m.x, b.x);
-                    m.y = fminf(m.y, b.y);
-                    m.z = fmaxf(m.z, b.z);
-                    m.w = fmaxf(m.w, b.w);
-                }
-            }
-            // B operands: users 16 s .. 16 s + 15 of tile j as columns
-#pragma unroll
-            for (int sg = 0; sg < 4; ++sg) {
-                const int src = sg * 16 + (int)col;
-                const float xw = __shfl(wx[j], src), yw = __shfl(wy[j], src);
-                bop[j][sg] = kk == 0 ? xw : (kk == 1 ? yw : (kk == 2 ? 1.0f : 0.0f));
-                acc[j][sg] = 0.f;
-            }
-        }
-        const float lx = m.x - q.qx, ly = m.y - q.qy, hx = m.z - q.qx, hy = m.w - q.qy;
-        for (uint32_t c0 = 0; c0 < q.nc; c0 += 64) {
-            const uint32_t jc = c0 + lane;
-            const bool have = jc < q.nc;
-            CandRec r = {0.f, 0.f, 0.f, 0.f};
-            if (have) r = rec[jc];
-            const float tlo = fmaf(r.gx >= 0.f ? lx : hx, r.gx, fmaf(r.gy >= 0.f ? ly : hy, r.gy, -r.sh));
-            const float thi = fmaf(r.gx >= 0.f ? hx : lx, r.gx, fmaf(r.gy >= 0.f ? hy : ly, r.gy, -r.sh));
-            const bool clean = have && (tlo >= 1.1875f || thi <= -0.1875f);
-            const bool slow = have && !clean;
-            const uint64_t mc = ballot(clean), ms = ballot(slow);
-            if (clean) wlds[rank_below(mc)] = r;
-            if (slow) {
-                wlds[64 + rank_below(ms)] = r;
-                worig[rank_below(ms)] = orig[jc];
-            }
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t n_clean = (uint32_t)__popcll(mc), n_slow = (uint32_t)__popcll(ms);
-            for (uint32_t b0 = 0; b0 < n_clean; b0 += 16) {
-                const uint32_t ci = b0 + col;
-                CandRec c = {0.f, 0.f, 1.f, 0.f}; // padding row: t = -1 for every user
-                if (ci < n_clean) c = wlds[ci];
-                const float a = kk == 0 ? c.gx : (kk == 1 ? c.gy : (kk == 2 ? -c.sh : 0.0f));
-#pragma unroll
-                for (int j = 0; j < kMfmaTiles; ++j)
-#pragma unroll
-                    for (int sg = 0; sg < 4; ++sg) {
-                        const mfma_f4 d = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bop[j][sg], zero, 0, 0, 0);
-                        acc[j][sg] += (__builtin_amdgcn_fmed3f(d[0], 0.0f, 1.0f) + __builtin_amdgcn_fmed3f(d[1], 0.0f, 1.0f)) +
-                                      (__builtin_amdgcn_fmed3f(d[2], 0.0f, 1.0f) + __builtin_amdgcn_fmed3f(d[3], 0.0f, 1.0f));
-                    }
-            }
-            for (uint32_t i = 0; i < n_slow; ++i) {
-                const CandRec c = wlds[64 + i];
-#pragma unroll
-                for (int j = 0; j < kMfmaTiles; ++j) {
-                    const float t = fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh));
-                    cnt[j] += (t >= kCertainHi) ? 1u : 0u;
-                    const bool inband = in_band(t);
-                    const uint64_t bm = ballot(inband);
-                    if (bm) { // rare: settle the band in binary64
-                        const float2 f = worig[i];
-                        if (inband && d2_exact(px[j], py[j], f.x, f.y) < dq[j]) ++cnt[j];
-                        rech += (uint32_t)__popcll(bm);
-                    }
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-        }
-#pragma unroll
-        for (int j = 0; j < kMfmaTiles; ++j) {
-            if ((uint32_t)j >= nt) break;
-            float mine = 0.f;
-#pragma unroll
-            for (int sg = 0; sg < 4; ++sg) { // the four lanes of a user (same col) hold disjoint candidates
-                float v = acc[j][sg];
-                v += __shfl_xor(v, 16);
-                v += __shfl_xor(v, 32);
-                if ((int)kk == sg) mine = v; // this lane's own user: 16 kk + col = lane
-            }
-            counts[(t0 + j) * kTile + lane] = (uint32_t)mine + cnt[j];
-        }
+    uint32_t g = blockIdx.x * kWaves + wave;
+    float px[U], py[U];
+    float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (g < n_groups) load_group<U>(ds, g, px, py, m);
+    if (lds_recs) {
+        for (uint32_t i = threadIdx.x; i < nc; i += kBlock) s_recs[i] = rec[i];
+        __syncthreads();
     }
-    if (lane == 0 && rech) atomicAdd(&rech_out[(blockIdx.x * kWaves + wave) % kBuckets], rech);
+    uint32_t rech = 0;
+    while (g < n_groups) {
+        count_group<U, OutT>(ds, q, nc, lds_recs, s_recs, rec, orig, counts, s_w[wave], s_o[wave], g, px, py, m,
+                             rech);
+        g += stride;
+        if (g < n_groups) load_group<U>(ds, g, px, py, m);
+    }
+    if (lane_id() == 0 && rech) atomicAdd(&rech_out[(blockIdx.x * kWaves + wave) % kBuckets], rech);
 }
 
-__global__ void k_scatter_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ perm, uint32_t n,
+template <typename SrcT>
+__global__ void k_scatter_u32(const SrcT* __restrict__ src, const uint32_t* __restrict__ perm, uint32_t n,
                               uint32_t* __restrict__ dst)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-        dst[perm[i]] = src[i];
+        dst[perm[i]] = (uint32_t)src[i];
 }
 
 // ------------------------------------------------------ dataset construction
@@ -1194,22 +1128,42 @@ void launch_plain(const DsDev& ds, QDev q, const CandRec* rec, const float2* ori
     k_plain<<<grid_for(ds.n_tiles, kWaves, grid), kBlock, 0, st>>>(ds, q, rec, orig, out_ids, counters);
 }
 
-void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, uint32_t* counts,
-                      uint32_t* rechecks, int grid, int variant, hipStream_t st)
+// Count-all grid: a block for every kWaves groups (capped; the kernel loops
+// past the cap).
+template <typename OutT>
+void launch_count_typed(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, OutT* counts,
+                        uint32_t* rechecks, int variant, hipStream_t st)
 {
-    if (variant == kCountMfma) {
-        const uint32_t groups = (ds.n_tiles + kMfmaTiles - 1) / kMfmaTiles;
-        k_count_all_mfma<<<grid_for(groups, kWaves, grid), kBlock, 0, st>>>(ds, q, rec, orig, counts, rechecks);
-        return;
+    const size_t lds = q.nc <= kCountLdsRecs ? q.nc * sizeof(CandRec) : 0;
+    if (variant == kCountValu8) {
+        const int g = grid_for((ds.n_tiles + 7) / 8, kWaves, 1 << 20);
+        k_count_all<8, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
+    } else {
+        const int g = grid_for((ds.n_tiles + 3) / 4, kWaves, 1 << 20);
+        k_count_all<4, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
     }
-    const uint32_t groups = (ds.n_tiles + kCountUsers - 1) / kCountUsers;
-    k_count_all<<<grid_for(groups, kWaves, grid), kBlock, 0, st>>>(ds, q, rec, orig, counts, rechecks);
 }
 
-void launch_scatter_u32(const uint32_t* src_sorted, const uint32_t* perm, uint32_t n, uint32_t* dst_orig,
-                        hipStream_t st)
+const char* count_variant_name(int variant) { return variant == kCountValu8 ? "valu8" : "valu4"; }
+
+bool count_narrow(uint32_t nc) { return nc < 65536u; }
+
+void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, void* counts,
+                      uint32_t* rechecks, int variant, hipStream_t st)
 {
-    k_scatter_u32<<<grid_for(n, 256, 4096), 256, 0, st>>>(src_sorted, perm, n, dst_orig);
+    if (count_narrow(q.nc))
+        launch_count_typed(ds, q, rec, orig, (uint16_t*)counts, rechecks, variant, st);
+    else
+        launch_count_typed(ds, q, rec, orig, (uint32_t*)counts, rechecks, variant, st);
+}
+
+void launch_scatter_counts(const void* src_sorted, bool narrow, const uint32_t* perm, uint32_t n,
+                           uint32_t* dst_orig, hipStream_t st)
+{
+    if (narrow)
+        k_scatter_u32<<<grid_for(n, 256, 4096), 256, 0, st>>>((const uint16_t*)src_sorted, perm, n, dst_orig);
+    else
+        k_scatter_u32<<<grid_for(n, 256, 4096), 256, 0, st>>>((const uint32_t*)src_sorted, perm, n, dst_orig);
 }
 
 size_t batch_query_lds_bytes(uint32_t n_levels)

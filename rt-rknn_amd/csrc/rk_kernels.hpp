@@ -56,16 +56,19 @@ constexpr int kPlainCounters = 1 + kBuckets;
 void launch_plain(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, int32_t* out_ids,
                   uint32_t* counters, int grid, hipStream_t st);
 
-// count-all form: exact per-user count (sorted order), no early exit;
-// rechecks has kBuckets slots
-// variants: kCountValu (3 VALU per test) or kCountMfma (matrix core for the
-// clean candidates, 2 VALU per test)
-constexpr int kCountValu = 0;
-constexpr int kCountMfma = 1;
-void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, uint32_t* counts,
-                      uint32_t* rechecks, int grid, int variant, hipStream_t st);
-void launch_scatter_u32(const uint32_t* src_sorted, const uint32_t* perm, uint32_t n, uint32_t* dst_orig,
-                        hipStream_t st);
+// count-all form: exact per-user count (sorted order), no early exit.
+// rechecks has kBuckets slots.  Counts are 16-bit when count_narrow(nc)
+// (fewer than 2^16 candidates), else 32-bit; the buffer holds n_pad of them.
+// variants: 4 or 8 users per lane (3 instructions per clean test in each)
+constexpr int kCountValu4 = 0;
+constexpr int kCountValu8 = 1;
+constexpr int kCountVariants = 2;
+const char* count_variant_name(int variant);
+bool count_narrow(uint32_t nc);
+void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, void* counts,
+                      uint32_t* rechecks, int variant, hipStream_t st);
+void launch_scatter_counts(const void* src_sorted, bool narrow, const uint32_t* perm, uint32_t n,
+                           uint32_t* dst_orig, hipStream_t st);
 
 // batched form: the whole step is the traversal launch (which also sorts
 // segments of up to 4096 ids) and the large-segment sort; capture them in a
