@@ -166,9 +166,10 @@ def main():
         sys.exit(1)
 
     # ---- roofline: dense counting kernel (count-all), HIP events on its stream;
-    # both variants (4 or 8 users per lane; 3 VALU instructions per clean test)
+    # every variant (4 or 8 users per lane, or 4 held to 8 waves per SIMD; 3 VALU
+    # instructions per clean test)
     variants = {}
-    kernel_names = {"valu4": "k_count_all<4>", "valu8": "k_count_all<8>"}
+    kernel_names = {"valu4": "k_count_all<4,1>", "valu8": "k_count_all<8,1>", "valu4o8": "k_count_all<4,8>"}
     for kern in kernel_names:
         os.environ["RK_COUNT_KERNEL"] = kern
         kt, flops = [], 0.0

@@ -345,6 +345,13 @@ rk_status rk_dataset_create(rk_ctx* c, const float* ux, const float* uy, const i
             ds->view.nlev[lvl] = cnt;
         }
         ds->view.n_levels = (uint32_t)lvl + 1;
+        // joint rectangles of the count-all kernels' 4- and 8-tile groups
+        RK_HIP(c, ds->gmbr4.reserve((ds->n_tiles + 3) / 4));
+        RK_HIP(c, ds->gmbr8.reserve((ds->n_tiles + 7) / 8));
+        launch_group_mbr(ds->mbr[0].p, ds->n_tiles, 4, ds->gmbr4.p, st);
+        RK_HIP(c, hipGetLastError());
+        launch_group_mbr(ds->mbr[0].p, ds->n_tiles, 8, ds->gmbr8.p, st);
+        RK_HIP(c, hipGetLastError());
         RK_HIP(c, hipStreamSynchronize(st)); // temporaries are freed on return
         RK_HIP(c, ds->out.reserve(ds->n_pad));
     }
@@ -352,6 +359,8 @@ rk_status rk_dataset_create(rk_ctx* c, const float* ux, const float* uy, const i
     ds->view.uy = ds->uy.p;
     ds->view.uid = ds->uid.p;
     for (int l = 0; l < kMaxLevels; ++l) ds->view.mbr[l] = ds->mbr[l].p;
+    ds->view.gmbr4 = ds->gmbr4.p;
+    ds->view.gmbr8 = ds->gmbr8.p;
     ds->view.nu = (uint32_t)nu;
     ds->view.n_tiles = ds->n_tiles;
     RK_HIP(c, hipStreamSynchronize(st));
@@ -379,6 +388,7 @@ rk_status rk_dataset_info(const rk_dataset* ds, uint64_t* nu, uint64_t* nf, uint
         uint64_t b = ds->ux.bytes() + ds->uy.bytes() + ds->uid.bytes() + ds->perm.bytes() + ds->fx.bytes() +
                      ds->fy.bytes() + ds->out.bytes();
         for (int l = 0; l < kMaxLevels; ++l) b += ds->mbr[l].bytes();
+        b += ds->gmbr4.bytes() + ds->gmbr8.bytes();
         *device_bytes = b;
     }
     return RK_OK;

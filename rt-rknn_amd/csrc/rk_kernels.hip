@@ -285,54 +285,91 @@ __device__ __forceinline__ bool clean_for(const CandRec& r, float lx, float ly, 
 }
 
 // Banded path for n_slow candidates s[0 .. n_slow) against U tiles of users
-// (lane = user), counting into cnt; band values are settled in binary64.
+// (lane = user), counting into cnt.  It runs in double-rate FP32 only:
+//   c = clamp(4 t - 3/2)
+// is exactly 1 for t >= 5/8 (certainly closer), exactly 0 for t <= 3/8
+// (certainly not), and strictly between 0 and 1 inside the band, where
+// 4 t - 3/2 is exact (4 t lies in [3/2, 5/2], the difference is a multiple of
+// its ulp below 1).  The sum of c counts the certain cases; the sum of
+// c - c^2 (each term exactly zero for c in {0, 1}, positive otherwise) is
+// zero unless a band value occurred.  A tile with one re-walks its slow
+// candidates with the oracle's binary64 comparison in the band.
 template <int U>
-__device__ __forceinline__ void count_slow(const CandRec* s, const float2* so, uint32_t n_slow, const float* px,
-                                           const float* py, float qx, float qy, uint32_t* cnt, uint32_t& rech)
+__device__ __forceinline__ void count_slow(const CandRec* s, const float2* so, uint32_t n_slow, const DsDev& ds,
+                                           uint32_t t0, uint32_t nt, const float* wx, const float* wy, float qx,
+                                           float qy, uint32_t* cnt, uint32_t& rech)
 {
+    float sacc[U], bacc[U];
+#pragma unroll
+    for (int j = 0; j < U; ++j) sacc[j] = bacc[j] = 0.f;
     for (uint32_t i = 0; i < n_slow; ++i) {
         const CandRec c = s[i];
 #pragma unroll
         for (int j = 0; j < U; ++j) {
-            const float t = fmaf(px[j] - qx, c.gx, fmaf(py[j] - qy, c.gy, -c.sh));
-            cnt[j] += (t >= kCertainHi) ? 1u : 0u;
-            const bool inband = in_band(t);
-            const uint64_t bm = ballot(inband);
-            if (bm) { // rare: settle the band in binary64
-                const float2 f = so[i];
-                if (inband && d2_exact(px[j], py[j], f.x, f.y) < d2_exact(px[j], py[j], qx, qy)) ++cnt[j];
-                rech += (uint32_t)__popcll(bm);
+            const float t = fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh));
+            const float v = __builtin_amdgcn_fmed3f(fmaf(t, 4.0f, -1.5f), 0.0f, 1.0f);
+            sacc[j] += v;
+            bacc[j] += fmaf(v, -v, v);
+        }
+    }
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+        const bool band = bacc[j] > 0.f;
+        if (ballot(band) == 0) {
+            cnt[j] += (uint32_t)sacc[j];
+            continue;
+        }
+        // rare: this tile met the band; recount its slow candidates exactly
+        uint32_t exact = 0;
+        if ((uint32_t)j < nt) {
+            const float px = ds.ux[(t0 + j) * kTile + lane], py = ds.uy[(t0 + j) * kTile + lane];
+            const double dq = d2_exact(px, py, qx, qy);
+            for (uint32_t i = 0; i < n_slow; ++i) {
+                const CandRec c = s[i];
+                const float t = fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh));
+                exact += (t >= kCertainHi) ? 1u : 0u;
+                const bool inband = in_band(t);
+                const uint64_t bm = ballot(inband);
+                if (bm) {
+                    const float2 f = so[i];
+                    if (inband && d2_exact(px, py, f.x, f.y) < dq) ++exact;
+                    rech += (uint32_t)__popcll(bm);
+                }
             }
         }
+        cnt[j] += band ? exact : (uint32_t)sacc[j];
     }
 }
 
-// U tiles per group, user j of the group's tile j on every lane (U users per
-// lane), so that each broadcast LDS read of a record feeds U tests.  A clean
-// test is 3 instructions: t = fma(wx, gx, fma(wy, gy, -sh)) with the second
-// FMA's clamp bit giving exactly 0 or 1, and one add (all three run at the
-// double FP32 rate of gfx950).
-// The users (lane = user, one per tile) and joint rectangle of group g.
+// The users (lane = user, one per tile) and joint rectangle of group g (kept
+// per group in the dataset: a union here would cost single-rate min/max and
+// NaN-canonicalising instructions per group).
 template <int U>
 __device__ __forceinline__ void load_group(const DsDev& ds, uint32_t g, float* px, float* py, float4& m)
 {
+    static_assert(U == 4 || U == 8, "group rectangles are kept for 4- and 8-tile groups");
     const uint32_t lane = lane_id();
     const uint32_t t0 = g * U;
-    const uint32_t nt = min((uint32_t)U, ds.n_tiles - t0);
-    m = ds.mbr[0][t0];
+    const uint32_t nt = uniform(min((uint32_t)U, ds.n_tiles - t0));
+    m = (U == 4 ? ds.gmbr4 : ds.gmbr8)[g];
+    const uint64_t base = (uint64_t)t0 * kTile + lane;
+    const float* ux = ds.ux + base;
+    const float* uy = ds.uy + base;
+    if (nt == (uint32_t)U) { // a full group: one address, immediate offsets
+#pragma unroll
+        for (int j = 0; j < U; ++j) {
+            px[j] = ux[j * kTile];
+            py[j] = uy[j * kTile];
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < U; ++j) {
         px[j] = py[j] = 0.f;
         if ((uint32_t)j < nt) {
-            px[j] = ds.ux[(t0 + j) * kTile + lane];
-            py[j] = ds.uy[(t0 + j) * kTile + lane];
-            if (j) {
-                const float4 b = ds.mbr[0][t0 + j];
-                m.x = fminf(m.x, b.x);
-                m.y = fminf(m.y, b.y);
-                m.z = fmaxf(m.z, b.z);
-                m.w = fmaxf(m.w, b.w);
-            }
+            px[j] = ux[j * kTile];
+            py[j] = uy[j * kTile];
         }
     }
 }
@@ -361,7 +398,11 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
         const uint32_t jc = c0 + lane;
         const bool have = jc < nc;
         CandRec r = {0.f, 0.f, 0.f, 0.f};
-        if (have) r = lds_recs ? srec[jc] : rec[jc];
+        if (lds_recs) { // two explicit paths: a selected pointer would become a flat load
+            if (have) r = srec[jc];
+        } else {
+            if (have) r = rec[jc];
+        }
         const bool clean = have && clean_for(r, lx, ly, hx, hy);
         const bool slow = have && !clean;
         const uint64_t mc = ballot(clean), ms = ballot(slow);
@@ -379,7 +420,7 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
             for (int j = 0; j < U; ++j) // med3(t,0,1) folds into the FMA's clamp bit
                 acc[j] += __builtin_amdgcn_fmed3f(fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh)), 0.0f, 1.0f);
         }
-        if (n_slow) count_slow<U>(wl + 64, wo, n_slow, px, py, q.qx, q.qy, cnt, rech);
+        if (n_slow) count_slow<U>(wl + 64, wo, n_slow, ds, t0, nt, wx, wy, q.qx, q.qy, cnt, rech);
         __builtin_amdgcn_wave_barrier();
     }
 #pragma unroll
@@ -389,9 +430,11 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
 
 // The first group's users are requested before the records are staged, so
 // the two loads overlap.  (Loading a second group ahead, two groups per
-// wave, measured slower: the tail doubles and occupancy drops.)
-template <int U, typename OutT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(U == 4 ? 8 : 1)))
+// wave, measured slower: the tail doubles and occupancy drops.)  W is the
+// occupancy the register allocation is held to (W = 8 with U = 4 spills a
+// few registers per group; W = 1 leaves the allocation free).
+template <int U, int W, typename OutT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W)))
 void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec, const float2* __restrict__ orig,
                  OutT* __restrict__ counts, uint32_t* __restrict__ rech_out)
 {
@@ -403,7 +446,7 @@ void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec, const float2
     const bool lds_recs = nc <= kCountLdsRecs;
     const uint32_t n_groups = (ds.n_tiles + U - 1) / U;
     const uint32_t stride = gridDim.x * kWaves;
-    uint32_t g = blockIdx.x * kWaves + wave;
+    uint32_t g = uniform(blockIdx.x * kWaves + wave);
     float px[U], py[U];
     float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
     if (g < n_groups) load_group<U>(ds, g, px, py, m);
@@ -512,6 +555,24 @@ __global__ __launch_bounds__(kBlock) void k_level_mbr(const float4* __restrict__
         const float4 m = child[c < n_child ? c : p * kFanout];
         const float a = wave_min(m.x), b = wave_min(m.y), cc = wave_max(m.z), d = wave_max(m.w);
         if (lane == 0) parent[p] = make_float4(a, b, cc, d);
+    }
+}
+
+__global__ void k_group_mbr(const float4* __restrict__ tiles, uint32_t n_tiles, uint32_t fan,
+                            float4* __restrict__ groups)
+{
+    const uint32_t n_groups = (n_tiles + fan - 1) / fan;
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n_groups; g += gridDim.x * blockDim.x) {
+        float4 m = tiles[g * fan];
+        const uint32_t e = min(n_tiles, (g + 1) * fan);
+        for (uint32_t t = g * fan + 1; t < e; ++t) {
+            const float4 b = tiles[t];
+            m.x = fminf(m.x, b.x);
+            m.y = fminf(m.y, b.y);
+            m.z = fmaxf(m.z, b.z);
+            m.w = fmaxf(m.w, b.w);
+        }
+        groups[g] = m;
     }
 }
 
@@ -1122,6 +1183,11 @@ void launch_level_mbr(const float4* child, uint32_t n_child, float4* parent, uin
     k_level_mbr<<<grid_for(n_parent, kWaves, 4096), kBlock, 0, st>>>(child, n_child, parent, n_parent);
 }
 
+void launch_group_mbr(const float4* tiles, uint32_t n_tiles, uint32_t fan, float4* groups, hipStream_t st)
+{
+    k_group_mbr<<<grid_for((n_tiles + fan - 1) / fan, 256, 4096), 256, 0, st>>>(tiles, n_tiles, fan, groups);
+}
+
 void launch_plain(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, int32_t* out_ids,
                   uint32_t* counters, int grid, hipStream_t st)
 {
@@ -1137,14 +1203,20 @@ void launch_count_typed(const DsDev& ds, QDev q, const CandRec* rec, const float
     const size_t lds = q.nc <= kCountLdsRecs ? q.nc * sizeof(CandRec) : 0;
     if (variant == kCountValu8) {
         const int g = grid_for((ds.n_tiles + 7) / 8, kWaves, 1 << 20);
-        k_count_all<8, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
+        k_count_all<8, 1, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
+    } else if (variant == kCountValu4Full) {
+        const int g = grid_for((ds.n_tiles + 3) / 4, kWaves, 1 << 20);
+        k_count_all<4, 8, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
     } else {
         const int g = grid_for((ds.n_tiles + 3) / 4, kWaves, 1 << 20);
-        k_count_all<4, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
+        k_count_all<4, 1, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
     }
 }
 
-const char* count_variant_name(int variant) { return variant == kCountValu8 ? "valu8" : "valu4"; }
+const char* count_variant_name(int variant)
+{
+    return variant == kCountValu8 ? "valu8" : variant == kCountValu4Full ? "valu4o8" : "valu4";
+}
 
 bool count_narrow(uint32_t nc) { return nc < 65536u; }
 

@@ -49,6 +49,8 @@ void launch_gather_users(const uint32_t* order, uint32_t n, uint32_t n_pad, cons
                          hipStream_t st);
 void launch_tile_mbr(const float* ux, const float* uy, uint32_t n_tiles, float4* mbr, hipStream_t st);
 void launch_level_mbr(const float4* child, uint32_t n_child, float4* parent, uint32_t n_parent, hipStream_t st);
+// rectangles of groups of fan consecutive tiles (the last group may be short)
+void launch_group_mbr(const float4* tiles, uint32_t n_tiles, uint32_t fan, float4* groups, hipStream_t st);
 
 // plain form: one launch over every tile for one query
 // counters: [0] = ids written, [1 .. kBuckets] = binary64 tests per wave slot
@@ -59,10 +61,12 @@ void launch_plain(const DsDev& ds, QDev q, const CandRec* rec, const float2* ori
 // count-all form: exact per-user count (sorted order), no early exit.
 // rechecks has kBuckets slots.  Counts are 16-bit when count_narrow(nc)
 // (fewer than 2^16 candidates), else 32-bit; the buffer holds n_pad of them.
-// variants: 4 or 8 users per lane (3 instructions per clean test in each)
+// variants: 4 or 8 users per lane, or 4 held to 8 waves per SIMD (3
+// instructions per clean test in each)
 constexpr int kCountValu4 = 0;
 constexpr int kCountValu8 = 1;
-constexpr int kCountVariants = 2;
+constexpr int kCountValu4Full = 2;
+constexpr int kCountVariants = 3;
 const char* count_variant_name(int variant);
 bool count_narrow(uint32_t nc);
 void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, void* counts,
