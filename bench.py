@@ -34,6 +34,12 @@ for _p in (ROOT, os.path.join(ROOT, "rt-rknn_amd")):
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
+# HBM bytes per count-all launch on the 10M-user, 100k-facility k=10 workload,
+# from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known
+# size (profiles/k_count_all_10M_100k.md); the kernel reads every user once and
+# writes one 16-bit count per user, so the figure does not depend on the query
+MEASURED_TRAFFIC = {"valu4": 101.4e6, "valu8": 100.8e6, "valu4o8": 195.2e6}
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -185,7 +191,10 @@ def main():
     achieved, avg_s, flops_launch = variants[best]
     roofline = {"bound": "valu", "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": None, "kernel": kernel_names[best],
+                "traffic": (MEASURED_TRAFFIC.get(best) if (args.users, args.facilities, args.k, args.config)
+                            == (10_000_000, 100_000, 10, 4) else None),
+                "traffic_source": "profiles/k_count_all_10M_100k.md (calibrated PMC, bytes per launch)",
+                "kernel": kernel_names[best],
                 "avg_launch_ms": round(avg_s * 1e3, 4), "flops_per_launch": flops_launch,
                 "variants_tflops": {k: round(v[0], 3) for k, v in variants.items()}}
 
