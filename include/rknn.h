@@ -89,6 +89,17 @@ rk_status rk_dataset_info(const rk_dataset* ds, uint64_t* nu, uint64_t* nf, uint
 rk_status rk_filter(rk_dataset* ds, uint32_t q, uint32_t k,
                     uint32_t* cand, size_t cap, size_t* nc);
 
+/* ---- the same influence-zone filter on the device, one wave per query over
+ *      a facility grid kept in HBM (no reference counterpart: the reference
+ *      filters on the host only).  For nq queries (q[i], k[i]), k[i] <= 64,
+ *      writes query i's candidate facility indices to
+ *      cand[i * cap_per_query ...] and their number to nc[i].  The list may
+ *      differ from rk_filter's (another visiting order) but is sufficient in
+ *      the same sense: every query form gives the same result over either.
+ *      RK_ERR_CAPACITY if a list exceeds cap_per_query (nc holds the sizes). ---- */
+rk_status rk_filter_device(rk_dataset* ds, const uint32_t* q, const uint32_t* k, size_t nq,
+                           uint32_t* cand, size_t cap_per_query, uint64_t* nc);
+
 /* Same filter on host arrays, no device involved (builds the facility index
  * per call).  Users only bound the zone (their bounding box). */
 rk_status rk_filter_points(const float* fx, const float* fy, size_t nf,

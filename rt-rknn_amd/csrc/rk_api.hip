@@ -293,6 +293,10 @@ rk_status rk_dataset_create(rk_ctx* c, const float* ux, const float* uy, const i
         RK_HIP(c, hipMemcpyAsync(ds->fy.p, fy, nf * sizeof(float), hipMemcpyHostToDevice, st));
     }
     ds->index.reset(new FacilityIndex(fx, fy, nf, ds->ubox));
+    {
+        const rk_status gs = build_facility_grid(ds.get());
+        if (gs != RK_OK) return gs;
+    }
 
     ds->n_tiles = (uint32_t)((nu + kTile - 1) / kTile);
     ds->n_pad = ds->n_tiles * kTile;
@@ -389,6 +393,7 @@ rk_status rk_dataset_info(const rk_dataset* ds, uint64_t* nu, uint64_t* nf, uint
                      ds->fy.bytes() + ds->out.bytes();
         for (int l = 0; l < kMaxLevels; ++l) b += ds->mbr[l].bytes();
         b += ds->gmbr4.bytes() + ds->gmbr8.bytes();
+        b += ds->g_start.bytes() + ds->g_fid.bytes() + ds->g_fx.bytes() + ds->g_fy.bytes() + ds->g_dir.bytes();
         *device_bytes = b;
     }
     return RK_OK;

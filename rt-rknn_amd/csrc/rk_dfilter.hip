@@ -1,0 +1,520 @@
+// rk_dfilter.hip -- the influence-zone filter on the device.
+//
+// The same sector-bound filter as the host's FacilityIndex::filter
+// (rk_filter.cpp; it replaces the bisector generation of
+// rknn-inf/inf.cpp:201-585), run by one wave per query over a facility grid
+// kept in HBM, so a batch of (q, k) requests goes from query to result ids
+// without leaving the device.
+//
+//  * Facilities are bucketed in a uniform grid (CSR: cell start offsets,
+//    coordinates and original indices sorted by cell).  The wave visits the
+//    cells in rings of growing Chebyshev radius around q's cell; a ring r >= 2
+//    is at least (r - 1) cell sides from q, so the walk stops once half that
+//    distance (shrunk by the oracle's rounding allowance) reaches the largest
+//    sector bound, exactly the host stream's stopping rule.
+//  * A ring's facilities are tested 64 at a time (lane = facility) against
+//    the sector bounds of the moment: a facility is relevant when its
+//    half-plane, loosened by the oracle's error, reaches inside some sector's
+//    bound.  The test walks the sectors outward from the one holding g = f - q,
+//    where the half-plane's entry distance is smallest and grows on both
+//    sides, so it stops early.
+//  * Each relevant facility joins the candidate list and then updates the
+//    sectors (lane = sector): every sector keeps the k smallest exit distances
+//    (tightened by the oracle's error, stored as fp32 rounded up) in an LDS
+//    max-heap, and once full its bound is the heap's largest.
+//
+// The visiting order differs from the host's nearest-first stream, and the
+// relevance test of a batch uses the bounds from before the batch; neither
+// changes the sufficiency argument (a pruned facility is pruned against
+// bounds backed by k candidates already in the list, and bounds only
+// shrink), so the candidate list may differ from the host's but every
+// count over it gives the oracle's result.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "rk_dfilter.hpp"
+#include "rk_internal.hpp"
+#include "rknn.h"
+
+namespace rk {
+
+namespace {
+
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+__device__ __forceinline__ double readlane_d(double v, uint32_t l)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double wave_max_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+
+// exclusive prefix sum over the wave; *total = the sum over all lanes
+__device__ __forceinline__ uint32_t wave_scan_excl(uint32_t v, uint32_t* total)
+{
+    const uint32_t lane = __lane_id();
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    *total = (uint32_t)__shfl((int)x, 63);
+    return x - v;
+}
+
+// Cell i of the ring of Chebyshev radius r around (cx, cy): 8r cells, four
+// sides of 2r cells each (r = 0: the centre).
+__device__ __forceinline__ void ring_cell(int cx, int cy, int r, uint32_t i, int& x, int& y)
+{
+    if (r == 0) {
+        x = cx;
+        y = cy;
+        return;
+    }
+    const int side = (int)i / (2 * r), pos = (int)i % (2 * r);
+    switch (side) {
+    case 0: x = cx - r + pos; y = cy - r; break;
+    case 1: x = cx + r; y = cy - r + pos; break;
+    case 2: x = cx + r - pos; y = cy + r; break;
+    default: x = cx - r; y = cy + r - pos; break;
+    }
+}
+
+// The binary64 oracle's error on D(u,f) - D(u,q) over the users' box,
+// doubled (as in the host filter).
+__device__ __forceinline__ double df_delta(double rbox2, double gg)
+{
+    return 8.02 * 0x1p-53 * (3.0 * rbox2 + 2.0 * gg) + 0x1p-100;
+}
+
+// Loosened smallest entry distance of half-plane g into sector s.
+__device__ __forceinline__ double df_loose(const double* ex, const double* ey, int s, double gx, double gy, double gg,
+                                           double glen, double itol, double dl)
+{
+    const double ca = ex[s] * gx + ey[s] * gy, cb = ex[s + 1] * gx + ey[s + 1] * gy;
+    const double xa = ex[s] * gy - ey[s] * gx, xb = ex[s + 1] * gy - ey[s + 1] * gx;
+    double rmin;
+    if (xa >= -itol && -xb >= -itol) {
+        rmin = 0.5 * glen;
+    } else {
+        const double ra = ca > 0.0 ? gg / (2.0 * ca) : INFINITY;
+        const double rb = cb > 0.0 ? gg / (2.0 * cb) : INFINITY;
+        rmin = fmin(ra, rb);
+    }
+    return rmin * (1.0 - dl) * (1.0 - 1e-12);
+}
+
+__device__ __forceinline__ bool df_inside(const double* ex, const double* ey, int s, double gx, double gy, double itol)
+{
+    const double xa = ex[s] * gy - ey[s] * gx, xb = ex[s + 1] * gy - ey[s + 1] * gx;
+    return xa >= -itol && -xb >= -itol;
+}
+
+// Is facility g relevant against the bounds?  R is the largest bound.
+__device__ bool df_relevant(const double* ex, const double* ey, const double* bound, double gx, double gy, double gg,
+                            double dl, double R)
+{
+    if (dl >= 1.0) return true;
+    const double glen = sqrt(gg);
+    if (0.5 * glen * (1.0 - dl) * (1.0 - 1e-12) >= R) return false; // every sector's entry is at least |g|/2
+    const double itol = 1e-9 * glen;
+    float ang = atan2f((float)gy, (float)gx);
+    if (ang < 0.f) ang += (float)kTwoPi;
+    int s0 = min(max((int)(ang * (float)(kDfSectors / kTwoPi)), 0), kDfSectors - 1);
+    if (!df_inside(ex, ey, s0, gx, gy, itol)) {
+        const int sl = (s0 + kDfSectors - 1) % kDfSectors, sr = (s0 + 1) % kDfSectors;
+        if (df_inside(ex, ey, sl, gx, gy, itol)) {
+            s0 = sl;
+        } else if (df_inside(ex, ey, sr, gx, gy, itol)) {
+            s0 = sr;
+        } else { // no sector found near the estimate: test them all
+            for (int s = 0; s < kDfSectors; ++s)
+                if (df_loose(ex, ey, s, gx, gy, gg, glen, itol, dl) < bound[s]) return true;
+            return false;
+        }
+    }
+    // from the sector holding g outwards the entry distance only grows
+    const double stop = R * (1.0 + 1e-9);
+    bool right = true, left = true;
+    for (int d = 0; d < kDfSectors / 2 && (right || left); ++d) {
+        if (right) {
+            const int s = (s0 + d) % kDfSectors;
+            const double l = df_loose(ex, ey, s, gx, gy, gg, glen, itol, dl);
+            if (l < bound[s]) return true;
+            right = l < stop;
+        }
+        if (left) {
+            const int s = (s0 + kDfSectors - 1 - d) % kDfSectors;
+            const double l = df_loose(ex, ey, s, gx, gy, gg, glen, itol, dl);
+            if (l < bound[s]) return true;
+            left = l < stop;
+        }
+    }
+    return false;
+}
+
+// fp32 value >= v
+__device__ __forceinline__ float round_up_f(double v)
+{
+    float f = (float)v;
+    if ((double)f < v) f = nextafterf(f, INFINITY);
+    return f;
+}
+
+__global__ __launch_bounds__(64) void k_dfilter(FacGridDev g, const uint32_t* __restrict__ qs,
+                                                const uint32_t* __restrict__ ks, uint32_t nq,
+                                                uint32_t* __restrict__ cand, uint32_t cap,
+                                                uint32_t* __restrict__ ncand)
+{
+    __shared__ float s_heap[kDfMaxK][kDfSectors]; // [i][sector]: sector's max-heap of k exit distances
+    __shared__ double s_bound[kDfSectors];
+    __shared__ double s_ex[kDfSectors + 1], s_ey[kDfSectors + 1];
+    __shared__ uint32_t s_st[64], s_pre[64];
+    const uint32_t lane = threadIdx.x;
+    s_ex[lane] = g.ex[lane];
+    s_ey[lane] = g.ey[lane];
+    if (lane == 0) {
+        s_ex[kDfSectors] = g.ex[kDfSectors];
+        s_ey[kDfSectors] = g.ey[kDfSectors];
+    }
+    __syncthreads();
+    const double ax = s_ex[lane], ay = s_ey[lane], bxs = s_ex[lane + 1], bys = s_ey[lane + 1];
+    const double bx0 = g.box[0], by0 = g.box[1], bx1 = g.box[2], by1 = g.box[3];
+    const int G = (int)g.G;
+    for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+        const uint32_t q = qs[qi];
+        const uint32_t k = min(ks[qi], kDfMaxK);
+        const double qx = g.ofx[q], qy = g.ofy[q];
+        uint32_t* out = cand + (uint64_t)qi * cap;
+        uint32_t n = 0;
+        // farthest squared distance from q to the users' box
+        double rbox2 = 0.0;
+        const double cx4[4] = {bx0, bx1, bx0, bx1}, cy4[4] = {by0, by0, by1, by1};
+        for (int i = 0; i < 4; ++i) {
+            const double dx = cx4[i] - qx, dy = cy4[i] - qy;
+            rbox2 = fmax(rbox2, dx * dx + dy * dy);
+        }
+        // this lane's sector: the farthest point of the users' box inside it
+        double best = -1.0;
+        {
+            const double tol = 1e-12;
+            if (qx >= bx0 && qx <= bx1 && qy >= by0 && qy <= by1) best = 0.0;
+            for (int i = 0; i < 4; ++i) {
+                const double dx = cx4[i] - qx, dy = cy4[i] - qy;
+                const double sc = tol * (fabs(dx) + fabs(dy));
+                if (ax * dy - ay * dx >= -sc && dx * bys - dy * bxs >= -sc) best = fmax(best, dx * dx + dy * dy);
+            }
+            const double span = (bx1 - bx0) + (by1 - by0) + 1.0;
+            for (int e = 0; e < 2; ++e) {
+                const double ux = e ? bxs : ax, uy = e ? bys : ay;
+                if (ux != 0.0) {
+                    for (int w = 0; w < 2; ++w) {
+                        const double X = w ? bx1 : bx0;
+                        const double t = (X - qx) / ux;
+                        const double y = qy + t * uy;
+                        if (t >= 0.0 && y >= by0 - tol * span && y <= by1 + tol * span) best = fmax(best, t * t);
+                    }
+                }
+                if (uy != 0.0) {
+                    for (int w = 0; w < 2; ++w) {
+                        const double Y = w ? by1 : by0;
+                        const double t = (Y - qy) / uy;
+                        const double x = qx + t * ux;
+                        if (t >= 0.0 && x >= bx0 - tol * span && x <= bx1 + tol * span) best = fmax(best, t * t);
+                    }
+                }
+            }
+        }
+        const double capv = best < 0.0 ? 0.0 : sqrt(best) * (1.0 + 1e-9) + 1e-9;
+        double bound = capv;
+        s_bound[lane] = bound;
+        double R = wave_max_d(bound);
+        uint32_t hs = 0; // heap size of this lane's sector
+        __syncthreads();
+        const bool have_users = bx0 <= bx1 && by0 <= by1;
+        const int cx = min(max((int)floor((qx - g.x0) / g.cs), 0), G - 1);
+        const int cy = min(max((int)floor((qy - g.y0) / g.cs), 0), G - 1);
+        for (int r = 0; have_users && k > 0; ++r) {
+            if (r >= 2) {
+                const double dmin = (double)(r - 1) * g.cs * (1.0 - 1e-9);
+                const double key = dmin * dmin;
+                const double dl = df_delta(rbox2, key) / key;
+                const double lower = dl >= 1.0 ? 0.0 : 0.5 * dmin * (1.0 - dl) * (1.0 - 1e-12);
+                if (lower >= R) break;
+            }
+            if (r > 0 && cx - r < 0 && cx + r >= G && cy - r < 0 && cy + r >= G) break; // past every cell
+            const uint32_t ncells = r == 0 ? 1u : 8u * (uint32_t)r;
+            for (uint32_t c0 = 0; c0 < ncells; c0 += 64) {
+                uint32_t st = 0, cnt = 0;
+                if (c0 + lane < ncells) {
+                    int x, y;
+                    ring_cell(cx, cy, r, c0 + lane, x, y);
+                    if (x >= 0 && x < G && y >= 0 && y < G) {
+                        const uint32_t c = (uint32_t)y * (uint32_t)G + (uint32_t)x;
+                        st = g.start[c];
+                        cnt = g.start[c + 1] - st;
+                    }
+                }
+                uint32_t total = 0;
+                const uint32_t pre = wave_scan_excl(cnt, &total);
+                s_st[lane] = st;
+                s_pre[lane] = pre;
+                __syncthreads();
+                for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+                    const uint32_t j = j0 + lane;
+                    bool rel = false;
+                    double gx = 0.0, gy = 0.0, gg = 0.0, dl = 0.0;
+                    uint32_t f = 0;
+                    if (j < total) {
+                        int lo = 0, hi = 63; // the last cell whose prefix is <= j (it holds slot j)
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (s_pre[mid] <= j) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        const uint32_t slot = s_st[lo] + (j - s_pre[lo]);
+                        f = g.fid[slot];
+                        gx = (double)g.fx[slot] - qx;
+                        gy = (double)g.fy[slot] - qy;
+                        gg = gx * gx + gy * gy;
+                        if (f != q && gg != 0.0) { // co-located with q: never strictly closer
+                            dl = df_delta(rbox2, gg) / gg;
+                            rel = df_relevant(s_ex, s_ey, s_bound, gx, gy, gg, dl, R);
+                        }
+                    }
+                    uint64_t rm = __builtin_amdgcn_ballot_w64(rel);
+                    while (rm) {
+                        const uint32_t l = (uint32_t)__ffsll((unsigned long long)rm) - 1;
+                        rm &= rm - 1;
+                        const double fgx = readlane_d(gx, l), fgy = readlane_d(gy, l);
+                        const double fgg = readlane_d(gg, l), fdl = readlane_d(dl, l);
+                        const uint32_t ff = __builtin_amdgcn_readlane(f, l);
+                        if (lane == 0 && n < cap) out[n] = ff;
+                        ++n;
+                        // lane = sector: the facility's exit distance from this sector
+                        const double ca = ax * fgx + ay * fgy, cb = bxs * fgx + bys * fgy;
+                        bool changed = false;
+                        if (ca > 0.0 && cb > 0.0) {
+                            const double rmax = fmax(fgg / (2.0 * ca), fgg / (2.0 * cb)) * (1.0 + fdl) * (1.0 + 1e-12);
+                            const float v = round_up_f(rmax);
+                            bool upd = false;
+                            if (hs < k) { // push, sift up
+                                uint32_t i = hs++;
+                                while (i > 0) {
+                                    const uint32_t p = (i - 1) >> 1;
+                                    const float pv = s_heap[p][lane];
+                                    if (pv >= v) break;
+                                    s_heap[i][lane] = pv;
+                                    i = p;
+                                }
+                                s_heap[i][lane] = v;
+                                upd = true;
+                            } else if (v < s_heap[0][lane]) { // replace the largest, sift down
+                                uint32_t i = 0;
+                                while (true) {
+                                    const uint32_t c1 = 2 * i + 1, c2 = c1 + 1;
+                                    if (c1 >= hs) break;
+                                    uint32_t c = c1;
+                                    float cv = s_heap[c1][lane];
+                                    if (c2 < hs && s_heap[c2][lane] > cv) {
+                                        c = c2;
+                                        cv = s_heap[c2][lane];
+                                    }
+                                    if (cv <= v) break;
+                                    s_heap[i][lane] = cv;
+                                    i = c;
+                                }
+                                s_heap[i][lane] = v;
+                                upd = true;
+                            }
+                            if (upd && hs == k) {
+                                const double b = fmin(capv, (double)s_heap[0][lane]);
+                                if (b < bound) {
+                                    bound = b;
+                                    s_bound[lane] = b;
+                                    changed = true;
+                                }
+                            }
+                        }
+                        if (__builtin_amdgcn_ballot_w64(changed)) R = wave_max_d(bound);
+                    }
+                    __syncthreads(); // s_bound is read by the next batch's relevance tests
+                }
+                __syncthreads(); // s_st / s_pre are rewritten for the next chunk
+            }
+        }
+        if (lane == 0) ncand[qi] = n;
+        __syncthreads();
+    }
+}
+
+// Per-query device view of a device-filtered batch: QDev from the filter's
+// counts (a count above the segment capacity marks the query overflowed).
+__global__ void k_dfilter_setup(FacGridDev g, const uint32_t* __restrict__ qs, const uint32_t* __restrict__ ks,
+                                uint32_t nq, const uint32_t* __restrict__ ncand, uint32_t cap, uint32_t nu,
+                                QDev* __restrict__ qd, double2* __restrict__ wb)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        const uint32_t q = qs[i];
+        QDev d;
+        d.qx = g.ofx[q];
+        d.qy = g.ofy[q];
+        d.k = ks[i];
+        d.nc = min(ncand[i], cap);
+        d.off = (uint64_t)i * cap;
+        d.trivial = (d.k > d.nc || nu == 0) ? 1u : 0u;
+        d.pad = 0;
+        qd[i] = d;
+        double wx = 0.0, wy = 0.0;
+        if (nu) axis_bounds(g.box[0], g.box[1], g.box[2], g.box[3], d.qx, d.qy, &wx, &wy);
+        wb[i] = make_double2(wx, wy);
+    }
+}
+
+} // namespace
+
+rk_status build_facility_grid(rk_dataset* ds)
+{
+    rk_ctx* c = ds->ctx;
+    const size_t nf = ds->nf;
+    const float* fx = ds->hfx.data();
+    const float* fy = ds->hfy.data();
+    FacGridDev& v = ds->fgrid;
+    v = FacGridDev{};
+    double x0 = 0, y0 = 0, x1 = 0, y1 = 0;
+    if (nf) {
+        x0 = x1 = fx[0];
+        y0 = y1 = fy[0];
+        for (size_t i = 1; i < nf; ++i) {
+            x0 = std::min(x0, (double)fx[i]);
+            y0 = std::min(y0, (double)fy[i]);
+            x1 = std::max(x1, (double)fx[i]);
+            y1 = std::max(y1, (double)fy[i]);
+        }
+    }
+    // about two facilities per cell
+    const uint32_t G = (uint32_t)std::max<double>(1.0, std::ceil(std::sqrt((double)nf / 2.0)));
+    const double ext = std::max(x1 - x0, y1 - y0);
+    const double cs = ext > 0 ? ext / G : 1.0;
+    std::vector<uint32_t> cell(nf), start((size_t)G * G + 1, 0);
+    for (size_t i = 0; i < nf; ++i) {
+        const int ix = std::min(std::max((int)std::floor(((double)fx[i] - x0) / cs), 0), (int)G - 1);
+        const int iy = std::min(std::max((int)std::floor(((double)fy[i] - y0) / cs), 0), (int)G - 1);
+        cell[i] = (uint32_t)iy * G + (uint32_t)ix;
+        ++start[cell[i] + 1];
+    }
+    for (size_t i = 1; i < start.size(); ++i) start[i] += start[i - 1];
+    std::vector<uint32_t> pos(start.begin(), start.end() - 1), gid(nf);
+    std::vector<float> gx(nf), gy(nf);
+    for (size_t i = 0; i < nf; ++i) {
+        const uint32_t p = pos[cell[i]]++;
+        gid[p] = (uint32_t)i;
+        gx[p] = fx[i];
+        gy[p] = fy[i];
+    }
+    double ex[kDfSectors + 1], ey[kDfSectors + 1];
+    for (int s = 0; s < kDfSectors; ++s) {
+        ex[s] = std::cos(kTwoPi * s / kDfSectors);
+        ey[s] = std::sin(kTwoPi * s / kDfSectors);
+    }
+    ex[kDfSectors] = ex[0];
+    ey[kDfSectors] = ey[0];
+    hipStream_t st = c->stream;
+    RK_HIP(c, ds->g_start.reserve(start.size()));
+    RK_HIP(c, ds->g_fid.reserve(std::max<size_t>(1, nf)));
+    RK_HIP(c, ds->g_fx.reserve(std::max<size_t>(1, nf)));
+    RK_HIP(c, ds->g_fy.reserve(std::max<size_t>(1, nf)));
+    RK_HIP(c, ds->g_dir.reserve(2 * (kDfSectors + 1)));
+    RK_HIP(c, hipMemcpyAsync(ds->g_start.p, start.data(), start.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    if (nf) {
+        RK_HIP(c, hipMemcpyAsync(ds->g_fid.p, gid.data(), nf * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        RK_HIP(c, hipMemcpyAsync(ds->g_fx.p, gx.data(), nf * sizeof(float), hipMemcpyHostToDevice, st));
+        RK_HIP(c, hipMemcpyAsync(ds->g_fy.p, gy.data(), nf * sizeof(float), hipMemcpyHostToDevice, st));
+    }
+    RK_HIP(c, hipMemcpyAsync(ds->g_dir.p, ex, sizeof ex, hipMemcpyHostToDevice, st));
+    RK_HIP(c, hipMemcpyAsync(ds->g_dir.p + kDfSectors + 1, ey, sizeof ey, hipMemcpyHostToDevice, st));
+    RK_HIP(c, hipStreamSynchronize(st)); // the host vectors die here
+    v.fx = ds->g_fx.p;
+    v.fy = ds->g_fy.p;
+    v.fid = ds->g_fid.p;
+    v.start = ds->g_start.p;
+    v.ofx = ds->fx.p;
+    v.ofy = ds->fy.p;
+    v.ex = ds->g_dir.p;
+    v.ey = ds->g_dir.p + kDfSectors + 1;
+    v.x0 = x0;
+    v.y0 = y0;
+    v.cs = cs;
+    v.G = G;
+    v.nf = (uint32_t)nf;
+    for (int i = 0; i < 4; ++i) v.box[i] = ds->ubox[i];
+    return RK_OK;
+}
+
+void launch_dfilter(const FacGridDev& g, const uint32_t* q, const uint32_t* k, uint32_t nq, uint32_t* cand,
+                    uint32_t cap, uint32_t* ncand, hipStream_t st)
+{
+    if (!nq) return;
+    k_dfilter<<<nq < 65535 ? nq : 65535, 64, 0, st>>>(g, q, k, nq, cand, cap, ncand);
+}
+
+void launch_dfilter_setup(const FacGridDev& g, const uint32_t* q, const uint32_t* k, uint32_t nq,
+                          const uint32_t* ncand, uint32_t cap, uint32_t nu, QDev* qd, double2* wb, hipStream_t st)
+{
+    if (!nq) return;
+    const uint32_t blocks = std::min<uint32_t>((nq + 255) / 256, 1024);
+    k_dfilter_setup<<<blocks, 256, 0, st>>>(g, q, k, nq, ncand, cap, nu, qd, wb);
+}
+
+} // namespace rk
+
+using namespace rk;
+
+extern "C" rk_status rk_filter_device(rk_dataset* ds, const uint32_t* q, const uint32_t* k, size_t nq, uint32_t* cand,
+                                      size_t cap_per_query, uint64_t* nc)
+{
+    if (!ds || !nc || (nq && (!q || !k)) || (nq && cap_per_query && !cand)) return RK_ERR_INVALID_ARG;
+    rk_ctx* c = ds->ctx;
+    if (nq > (1u << 24)) return fail(c, RK_ERR_INVALID_ARG, "too many queries in one call");
+    if (cap_per_query > 0xffffffffull) return fail(c, RK_ERR_INVALID_ARG, "capacity per query too large");
+    for (size_t i = 0; i < nq; ++i) {
+        if (q[i] >= ds->nf) return fail(c, RK_ERR_INVALID_ARG, "query index q must be < number of facilities");
+        if (k[i] == 0) return fail(c, RK_ERR_INVALID_ARG, "k must be a positive integer");
+        if (k[i] > kDfMaxK) return fail(c, RK_ERR_INVALID_ARG, "the device filter takes k <= 64");
+    }
+    if (nq == 0) return RK_OK;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    RK_HIP(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const uint32_t capq = (uint32_t)std::max<size_t>(1, cap_per_query);
+    DevBuf<uint32_t> dq, dk, dc, dn;
+    RK_HIP(c, dq.reserve(nq));
+    RK_HIP(c, dk.reserve(nq));
+    RK_HIP(c, dn.reserve(nq));
+    RK_HIP(c, dc.reserve((uint64_t)nq * capq));
+    RK_HIP(c, hipMemcpyAsync(dq.p, q, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    RK_HIP(c, hipMemcpyAsync(dk.p, k, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    launch_dfilter(ds->fgrid, dq.p, dk.p, (uint32_t)nq, dc.p, capq, dn.p, st);
+    RK_HIP(c, hipGetLastError());
+    std::vector<uint32_t> cnt(nq);
+    RK_HIP(c, hipMemcpyAsync(cnt.data(), dn.p, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    if (cap_per_query)
+        RK_HIP(c, hipMemcpyAsync(cand, dc.p, (uint64_t)nq * capq * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    RK_HIP(c, hipStreamSynchronize(st));
+    bool over = false;
+    for (size_t i = 0; i < nq; ++i) {
+        nc[i] = cnt[i];
+        over |= cnt[i] > cap_per_query;
+    }
+    return over ? fail(c, RK_ERR_CAPACITY, "a query's candidate list exceeds cap_per_query") : RK_OK;
+}

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "rk_common.hpp"
+#include "rk_dfilter.hpp"
 #include "rk_filter.hpp"
 #include "rk_kernels.hpp"
 #include "rknn.h"
@@ -112,6 +113,11 @@ struct rk_dataset {
     uint32_t id_bits = 32;               // ids are non-negative and below 2^id_bits (32: any int32)
     std::unique_ptr<rk::FacilityIndex> index;
     rk::DsDev view{};
+    // facility grid of the device filter
+    rk::FacGridDev fgrid{};
+    rk::DevBuf<uint32_t> g_start, g_fid;
+    rk::DevBuf<float> g_fx, g_fy;
+    rk::DevBuf<double> g_dir;
     // scratch of the plain / count-all forms
     rk::DevBuf<rk::CandRec> rec;
     rk::DevBuf<float2> orig;
@@ -167,6 +173,8 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq);
 // picks the traversal block shape for a filled batch (autotuner); caller
 // holds the context lock
 rk_status batch_tune(rk_batch* b);
+// uploads the dataset's facility grid for the device filter (rk_dfilter.hip)
+rk_status build_facility_grid(rk_dataset* ds);
 // autotuner memory (rk_tune.cpp): the variant chosen for a workload class,
 // -1 if none yet
 std::string device_arch(rk_ctx* c);

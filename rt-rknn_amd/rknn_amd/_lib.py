@@ -74,6 +74,7 @@ SIGNATURES = {
     "rk_filter": (ST, [P, C.c_uint32, C.c_uint32, U32P, C.c_size_t, SZP]),
     "rk_filter_points": (ST, [F32P, F32P, C.c_size_t, F32P, F32P, C.c_size_t, C.c_uint32, C.c_uint32, U32P,
                               C.c_size_t, SZP]),
+    "rk_filter_device": (ST, [P, U32P, U32P, C.c_size_t, U32P, C.c_size_t, U64P]),
     "rk_query_plain": (ST, [P, C.c_uint32, C.c_uint32, U32P, C.c_size_t, I32P, C.c_size_t, SZP, TP]),
     "rk_query_bruteforce": (ST, [P, C.c_uint32, C.c_uint32, I32P, C.c_size_t, SZP, TP]),
     "rk_query": (ST, [P, C.c_uint32, C.c_uint32, I32P, C.c_size_t, SZP, TP]),

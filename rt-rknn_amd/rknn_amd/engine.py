@@ -100,6 +100,19 @@ class Dataset:
         self.ctx.check(self._lib.rk_filter(self.h, q, k, _ptr(out, C.c_uint32), len(out), C.byref(n)))
         return out[: n.value].copy()
 
+    def filter_device(self, qs, ks, cap=None):
+        """Candidate lists of the device filter (rk_filter_device), one
+        uint32 array per query."""
+        qs, ks = _u32(qs), _u32(ks)
+        n = len(qs)
+        nc = np.zeros(max(1, n), dtype=np.uint64)
+        cap = cap if cap is not None else min(max(1, self.nf), 16384)
+        out = np.zeros(max(1, n * cap), dtype=np.uint32)
+        st = self._lib.rk_filter_device(self.h, _ptr(qs, C.c_uint32), _ptr(ks, C.c_uint32), n,
+                                        _ptr(out, C.c_uint32), cap, _ptr(nc, C.c_uint64))
+        self.ctx.check(st)
+        return [out[i * cap: i * cap + int(nc[i])].copy() for i in range(n)]
+
     def _ids_call(self, fn, *args):
         cap = max(1, self.nu)
         out = np.empty(cap, dtype=np.int32)
