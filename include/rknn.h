@@ -157,6 +157,16 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
 rk_status rk_batch_run(rk_batch* b, double* device_seconds);
 rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets);
 rk_status rk_batch_destroy(rk_batch* b);
+/* Device-filtered batch: the batch holds (q[i], k[i]) pairs, k[i] <= 64, and
+ * each run filters on the device (rk_filter_device's filter) before the
+ * traversal, all in one graph, so a run goes from queries to result ids
+ * without the host.  run returns RK_ERR_CAPACITY if a candidate list
+ * exceeds its device segment (4096 facilities); rk_query_batch then filters
+ * on the host instead.  rk_query_batch uses this form for queries without
+ * candidate lists and k <= 64 unless RK_FILTER=host.  No reference
+ * counterpart. */
+rk_status rk_batch_prepare_device(rk_dataset* ds, const uint32_t* q, const uint32_t* k, size_t nq,
+                                  rk_batch** out);
 /* Work counters of the last run: items[l] = tree nodes of level l that
  * entered a query's frontier (0 = leaf tiles tested; n_levels entries, at
  * most 8), result ids, (user, candidate) tests settled in binary64. */

@@ -33,7 +33,7 @@ rk_status build_graph(rk_batch* b)
     rk_ctx* c = ds->ctx;
     hipStream_t st = c->stream;
     RK_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    launch_batch_query(ds->view, b->view, b->shape, st);
+    batch_launch(b, st);
     hipError_t e = hipStreamEndCapture(st, &b->graph);
     if (e != hipSuccess) return fail_hip(c, e, "hipStreamEndCapture", __LINE__);
     RK_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
@@ -198,10 +198,128 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
     return RK_OK;
 }
 
+void batch_launch(rk_batch* b, hipStream_t st)
+{
+    rk_dataset* ds = b->ds;
+    if (b->dfilter) {
+        launch_dfilter(ds->fgrid, b->dq.p, b->dk.p, b->nq, b->cand.p, b->cand_cap, b->dn.p, st);
+        launch_dfilter_setup(ds->fgrid, b->dq.p, b->dk.p, b->nq, b->dn.p, b->cand_cap, (uint32_t)ds->nu, b->q.p,
+                             b->wb.p, st);
+    }
+    launch_batch_query(ds->view, b->view, b->shape, st);
+}
+
+rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, size_t nq)
+{
+    rk_dataset* ds = b->ds;
+    rk_ctx* c = ds->ctx;
+    if (nq > (1u << 24)) return fail(c, RK_ERR_INVALID_ARG, "too many queries in one batch");
+    for (size_t i = 0; i < nq; ++i) {
+        if (q[i] >= ds->nf) return fail(c, RK_ERR_INVALID_ARG, "query index q must be < number of facilities");
+        if (k[i] == 0) return fail(c, RK_ERR_INVALID_ARG, "k must be a positive integer");
+        if (k[i] > kDfMaxK) return fail(c, RK_ERR_INVALID_ARG, "the device filter takes k <= 64");
+    }
+    b->nq = (uint32_t)nq;
+    b->ran = false;
+    b->dfilter = true;
+    b->filter_s = 0.0;
+    b->total_cand = 0;
+    b->hq.assign(nq, QDev{});
+    b->h_dn.assign(nq, 0);
+    // candidate segment per query: every facility at most, 4096 slots at most
+    b->cand_cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ds->nf, kDfBatchCand));
+    const uint64_t slots = std::max<uint64_t>(1, (uint64_t)nq * b->cand_cap);
+
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    RK_HIP(c, hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    RK_HIP(c, b->dq.reserve(nq));
+    RK_HIP(c, b->dk.reserve(nq));
+    RK_HIP(c, b->dn.reserve(nq));
+    RK_HIP(c, b->q.reserve(nq));
+    RK_HIP(c, b->wb.reserve(nq));
+    RK_HIP(c, b->cand.reserve(slots));
+    RK_HIP(c, b->rec.reserve(slots));
+    RK_HIP(c, b->orig.reserve(slots));
+    RK_HIP(c, b->qcount.reserve(nq));
+    RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
+    RK_HIP(c, b->qoff.reserve(nq + 1));
+    RK_HIP(c, b->h_qcount.reserve(nq));
+    uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? kPoolBudget / nq : 1));
+    seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
+    RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
+    if (nq) {
+        RK_HIP(c, hipMemcpyAsync(b->dq.p, q, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        RK_HIP(c, hipMemcpyAsync(b->dk.p, k, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    }
+    RK_HIP(c, hipStreamSynchronize(st));
+
+    BatchDev& v = b->view;
+    v.q = b->q.p;
+    v.nq = b->nq;
+    v.seg_cap = (uint32_t)seg_cap;
+    v.cand = b->cand.p;
+    v.fx = ds->fx.p;
+    v.fy = ds->fy.p;
+    v.wbound = b->wb.p;
+    v.rec = b->rec.p;
+    v.orig = b->orig.p;
+    v.seg = b->seg.p;
+    v.qcount = b->qcount.p;
+    v.qstat = b->qstat.p;
+    v.qoff = b->qoff.p;
+    v.out = b->out.p;
+    v.id_bits = ds->id_bits;
+    return RK_OK;
+}
+
+// After a device-filtered run: the per-query views (for fetch and the
+// autotuner's class) and the candidate counts (overflow check).
+rk_status batch_download_views(rk_batch* b)
+{
+    rk_ctx* c = b->ds->ctx;
+    hipStream_t st = c->stream;
+    if (!b->nq) return RK_OK;
+    RK_HIP(c, hipMemcpyAsync(b->hq.data(), b->q.p, b->nq * sizeof(QDev), hipMemcpyDeviceToHost, st));
+    RK_HIP(c, hipMemcpyAsync(b->h_dn.data(), b->dn.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    RK_HIP(c, hipStreamSynchronize(st));
+    b->cand_overflow = false;
+    b->total_cand = 0;
+    for (uint32_t i = 0; i < b->nq; ++i) {
+        b->cand_overflow |= b->h_dn[i] > b->cand_cap;
+        b->total_cand += b->hq[i].nc;
+    }
+    return RK_OK;
+}
 
 } // namespace rk
 
 extern "C" {
+
+rk_status rk_batch_prepare_device(rk_dataset* ds, const uint32_t* q, const uint32_t* k, size_t nq, rk_batch** out)
+{
+    if (!ds || !out || (nq && (!q || !k))) return RK_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::unique_ptr<rk_batch> b(new rk_batch());
+    b->ds = ds;
+    rk_status s = batch_fill_device(b.get(), q, k, nq);
+    if (s != RK_OK) return s;
+    if (ds->nu && nq) {
+        rk_ctx* c = ds->ctx;
+        std::lock_guard<std::recursive_mutex> lk(c->mu);
+        // one filter pass first: the autotuner times the traversal on these lists
+        launch_dfilter(ds->fgrid, b->dq.p, b->dk.p, b->nq, b->cand.p, b->cand_cap, b->dn.p, c->stream);
+        launch_dfilter_setup(ds->fgrid, b->dq.p, b->dk.p, b->nq, b->dn.p, b->cand_cap, (uint32_t)ds->nu, b->q.p,
+                             b->wb.p, c->stream);
+        RK_HIP(c, hipGetLastError());
+        s = batch_download_views(b.get());
+        if (s == RK_OK) s = batch_tune(b.get());
+        if (s == RK_OK) s = build_graph(b.get());
+        if (s != RK_OK) return s;
+    }
+    *out = b.release();
+    return RK_OK;
+}
 
 rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, rk_batch** out)
 {
@@ -236,7 +354,7 @@ rk_status rk_batch_run(rk_batch* b, double* device_seconds)
     }
     RK_HIP(c, hipEventRecord(c->ev0, c->stream));
     if (direct)
-        launch_batch_query(b->ds->view, b->view, b->shape, c->stream);
+        batch_launch(b, c->stream);
     else
         RK_HIP(c, hipGraphLaunch(b->exec, c->stream));
     RK_HIP(c, hipGetLastError());
@@ -247,6 +365,12 @@ rk_status rk_batch_run(rk_batch* b, double* device_seconds)
     float ms = 0.f;
     RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (device_seconds) *device_seconds = ms * 1e-3;
+    if (b->dfilter) {
+        const rk_status s = batch_download_views(b);
+        if (s != RK_OK) return s;
+        if (b->cand_overflow)
+            return fail(c, RK_ERR_CAPACITY, "a query's device candidate list exceeds its segment; filter on the host");
+    }
     for (uint32_t i = 0; i < b->nq; ++i) b->overflow |= b->h_qcount.p[i] > b->view.seg_cap;
     if (b->overflow) return fail(c, RK_ERR_CAPACITY, "a query's result exceeds its segment; split the batch");
     return RK_OK;
@@ -354,18 +478,63 @@ rk_status rk_batch_destroy(rk_batch* b)
     return RK_OK;
 }
 
-rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq, int32_t* out_ids, size_t cap,
-                         uint64_t* offsets, rk_timing* t)
+} // extern "C"
+
+namespace {
+
+// Queries without candidate lists and with k <= 64 are filtered on the
+// device (inside the batch's graph) unless RK_FILTER=host; a candidate list
+// beyond the device segment sends the batch back to the host filter.
+bool device_filterable(const rk_query_desc* qs, size_t nq)
+{
+    const char* env = std::getenv("RK_FILTER");
+    if (env && std::strcmp(env, "host") == 0) return false;
+    if (!nq) return false;
+    for (size_t i = 0; i < nq; ++i)
+        if (qs[i].cand || qs[i].nc || qs[i].k == 0 || qs[i].k > kDfMaxK) return false;
+    return true;
+}
+
+rk_status query_batch_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, int32_t* out_ids, size_t cap,
+                           uint64_t* offsets, rk_timing* t, bool allow_device);
+
+} // namespace
+
+extern "C" rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq, int32_t* out_ids, size_t cap,
+                                    uint64_t* offsets, rk_timing* t)
+{
+    return query_batch_impl(ds, qs, nq, out_ids, cap, offsets, t, true);
+}
+
+namespace {
+
+rk_status query_batch_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, int32_t* out_ids, size_t cap,
+                           uint64_t* offsets, rk_timing* t, bool allow_device)
 {
     if (!ds || !offsets || (nq && !qs)) return RK_ERR_INVALID_ARG;
     const double t0 = now_s();
     if (t) *t = rk_timing{};
     rk_batch* b = nullptr;
-    rk_status s = rk_batch_prepare(ds, qs, nq, &b);
+    const bool on_device = allow_device && device_filterable(qs, nq);
+    rk_status s;
+    if (on_device) {
+        std::vector<uint32_t> hq(nq), hk(nq);
+        for (size_t i = 0; i < nq; ++i) {
+            hq[i] = qs[i].q;
+            hk[i] = qs[i].k;
+        }
+        s = rk_batch_prepare_device(ds, hq.data(), hk.data(), nq, &b);
+    } else {
+        s = rk_batch_prepare(ds, qs, nq, &b);
+    }
     if (s != RK_OK) return s;
     const double t1 = now_s();
     double dev = 0.0;
     s = rk_batch_run(b, &dev);
+    if (s == RK_ERR_CAPACITY && b->cand_overflow) { // a list beyond the device segment: filter on the host
+        rk_batch_destroy(b);
+        return query_batch_impl(ds, qs, nq, out_ids, cap, offsets, t, false);
+    }
     if (s == RK_ERR_CAPACITY && nq > 1) {
         // over the item/result budget: split in halves (results are per query)
         rk_batch_destroy(b);
@@ -410,4 +579,5 @@ rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq, int
     return s;
 }
 
-} // extern "C"
+} // namespace
+

@@ -10,6 +10,7 @@ namespace rk {
 
 constexpr int kDfSectors = 64;     // angular sectors around q (one per lane)
 constexpr uint32_t kDfMaxK = 64;   // k the device filter takes (sector heaps in LDS)
+constexpr uint32_t kDfBatchCand = 4096; // candidate slots per query of a device-filtered batch
 
 // Facility grid of a dataset: a uniform grid over the facilities' bounding
 // box, about two facilities per cell, cells in row-major order.

@@ -154,6 +154,12 @@ struct rk_batch {
     int shape = 0;               // traversal block shape (rk::kShape*), set by the autotuner
     bool overflow = false;       // the last run had a query beyond its result segment
     double filter_s = 0.0;
+    // device-filtered batch: (q, k) on the device, the filter inside the step
+    bool dfilter = false;
+    uint32_t cand_cap = 0;       // candidate slots per query (the filter's segment)
+    bool cand_overflow = false;  // the last run had a candidate list beyond cand_cap
+    rk::DevBuf<uint32_t> dq, dk, dn;
+    std::vector<uint32_t> h_dn;
     ~rk_batch()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -170,6 +176,11 @@ rk_status clean_candidates(rk_dataset* ds, uint32_t q, const uint32_t* cand, siz
 // (re)fills a batch with queries; device buffers only grow, so a batch
 // refilled step after step allocates nothing once warm
 rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq);
+// (re)fills a batch with (q, k) pairs for the device filter (k <= kDfMaxK);
+// the per-query views are built on the device by each run
+rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, size_t nq);
+// the device work of one step of a batch (filter stages when device-filtered)
+void batch_launch(rk_batch* b, hipStream_t st);
 // picks the traversal block shape for a filled batch (autotuner); caller
 // holds the context lock
 rk_status batch_tune(rk_batch* b);

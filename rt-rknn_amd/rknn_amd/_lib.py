@@ -81,6 +81,7 @@ SIGNATURES = {
     "rk_count_all": (ST, [P, C.c_uint32, U32P, C.c_size_t, U32P, TP]),
     "rk_query_batch": (ST, [P, C.POINTER(rk_query_desc), C.c_size_t, I32P, C.c_size_t, U64P, TP]),
     "rk_batch_prepare": (ST, [P, C.POINTER(rk_query_desc), C.c_size_t, C.POINTER(P)]),
+    "rk_batch_prepare_device": (ST, [P, U32P, U32P, C.c_size_t, C.POINTER(P)]),
     "rk_batch_run": (ST, [P, C.POINTER(C.c_double)]),
     "rk_batch_fetch": (ST, [P, I32P, C.c_size_t, U64P]),
     "rk_batch_destroy": (ST, [P]),

@@ -180,20 +180,28 @@ class Dataset:
         res = [out[int(offs[i]): int(offs[i + 1])].copy() for i in range(nq)]
         return (res, t) if timing else res
 
-    def prepare_batch(self, queries):
-        return Batch(self, list(queries))
+    def prepare_batch(self, queries, device_filter=False):
+        """queries: (q, k) or (q, k, cand); with device_filter=True only
+        (q, k) pairs (k <= 64), filtered on the device inside each run."""
+        return Batch(self, list(queries), device_filter)
 
 
 class Batch:
-    """Prepared graph-captured batch (rk_batch_prepare / run / fetch)."""
+    """Prepared graph-captured batch (rk_batch_prepare[_device] / run / fetch)."""
 
-    def __init__(self, ds, queries):
+    def __init__(self, ds, queries, device_filter=False):
         self.ds = ds
         self._lib = ds._lib
         self.nq = len(queries)
-        self._descs, self._keep = Dataset._descs(queries)
         h = C.c_void_p()
-        ds.ctx.check(self._lib.rk_batch_prepare(ds.h, self._descs, self.nq, C.byref(h)))
+        if device_filter:
+            self._q = _u32([int(qd[0]) for qd in queries])
+            self._k = _u32([int(qd[1]) for qd in queries])
+            ds.ctx.check(self._lib.rk_batch_prepare_device(ds.h, _ptr(self._q, C.c_uint32), _ptr(self._k, C.c_uint32),
+                                                           self.nq, C.byref(h)))
+        else:
+            self._descs, self._keep = Dataset._descs(queries)
+            ds.ctx.check(self._lib.rk_batch_prepare(ds.h, self._descs, self.nq, C.byref(h)))
         self.h = h
 
     def run(self):

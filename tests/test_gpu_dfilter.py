@@ -115,3 +115,49 @@ def test_device_filter_arguments(rk, ctx):
         assert len(ds.query_bruteforce(3, 64)) == nu
     finally:
         ds.close()
+
+
+@pytest.mark.parametrize("name", ["config1.npz", "ties.npz"])
+def test_device_batch_golden(rk, ctx, name, monkeypatch):
+    """Device-filtered batches (filter, traversal and sort in one graph),
+    replayed, and the one-call form with and without RK_FILTER=host: every
+    result equals the golden fixtures."""
+    d, qk, res = golden(name)
+    ds = rk.Dataset(ctx, d["ux"], d["uy"], d["uid"], d["fx"], d["fy"], d["fid"])
+    try:
+        keep = [(i, q, k) for i, (q, k) in enumerate(qk) if k <= 64]
+        b = ds.prepare_batch([(q, k) for _, q, k in keep], device_filter=True)
+        for _ in range(3):
+            b.run()
+            for (i, q, k), g in zip(keep, b.fetch()):
+                assert np.array_equal(g, res[i]), (name, q, k)
+        b.close()
+        for filt in ("device", "host"):
+            monkeypatch.setenv("RK_FILTER", filt)
+            got = ds.query_batch(qk)
+            for g, exp in zip(got, res):
+                assert np.array_equal(g, exp)
+    finally:
+        ds.close()
+
+
+def test_device_batch_mixed_k_matches_host(rk, ctx):
+    """Clustered data with exact ties, 256 queries with k from 1 to 64 in one
+    device-filtered batch: each result equals the host-filtered batch's."""
+    seed = rk.default_seed(4)
+    nu, nf = 300_000, 20_000
+    ux, uy, uid, fx, fy, fid = rk.generate(4, nu, nf, seed=seed, nq_ties=256, seed_ties=seed)
+    qs = [int(q) for q in rk.generate_queries(nf, 256, seed)]
+    r = np.random.default_rng(3)
+    ks = [int(x) for x in r.integers(1, 65, len(qs))]
+    ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    try:
+        b = ds.prepare_batch(list(zip(qs, ks)), device_filter=True)
+        b.run()
+        got = b.fetch()
+        b.close()
+        host = ds.query_batch([(q, k, ds.filter(q, k)) for q, k in zip(qs, ks)])
+        for q, k, g, h in zip(qs, ks, got, host):
+            assert np.array_equal(g, h), (q, k)
+    finally:
+        ds.close()
