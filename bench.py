@@ -14,7 +14,9 @@ over exactly --steps timed steps.  The JSON line also carries:
                 cpu_bruteforce: rknn-naive semantics on two queries)
   plain_*       the one-launch-per-query form on the same queries
   mixed_k       the same step with k drawn uniformly from 1..50 per query
-  end_to_end    filter (host cores) + batched step + result download
+  device_filtered_step  the same step with the influence-zone filter inside it
+                (queries in, results in HBM)
+  end_to_end    one call: filter (device, or host cores) + step + download
   serve         the serving loop under Poisson arrivals (rknn-serve-bench)
 """
 import argparse
@@ -60,7 +62,7 @@ def main():
     ap.add_argument("--cpu-queries", type=int, default=2, help="queries of the brute-force CPU context run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-qps", default="50000,100000,200000")
+    ap.add_argument("--serve-qps", default="100000,300000,500000,1000000")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
     ap.add_argument("--table", action="store_true",
                     help="instead of the headline: one JSON line per config 1-3 (plain and batched forms, the CPU "
@@ -139,14 +141,48 @@ def main():
     log(f"mixed k 1..50: {mixed['ms_per_step']:.3f} ms/step (device {mixed['device_ms']:.3f} ms), "
         f"{mixed['qps']:,.0f} queries/s, |C| mean {mixed['candidates_mean']}")
 
-    # ---- end to end: host filter on every core + batched step + download
-    t0 = time.perf_counter()
-    e2e_res = ds.query_batch([(q, args.k) for q in qs])
-    e2e_s = time.perf_counter() - t0
-    e2e_ok = all(np.array_equal(a, b) for a, b in zip(e2e_res, batched))
-    end_to_end = {"qps": round(args.queries / e2e_s, 1), "ms": round(e2e_s * 1e3, 3), "matches": bool(e2e_ok)}
-    log(f"end to end (filter + step + download): {args.queries / e2e_s:,.0f} queries/s; same sets: {e2e_ok}")
-    if not e2e_ok:
+    # ---- device-filtered step: (q, k) in HBM, the influence-zone filter inside
+    # the step's graph (filter, per-query setup, traversal, sort); same queries
+    dfilter = {}
+    for name, kk, ref in (("k10", [args.k] * args.queries, batched), ("mixed_k", [int(k) for k in ks], mixed_res)):
+        db = ds.prepare_batch(list(zip(qs, kk)), device_filter=True)
+        for _ in range(args.warmup):
+            db.run()
+        ddev = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ddev.append(db.run())
+        dwall = time.perf_counter() - t0
+        dres = db.fetch()
+        db.close()
+        same = all(np.array_equal(a, b) for a, b in zip(dres, ref))
+        dfilter[name] = {"qps": round(args.queries * args.steps / dwall, 1),
+                         "ms_per_step": round(dwall / args.steps * 1e3, 4),
+                         "device_ms": round(float(np.mean(ddev)) * 1e3, 4), "same_sets": bool(same)}
+        log(f"device-filtered step ({name}): {dwall / args.steps * 1e3:.3f} ms/step (device "
+            f"{np.mean(ddev) * 1e3:.3f} ms), {args.queries * args.steps / dwall:,.0f} queries/s; same sets: {same}")
+        if not same:
+            log("ERROR: device-filtered and host-filtered batched result sets differ")
+            sys.exit(1)
+
+    # ---- end to end, one call (rk_query_batch): filter + step + download;
+    # the filter on the device (default for k <= 64) and on the host's cores
+    end_to_end = {}
+    for filt in ("device", "host"):
+        os.environ["RK_FILTER"] = filt
+        ds.query_batch([(q, args.k) for q in qs[:8]])  # warm the path's buffers
+        t0 = time.perf_counter()
+        e2e_res = ds.query_batch([(q, args.k) for q in qs])
+        e2e_s = time.perf_counter() - t0
+        e2e_ok = all(np.array_equal(a, b) for a, b in zip(e2e_res, batched))
+        end_to_end[f"{filt}_filter"] = {"qps": round(args.queries / e2e_s, 1), "ms": round(e2e_s * 1e3, 3),
+                                        "matches": bool(e2e_ok)}
+        log(f"end to end, {filt} filter (filter + step + download): {args.queries / e2e_s:,.0f} queries/s; "
+            f"same sets: {e2e_ok}")
+        if not e2e_ok:
+            break
+    del os.environ["RK_FILTER"]
+    if not all(v["matches"] for v in end_to_end.values()):
         log("ERROR: end-to-end results differ from the batched step")
         sys.exit(1)
 
@@ -297,6 +333,7 @@ def main():
         "cpu_baseline_mt": cpu_mt,
         "cpu_bruteforce": cpu_bf,
         "mixed_k": mixed,
+        "device_filtered_step": dfilter,
         "end_to_end": end_to_end,
         "serve": serve,
     }

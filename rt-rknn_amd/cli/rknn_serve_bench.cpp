@@ -103,6 +103,18 @@ int main(int argc, char** argv)
         rk_serve_config cfg{max_batch, 0, wait};
         rk_server* srv = nullptr;
         if ((st = rk_serve_start(ctx, &cfg, &srv)) != RK_OK) return die("rk_serve_start", ctx, st);
+        {   // untimed warm-up: one full step's worth of requests grows the server's buffers
+            std::vector<uint64_t> wt(std::min<uint32_t>(max_batch, (uint32_t)pool));
+            for (size_t i = 0; i < wt.size(); ++i)
+                if (rk_serve_submit(srv, ds, qs[i], fixed_k ? (uint32_t)kmax : 1u + (uint32_t)(i % kmax), &wt[i]) !=
+                    RK_OK)
+                    wt[i] = 0;
+            for (uint64_t t : wt) {
+                size_t n = 0;
+                while (t && rk_serve_poll(srv, t, out.data(), out.size(), &n, nullptr) == RK_ERR_NOT_READY)
+                    std::this_thread::yield();
+            }
+        }
         std::mt19937_64 rng(seed ^ (uint64_t)rate);
         std::exponential_distribution<double> gap(rate);
         std::uniform_int_distribution<int> pick(0, pool - 1), kd(1, kmax);

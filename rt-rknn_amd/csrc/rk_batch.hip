@@ -407,15 +407,17 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
     }
     offsets[nq] = total;
     if (total > cap) return fail(c, RK_ERR_CAPACITY, "output buffer too small; offsets[nq] holds the size needed");
-    std::vector<int32_t> dev;
+    const int32_t* dev = nullptr;
     if (dev_ran && qoff[nq]) {
         RK_HIP(c, b->out.reserve(qoff[nq]));
         b->view.out = b->out.p; // compaction is outside the graph, so the pointer may change
         launch_batch_compact(b->view, c->stream);
         RK_HIP(c, hipGetLastError());
-        dev.resize(qoff[nq]);
-        RK_HIP(c, hipMemcpyAsync(dev.data(), b->out.p, qoff[nq] * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        RK_HIP(c, b->h_out.reserve(qoff[nq])); // pinned: a direct DMA, no zero-filled staging vector
+        RK_HIP(c, hipMemcpyAsync(b->h_out.p, b->out.p, qoff[nq] * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 c->stream));
         RK_HIP(c, hipStreamSynchronize(c->stream));
+        dev = b->h_out.p;
     }
     for (uint32_t i = 0; i < nq; ++i) {
         int32_t* dst = out_ids + offsets[i];
@@ -423,7 +425,7 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
         if (b->hq[i].trivial) {
             if (n) std::memcpy(dst, ds->all_ids_sorted.data(), n * sizeof(int32_t));
         } else if (n) {
-            std::memcpy(dst, dev.data() + qoff[i], n * sizeof(int32_t));
+            std::memcpy(dst, dev + qoff[i], n * sizeof(int32_t));
             if (n > kDeviceSortMax) std::sort(dst, dst + n); // larger than the device block sort
         }
     }

@@ -146,6 +146,7 @@ struct rk_batch {
     rk::DevBuf<uint64_t> qoff;
     rk::DevBuf<int32_t> out;
     rk::HostBuf<uint32_t> h_qcount;
+    rk::HostBuf<int32_t> h_out;  // pinned staging of the compacted results (fetch)
     rk::BatchDev view{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

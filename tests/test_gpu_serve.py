@@ -22,13 +22,18 @@ def datasets(rk, ctx):
         ds.close()
 
 
-def test_alone_equals_in_the_mix(rk, ctx, datasets):
+@pytest.mark.parametrize("filt", ["device", "host"])
+def test_alone_equals_in_the_mix(rk, ctx, datasets, filt, monkeypatch):
+    """k up to 80: with the device filter the requests with k <= 64 are
+    filtered inside the device steps and the others on the host, in separate
+    steps; RK_FILTER=host filters all of them on the host."""
+    monkeypatch.setenv("RK_FILTER", filt)
     r = np.random.default_rng(1)
     reqs = []
     for i in range(300):
         d = int(r.integers(0, len(datasets)))
         q = int(datasets[d][2][r.integers(0, 32)])
-        k = int(r.integers(1, 51))
+        k = int(r.integers(1, 81))
         reqs.append((d, q, k))
     srv = rk.Server(ctx, max_batch=64, max_wait_s=0.002)
     try:
