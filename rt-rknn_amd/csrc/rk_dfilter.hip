@@ -18,10 +18,13 @@
 //    bound.  The test walks the sectors outward from the one holding g = f - q,
 //    where the half-plane's entry distance is smallest and grows on both
 //    sides, so it stops early.
-//  * Each relevant facility joins the candidate list and then updates the
-//    sectors (lane = sector): every sector keeps the k smallest exit distances
-//    (tightened by the oracle's error, stored as fp32 rounded up) in an LDS
-//    max-heap, and once full its bound is the heap's largest.
+//  * The batch's relevant facilities join the candidate list (compacted by a
+//    ballot into LDS) and then update the sectors (lane = sector, each lane
+//    walking the batch's relevant facilities on its own): every sector keeps
+//    the k smallest exit distances (tightened by the oracle's error, stored
+//    as fp32 rounded up) in an LDS max-heap, and once full its bound is the
+//    heap's largest.  The largest bound over the sectors is refreshed once
+//    per batch (a stale, larger value only stops the walk later).
 //
 // The visiting order differs from the host's nearest-first stream, and the
 // relevance test of a batch uses the bounds from before the batch; neither
@@ -44,14 +47,6 @@ namespace rk {
 namespace {
 
 constexpr double kTwoPi = 6.283185307179586476925286766559;
-
-__device__ __forceinline__ double readlane_d(double v, uint32_t l)
-{
-    const uint64_t b = (uint64_t)__double_as_longlong(v);
-    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
-    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
 
 __device__ __forceinline__ double wave_max_d(double v)
 {
@@ -125,8 +120,11 @@ __device__ bool df_relevant(const double* ex, const double* ey, const double* bo
                             double dl, double R)
 {
     if (dl >= 1.0) return true;
+    // every sector's entry is at least |g|/2 (squared test first: no root)
+    const double shrink = (1.0 - dl) * (1.0 - 1e-12);
+    if (0.25 * gg * shrink * shrink >= R * R * (1.0 + 1e-9)) return false;
     const double glen = sqrt(gg);
-    if (0.5 * glen * (1.0 - dl) * (1.0 - 1e-12) >= R) return false; // every sector's entry is at least |g|/2
+    if (0.5 * glen * shrink >= R) return false;
     const double itol = 1e-9 * glen;
     float ang = atan2f((float)gy, (float)gx);
     if (ang < 0.f) ang += (float)kTwoPi;
@@ -180,6 +178,7 @@ __global__ __launch_bounds__(64) void k_dfilter(FacGridDev g, const uint32_t* __
     __shared__ double s_bound[kDfSectors];
     __shared__ double s_ex[kDfSectors + 1], s_ey[kDfSectors + 1];
     __shared__ uint32_t s_st[64], s_pre[64];
+    __shared__ double4 s_rel[64]; // the batch's relevant facilities: (gx, gy, |g|^2, oracle allowance)
     const uint32_t lane = threadIdx.x;
     s_ex[lane] = g.ex[lane];
     s_ey[lane] = g.ey[lane];
@@ -292,62 +291,63 @@ __global__ __launch_bounds__(64) void k_dfilter(FacGridDev g, const uint32_t* __
                             rel = df_relevant(s_ex, s_ey, s_bound, gx, gy, gg, dl, R);
                         }
                     }
-                    uint64_t rm = __builtin_amdgcn_ballot_w64(rel);
-                    while (rm) {
-                        const uint32_t l = (uint32_t)__ffsll((unsigned long long)rm) - 1;
-                        rm &= rm - 1;
-                        const double fgx = readlane_d(gx, l), fgy = readlane_d(gy, l);
-                        const double fgg = readlane_d(gg, l), fdl = readlane_d(dl, l);
-                        const uint32_t ff = __builtin_amdgcn_readlane(f, l);
-                        if (lane == 0 && n < cap) out[n] = ff;
-                        ++n;
-                        // lane = sector: the facility's exit distance from this sector
-                        const double ca = ax * fgx + ay * fgy, cb = bxs * fgx + bys * fgy;
-                        bool changed = false;
-                        if (ca > 0.0 && cb > 0.0) {
-                            const double rmax = fmax(fgg / (2.0 * ca), fgg / (2.0 * cb)) * (1.0 + fdl) * (1.0 + 1e-12);
-                            const float v = round_up_f(rmax);
-                            bool upd = false;
-                            if (hs < k) { // push, sift up
-                                uint32_t i = hs++;
-                                while (i > 0) {
-                                    const uint32_t p = (i - 1) >> 1;
-                                    const float pv = s_heap[p][lane];
-                                    if (pv >= v) break;
-                                    s_heap[i][lane] = pv;
-                                    i = p;
-                                }
-                                s_heap[i][lane] = v;
-                                upd = true;
-                            } else if (v < s_heap[0][lane]) { // replace the largest, sift down
-                                uint32_t i = 0;
-                                while (true) {
-                                    const uint32_t c1 = 2 * i + 1, c2 = c1 + 1;
-                                    if (c1 >= hs) break;
-                                    uint32_t c = c1;
-                                    float cv = s_heap[c1][lane];
-                                    if (c2 < hs && s_heap[c2][lane] > cv) {
-                                        c = c2;
-                                        cv = s_heap[c2][lane];
-                                    }
-                                    if (cv <= v) break;
-                                    s_heap[i][lane] = cv;
-                                    i = c;
-                                }
-                                s_heap[i][lane] = v;
-                                upd = true;
-                            }
-                            if (upd && hs == k) {
-                                const double b = fmin(capv, (double)s_heap[0][lane]);
-                                if (b < bound) {
-                                    bound = b;
-                                    s_bound[lane] = b;
-                                    changed = true;
-                                }
-                            }
-                        }
-                        if (__builtin_amdgcn_ballot_w64(changed)) R = wave_max_d(bound);
+                    // candidates, in lane order, and their parameters for the sector lanes
+                    const uint64_t rm = __builtin_amdgcn_ballot_w64(rel);
+                    const uint32_t nrel = (uint32_t)__popcll(rm);
+                    if (rel) {
+                        const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(rm >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)rm, 0u));
+                        s_rel[rk] = make_double4(gx, gy, gg, dl);
+                        if (n + rk < cap) out[n + rk] = f;
                     }
+                    n += nrel;
+                    __syncthreads();
+                    // lane = sector: every relevant facility's exit distance from this sector
+                    bool upd = false;
+                    for (uint32_t i = 0; i < nrel; ++i) {
+                        const double4 e = s_rel[i];
+                        const double ca = ax * e.x + ay * e.y, cb = bxs * e.x + bys * e.y;
+                        if (!(ca > 0.0 && cb > 0.0)) continue;
+                        const double rmax = e.z * (1.0 + e.w) * (1.0 + 1e-12) / (2.0 * fmin(ca, cb));
+                        const float v = round_up_f(rmax);
+                        if (hs < k) { // push, sift up
+                            uint32_t j = hs++;
+                            while (j > 0) {
+                                const uint32_t p = (j - 1) >> 1;
+                                const float pv = s_heap[p][lane];
+                                if (pv >= v) break;
+                                s_heap[j][lane] = pv;
+                                j = p;
+                            }
+                            s_heap[j][lane] = v;
+                            upd = true;
+                        } else if (v < s_heap[0][lane]) { // replace the largest, sift down
+                            uint32_t j = 0;
+                            while (true) {
+                                const uint32_t c1 = 2 * j + 1, c2 = c1 + 1;
+                                if (c1 >= hs) break;
+                                uint32_t c = c1;
+                                float cv = s_heap[c1][lane];
+                                if (c2 < hs && s_heap[c2][lane] > cv) {
+                                    c = c2;
+                                    cv = s_heap[c2][lane];
+                                }
+                                if (cv <= v) break;
+                                s_heap[j][lane] = cv;
+                                j = c;
+                            }
+                            s_heap[j][lane] = v;
+                            upd = true;
+                        }
+                    }
+                    if (upd && hs == k) {
+                        const double bnd = fmin(capv, (double)s_heap[0][lane]);
+                        if (bnd < bound) {
+                            bound = bnd;
+                            s_bound[lane] = bnd;
+                        }
+                    }
+                    if (nrel) R = wave_max_d(bound);
                     __syncthreads(); // s_bound is read by the next batch's relevance tests
                 }
                 __syncthreads(); // s_st / s_pre are rewritten for the next chunk
