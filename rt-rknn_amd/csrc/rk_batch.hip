@@ -27,6 +27,23 @@ namespace {
 
 constexpr uint64_t kPoolBudget = 1ull << 26; // result ids per batch (all segments)
 
+// A positive integer from the environment, or dflt.
+uint64_t env_count(const char* name, uint64_t dflt)
+{
+    const char* v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    char* end = nullptr;
+    const unsigned long long x = std::strtoull(v, &end, 10);
+    return (end && *end == 0 && x > 0) ? (uint64_t)x : dflt;
+}
+
+// Result ids per batch (RK_BATCH_POOL_IDS) and device candidate slots per
+// query (RK_DFILTER_SLOTS, at most the default): smaller values hold less
+// memory per batch, and the queries that outgrow them run again
+// (batch_collect).
+uint64_t pool_budget() { return env_count("RK_BATCH_POOL_IDS", kPoolBudget); }
+uint64_t dfilter_slots() { return std::min<uint64_t>(env_count("RK_DFILTER_SLOTS", kDfBatchCand), kDfBatchCand); }
+
 rk_status build_graph(rk_batch* b)
 {
     rk_dataset* ds = b->ds;
@@ -115,6 +132,8 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
     if (nq > (1u << 24)) return fail(c, RK_ERR_INVALID_ARG, "too many queries in one batch");
     b->nq = (uint32_t)nq;
     b->ran = false;
+    b->dfilter = false; // a batch object may alternate between the two kinds of fill
+    b->cand_overflow = false;
     b->hq.assign(nq, QDev{});
     std::vector<uint32_t> all;
     std::vector<double2> wb(nq);
@@ -170,7 +189,8 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
     RK_HIP(c, b->qoff.reserve(nq + 1));
     RK_HIP(c, b->h_qcount.reserve(nq));
     // result segment per query: the whole budget split evenly, never more than the users
-    uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? kPoolBudget / nq : 1));
+    uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? pool_budget() / nq : 1));
+    seg_cap = std::max<uint64_t>(seg_cap, std::min<uint64_t>(ds->nu, b->seg_min));
     seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
     RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
     if (nq) RK_HIP(c, hipMemcpyAsync(b->q.p, b->hq.data(), nq * sizeof(QDev), hipMemcpyHostToDevice, st));
@@ -227,7 +247,7 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
     b->hq.assign(nq, QDev{});
     b->h_dn.assign(nq, 0);
     // candidate segment per query: every facility at most, 4096 slots at most
-    b->cand_cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ds->nf, kDfBatchCand));
+    b->cand_cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ds->nf, dfilter_slots()));
     const uint64_t slots = std::max<uint64_t>(1, (uint64_t)nq * b->cand_cap);
 
     std::lock_guard<std::recursive_mutex> lk(c->mu);
@@ -245,7 +265,8 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
     RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
     RK_HIP(c, b->qoff.reserve(nq + 1));
     RK_HIP(c, b->h_qcount.reserve(nq));
-    uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? kPoolBudget / nq : 1));
+    uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? pool_budget() / nq : 1));
+    seg_cap = std::max<uint64_t>(seg_cap, std::min<uint64_t>(ds->nu, b->seg_min));
     seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
     RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
     if (nq) {
@@ -291,6 +312,8 @@ rk_status batch_download_views(rk_batch* b)
     }
     return RK_OK;
 }
+
+rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets, std::vector<uint32_t>* skip);
 
 } // namespace rk
 
@@ -379,13 +402,35 @@ rk_status rk_batch_run(rk_batch* b, double* device_seconds)
 rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets)
 {
     if (!b || !offsets || (cap && !out_ids)) return RK_ERR_INVALID_ARG;
+    return rk::fetch_impl(b, out_ids, cap, offsets, nullptr);
+}
+
+} // extern "C"
+
+namespace rk {
+
+// The results of the last run.  With skip == nullptr every query must have
+// fit its segments; otherwise the queries that did not get empty ranges and
+// are listed in *skip.
+rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets, std::vector<uint32_t>* skip)
+{
     rk_dataset* ds = b->ds;
     rk_ctx* c = ds->ctx;
     if (!b->ran) return fail(c, RK_ERR_INVALID_ARG, "rk_batch_run has not been called");
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
-    if (b->overflow) return fail(c, RK_ERR_CAPACITY, "the last run overflowed a result segment; split the batch");
+    if (b->overflow && !skip)
+        return fail(c, RK_ERR_CAPACITY, "the last run overflowed a result segment; split the batch");
     const uint32_t nq = b->nq;
+    std::vector<uint8_t> out_of_seg(nq, 0);
+    if (skip) {
+        skip->clear();
+        for (uint32_t i = 0; i < nq; ++i) {
+            const bool lost = b->h_qcount.p[i] > b->view.seg_cap || (b->dfilter && b->h_dn[i] > b->cand_cap);
+            out_of_seg[i] = lost ? 1 : 0;
+            if (lost) skip->push_back(i);
+        }
+    }
     std::vector<uint64_t> qoff(nq + 1, 0);
     const bool dev_ran = b->exec || (b->direct && ds->nu && nq);
     if (dev_ran) {
@@ -403,6 +448,7 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
     uint64_t total = 0;
     for (uint32_t i = 0; i < nq; ++i) {
         offsets[i] = total;
+        if (out_of_seg[i]) continue;
         total += b->hq[i].trivial ? (b->hq[i].k > b->hq[i].nc ? ds->nu : 0) : (qoff[i + 1] - qoff[i]);
     }
     offsets[nq] = total;
@@ -425,12 +471,94 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
         if (b->hq[i].trivial) {
             if (n) std::memcpy(dst, ds->all_ids_sorted.data(), n * sizeof(int32_t));
         } else if (n) {
-            std::memcpy(dst, dev + qoff[i], n * sizeof(int32_t));
-            if (n > kDeviceSortMax) std::sort(dst, dst + n); // larger than the device block sort
+            std::memcpy(dst, dev + qoff[i], n * sizeof(int32_t)); // sorted on the device
         }
     }
     return RK_OK;
 }
+
+rk_status batch_fetch_partial(rk_batch* b, std::vector<int32_t>& ids, std::vector<uint64_t>& off,
+                              std::vector<uint32_t>& redo)
+{
+    off.assign(b->nq + 1, 0);
+    ids.resize(std::max<size_t>(ids.size(), 1));
+    while (true) {
+        const rk_status s = fetch_impl(b, ids.data(), ids.size(), off.data(), &redo);
+        if (s != RK_ERR_CAPACITY || off[b->nq] <= ids.size()) return s;
+        ids.resize(off[b->nq]); // the size needed is known now
+    }
+}
+
+rk_status batch_collect(rk_batch* b, rk_batch* r, const rk_query_desc* qd, size_t nq, std::vector<int32_t>& ids,
+                        std::vector<uint64_t>& off)
+{
+    rk_dataset* ds = b->ds;
+    std::vector<uint32_t> redo;
+    rk_status st = batch_fetch_partial(b, ids, off, redo);
+    if (st != RK_OK || redo.empty()) return st;
+    const bool dfilter = b->dfilter;
+    std::vector<rk_query_desc> rq(redo.size());
+    std::vector<std::vector<uint32_t>> cands(redo.size());
+    uint64_t need = 0;
+    bool host = !dfilter;
+    for (size_t j = 0; j < redo.size(); ++j) {
+        const uint32_t i = redo[j];
+        rq[j] = qd[i];
+        need = std::max<uint64_t>(need, b->h_qcount.p[i]);
+        host |= dfilter && b->h_dn[i] > b->cand_cap;
+    }
+    if (host) // candidate lists from the host filter for every re-run query without one
+        for (size_t j = 0; j < rq.size(); ++j)
+            if (!rq[j].cand) {
+                ds->index->filter(rq[j].q, rq[j].k, cands[j]);
+                rq[j].cand = cands[j].data();
+                rq[j].nc = cands[j].size();
+            }
+    std::vector<int32_t> rids;
+    std::vector<uint64_t> roff;
+    r->direct = true;
+    r->seg_min = need;
+    if (host) {
+        st = batch_fill(r, rq.data(), rq.size());
+    } else {
+        std::vector<uint32_t> q(rq.size()), k(rq.size());
+        for (size_t j = 0; j < rq.size(); ++j) {
+            q[j] = rq[j].q;
+            k[j] = rq[j].k;
+        }
+        st = batch_fill_device(r, q.data(), k.data(), q.size());
+    }
+    if (st == RK_OK) st = rk_batch_run(r, nullptr);
+    if (st == RK_OK || st == RK_ERR_CAPACITY) {
+        std::vector<uint32_t> again;
+        st = batch_fetch_partial(r, rids, roff, again);
+        // the segments hold the measured counts and host lists never
+        // overflow, so nothing is left over
+        if (st == RK_OK && !again.empty())
+            return fail(ds->ctx, RK_ERR_INTERNAL, "a re-run query outgrew segments sized to its count");
+    }
+    if (st != RK_OK) return st;
+    std::vector<int32_t> all;
+    all.reserve(off[nq] + roff[rq.size()]);
+    std::vector<uint64_t> noff(nq + 1, 0);
+    for (size_t i = 0, j = 0; i < nq; ++i) {
+        noff[i] = all.size();
+        if (j < redo.size() && redo[j] == i) {
+            all.insert(all.end(), rids.begin() + roff[j], rids.begin() + roff[j + 1]);
+            ++j;
+        } else {
+            all.insert(all.end(), ids.begin() + off[i], ids.begin() + off[i + 1]);
+        }
+    }
+    noff[nq] = all.size();
+    ids.swap(all);
+    off.swap(noff);
+    return RK_OK;
+}
+
+} // namespace rk
+
+extern "C" {
 
 rk_status rk_batch_stats(const rk_batch* b, uint64_t* items, size_t n_items_cap, size_t* n_levels, uint64_t* results,
                          uint64_t* rechecks)
@@ -485,8 +613,9 @@ rk_status rk_batch_destroy(rk_batch* b)
 namespace {
 
 // Queries without candidate lists and with k <= 64 are filtered on the
-// device (inside the batch's graph) unless RK_FILTER=host; a candidate list
-// beyond the device segment sends the batch back to the host filter.
+// device (inside the batch's graph) unless RK_FILTER=host; a query whose
+// device candidate list outgrew its segment runs again through the host
+// filter (batch_collect).
 bool device_filterable(const rk_query_desc* qs, size_t nq)
 {
     const char* env = std::getenv("RK_FILTER");
@@ -497,27 +626,16 @@ bool device_filterable(const rk_query_desc* qs, size_t nq)
     return true;
 }
 
-rk_status query_batch_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, int32_t* out_ids, size_t cap,
-                           uint64_t* offsets, rk_timing* t, bool allow_device);
-
 } // namespace
 
 extern "C" rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq, int32_t* out_ids, size_t cap,
                                     uint64_t* offsets, rk_timing* t)
 {
-    return query_batch_impl(ds, qs, nq, out_ids, cap, offsets, t, true);
-}
-
-namespace {
-
-rk_status query_batch_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, int32_t* out_ids, size_t cap,
-                           uint64_t* offsets, rk_timing* t, bool allow_device)
-{
     if (!ds || !offsets || (nq && !qs)) return RK_ERR_INVALID_ARG;
     const double t0 = now_s();
     if (t) *t = rk_timing{};
     rk_batch* b = nullptr;
-    const bool on_device = allow_device && device_filterable(qs, nq);
+    const bool on_device = device_filterable(qs, nq);
     rk_status s;
     if (on_device) {
         std::vector<uint32_t> hq(nq), hk(nq);
@@ -533,40 +651,29 @@ rk_status query_batch_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, i
     const double t1 = now_s();
     double dev = 0.0;
     s = rk_batch_run(b, &dev);
-    if (s == RK_ERR_CAPACITY && b->cand_overflow) { // a list beyond the device segment: filter on the host
-        rk_batch_destroy(b);
-        return query_batch_impl(ds, qs, nq, out_ids, cap, offsets, t, false);
-    }
-    if (s == RK_ERR_CAPACITY && nq > 1) {
-        // over the item/result budget: split in halves (results are per query)
-        rk_batch_destroy(b);
-        const size_t h = nq / 2;
-        std::vector<uint64_t> o1(h + 1), o2(nq - h + 1);
-        rk_timing ta{}, tb{};
-        s = rk_query_batch(ds, qs, h, out_ids, cap, o1.data(), &ta);
-        if (s != RK_OK && s != RK_ERR_CAPACITY) return s;
-        const uint64_t used = o1[h];
-        const bool fit1 = s == RK_OK;
-        s = rk_query_batch(ds, qs + h, nq - h, fit1 && out_ids ? out_ids + used : nullptr,
-                           fit1 && cap > used ? cap - used : 0, o2.data(), &tb);
-        for (size_t i = 0; i <= h; ++i) offsets[i] = o1[i];
-        for (size_t i = 1; i <= nq - h; ++i) offsets[h + i] = used + o2[i];
-        if (t) {
-            t->filter_s = ta.filter_s + tb.filter_s;
-            t->kernel_s = ta.kernel_s + tb.kernel_s;
-            t->n_candidates = ta.n_candidates + tb.n_candidates;
-            t->n_fp64_rechecks = ta.n_fp64_rechecks + tb.n_fp64_rechecks;
-            t->total_s = now_s() - t0;
-        }
-        if (!fit1 || s != RK_OK) return fail(ds->ctx, RK_ERR_CAPACITY, "output buffer too small");
-        return RK_OK;
-    }
-    if (s != RK_OK) {
+    if (s != RK_OK && s != RK_ERR_CAPACITY) {
         rk_batch_destroy(b);
         return s;
     }
     const double t2 = now_s();
-    s = rk_batch_fetch(b, out_ids, cap, offsets);
+    if (s == RK_OK) {
+        s = rk_batch_fetch(b, out_ids, cap, offsets);
+    } else {
+        // a result (or device candidate list) beyond its segment: the others
+        // come from this run, those queries run again (batch_collect)
+        std::vector<int32_t> ids;
+        std::vector<uint64_t> off;
+        rk_batch redo;
+        redo.ds = ds;
+        s = batch_collect(b, &redo, qs, nq, ids, off);
+        if (s == RK_OK) {
+            std::copy(off.begin(), off.end(), offsets);
+            if (off[nq] > cap)
+                s = fail(ds->ctx, RK_ERR_CAPACITY, "output buffer too small; offsets[nq] holds the size needed");
+            else if (off[nq])
+                std::memcpy(out_ids, ids.data(), off[nq] * sizeof(int32_t));
+        }
+    }
     if (t) {
         t->filter_s = b->filter_s;
         t->upload_s = t1 - t0 - b->filter_s;
@@ -580,6 +687,3 @@ rk_status query_batch_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, i
     rk_batch_destroy(b);
     return s;
 }
-
-} // namespace
-

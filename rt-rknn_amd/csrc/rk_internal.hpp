@@ -154,6 +154,7 @@ struct rk_batch {
     bool direct = false;         // run launches the kernels without a graph (refilled batches)
     int shape = 0;               // traversal block shape (rk::kShape*), set by the autotuner
     bool overflow = false;       // the last run had a query beyond its result segment
+    uint64_t seg_min = 0;        // result segment per query at least this (capped by N_u), set before a fill
     double filter_s = 0.0;
     // device-filtered batch: (q, k) on the device, the filter inside the step
     bool dfilter = false;
@@ -180,6 +181,20 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq);
 // (re)fills a batch with (q, k) pairs for the device filter (k <= kDfMaxK);
 // the per-query views are built on the device by each run
 rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, size_t nq);
+// after a run: the results of every query whose result (and device
+// candidate list) fit its segment, in query order (ids, off[nq + 1]); the
+// others get empty ranges and their indices in redo (a result is complete
+// or absent, never truncated)
+rk_status batch_fetch_partial(rk_batch* b, std::vector<int32_t>& ids, std::vector<uint64_t>& off,
+                              std::vector<uint32_t>& redo);
+// the results of a batch that ran over qd[0 .. nq) (rk_batch_run returned
+// RK_OK or RK_ERR_CAPACITY), in query order: queries that fit their segments
+// come from b; the others are run again in redo (a direct batch, refilled)
+// and merged in: a result beyond its segment with segments that hold the
+// exact counts b measured, a device candidate list beyond its segment
+// through the host filter
+rk_status batch_collect(rk_batch* b, rk_batch* redo, const rk_query_desc* qd, size_t nq, std::vector<int32_t>& ids,
+                        std::vector<uint64_t>& off);
 // the device work of one step of a batch (filter stages when device-filtered)
 void batch_launch(rk_batch* b, hipStream_t st);
 // picks the traversal block shape for a filled batch (autotuner); caller

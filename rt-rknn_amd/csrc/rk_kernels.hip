@@ -1086,7 +1086,7 @@ void k_batch_query(DsDev ds, BatchDev b)
 
 // Segments the traversal kernel left unsorted (larger than kQuerySortMax,
 // or bunched ids): bucket sort with up to kDeviceSortMax keys, else the
-// radix sort.  Larger segments are sorted by the host when fetched.
+// radix sort.  Larger segments are sorted when fetched (k_batch_compact).
 constexpr int kSortBlock = 512;
 constexpr uint32_t kSortBuckets = 4096;
 static_assert(kDeviceSortMax == kSortBlock * 16, "device sort capacity");
@@ -1139,14 +1139,104 @@ __global__ __launch_bounds__(1024) void k_batch_scan(BatchDev b)
     if (t == 1023) b.qoff[b.nq] = s_part[1023];
 }
 
-// Fetch-time: copy each query's segment to its place in the compacted array.
-__global__ void k_batch_compact(BatchDev b)
+// Fetch-time: each query's segment to its place in the compacted array.
+// Segments larger than kDeviceSortMax (left unsorted by the traversal and
+// k_batch_sort) are sorted on the way by a stable LSD radix sort: 8-bit
+// digits over the bits below the largest id, passes alternating between the
+// segment and its destination so that the last one lands in the
+// destination.  Each wave owns a contiguous eighth of the keys: it counts
+// its digits, and the block turns the counts into a start per (wave, digit)
+// (digit order first, wave order within a digit).  Then each wave scatters
+// its keys 64 at a time in order, a key's place among the equal digits of
+// its chunk coming from eight ballots.  The cost does not depend on how the
+// ids are spread (a bucket sort's does).  The segment itself is scratch for
+// half the passes; its contents stay a permutation of the result set, so a
+// second fetch gives the same output.
+constexpr int kCompactBlock = 512;
+constexpr uint32_t kCompactWaves = kCompactBlock / 64;
+constexpr uint32_t kRadixDigits = 256;
+
+__device__ void radix_pass(const int32_t* __restrict__ src, int32_t* __restrict__ dst, uint32_t n, uint32_t per,
+                           uint32_t shift, uint32_t flip, uint32_t (*hist)[kRadixDigits], uint32_t* red)
 {
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+    const uint32_t lo = min(n, wave * per), hi = min(n, lo + per);
+    for (uint32_t d = lane; d < kRadixDigits; d += 64) hist[wave][d] = 0; // the wave's own row
+    for (uint32_t j = lo + lane; j < hi; j += 64) atomicAdd(&hist[wave][(((uint32_t)src[j] ^ flip) >> shift) & 255u], 1u);
+    __syncthreads();
+    // thread d < 256 (waves 0-3): digit d's total, its start (a scan over
+    // digits) and each wave's start within the digit (a scan over waves)
+    uint32_t tot = 0, inc = 0;
+    if (tid < kRadixDigits) {
+        for (uint32_t w = 0; w < kCompactWaves; ++w) {
+            const uint32_t c = hist[w][tid];
+            hist[w][tid] = tot;
+            tot += c;
+        }
+        inc = tot;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = (uint32_t)__shfl_up((int)inc, o);
+            if ((int)lane >= o) inc += v;
+        }
+        if (lane == 63) red[wave] = inc;
+    }
+    __syncthreads();
+    if (tid < kRadixDigits) {
+        uint32_t base = inc - tot;
+        for (uint32_t w = 0; w < wave; ++w) base += red[w];
+        for (uint32_t w = 0; w < kCompactWaves; ++w) hist[w][tid] += base;
+    }
+    __syncthreads();
+    for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
+        const uint32_t j = c0 + lane;
+        const bool valid = j < hi;
+        const int32_t key = valid ? src[j] : 0;
+        const uint32_t d = (((uint32_t)key ^ flip) >> shift) & 255u;
+        uint64_t peers = ballot(valid); // lanes of this chunk with the same digit
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const uint64_t m = ballot((d >> bit) & 1u);
+            peers &= ((d >> bit) & 1u) ? m : ~m;
+        }
+        const uint32_t r = rank_below(peers);
+        if (valid) dst[hist[wave][d] + r] = key;
+        if (valid && (peers >> lane) == 1) hist[wave][d] += r + 1; // the digit's last lane moves its start
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kCompactBlock) void k_batch_compact(BatchDev b)
+{
+    __shared__ uint32_t hist[kCompactWaves][kRadixDigits];
+    __shared__ uint32_t red[kCompactWaves];
     for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
         const uint32_t n = min(b.qcount[qi], b.seg_cap);
-        const int32_t* src = b.seg + (uint64_t)qi * b.seg_cap;
+        int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
         int32_t* dst = b.out + b.qoff[qi];
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+        if (n <= kDeviceSortMax) { // sorted already
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = seg[i];
+            continue;
+        }
+        // ids below 2^id_bits (non-negative), or any int32 (id_bits = 32:
+        // the sign bit flipped so that unsigned digit order is id order)
+        const uint32_t bits = b.id_bits < 32 ? b.id_bits : 32;
+        const uint32_t flip = bits == 32 ? 0x80000000u : 0u;
+        const uint32_t passes = (bits + 7) / 8;
+        const uint32_t per = (n + kCompactWaves - 1) / kCompactWaves;
+        int32_t* from = seg;
+        int32_t* to = dst;
+        if ((passes & 1u) == 0) { // an even pass count ends where it starts: start in the destination
+            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = seg[i];
+            __syncthreads();
+            from = dst;
+            to = seg;
+        }
+        for (uint32_t p = 0; p < passes; ++p) {
+            radix_pass(from, to, n, per, 8 * p, flip, hist, red);
+            int32_t* t = from;
+            from = to;
+            to = t;
+        }
     }
 }
 int grid_for(uint64_t n, int per_block, int max_blocks)
@@ -1260,7 +1350,7 @@ void launch_batch_scan(const BatchDev& b, hipStream_t st)
 
 void launch_batch_compact(const BatchDev& b, hipStream_t st)
 {
-    k_batch_compact<<<b.nq < 65535 ? (b.nq ? b.nq : 1) : 65535, 256, 0, st>>>(b);
+    k_batch_compact<<<b.nq < 65535 ? (b.nq ? b.nq : 1) : 65535, kCompactBlock, 0, st>>>(b);
 }
 
 } // namespace rk

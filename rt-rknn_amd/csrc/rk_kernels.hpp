@@ -14,7 +14,7 @@ constexpr int kBuckets = 64;
 
 // Batched form: one block of up to kQueryBlock threads per query; tree frontiers
 // of kFront nodes per level in LDS; result segments of up to kDeviceSortMax
-// ids are sorted on the device, larger ones by the host when fetched.
+// ids are sorted inside the step, larger ones by the fetch-time compaction.
 constexpr int kQueryBlock = 512;
 constexpr uint32_t kFront = 512;
 constexpr uint32_t kDeviceSortMax = 8192;

@@ -58,7 +58,10 @@ struct rk_server {
     std::condition_variable cv;      // wakes the host stage
     std::condition_variable cv_done; // wakes pollers (unused by the C API, kept for stop)
     std::deque<Request> waiting;     // submitted, not yet admitted
-    std::deque<Request> ready;       // admitted and filtered, waiting for a device step
+    // admitted and filtered, waiting for a device step: one queue per
+    // (dataset, filtered on the device) -- the requests a step may take
+    std::map<std::pair<rk_dataset*, bool>, std::deque<Request>> ready;
+    size_t n_ready = 0;
     std::unordered_map<uint64_t, Done> done;
     uint64_t next_ticket = 1;
     uint64_t reserved = 0;
@@ -68,6 +71,7 @@ struct rk_server {
     std::condition_variable cv_dev;
     std::map<rk_dataset*, std::unique_ptr<rk_batch>> batches;  // device stage only: host-filtered steps
     std::map<rk_dataset*, std::unique_ptr<rk_batch>> dbatches; // device-filtered steps
+    std::map<rk_dataset*, std::unique_ptr<rk_batch>> rbatches; // re-runs of the queries that outgrew a segment
     bool host_filter_only = false;                             // RK_FILTER=host
 };
 
@@ -86,6 +90,23 @@ uint64_t estimate_bytes(const rk_dataset* ds, uint32_t k, bool dev)
     return cand * 28 + results * 16 + 8 * 4096 + 4096;
 }
 
+// caller holds s->mu
+void push_ready(rk_server* s, Request&& r)
+{
+    s->ready[std::make_pair(r.ds, r.dev)].push_back(std::move(r));
+    ++s->n_ready;
+    s->cv_dev.notify_one();
+}
+
+// The queue whose head waited longest (caller holds s->mu, n_ready > 0).
+std::deque<Request>& oldest_ready(rk_server* s)
+{
+    std::deque<Request>* best = nullptr;
+    for (auto& kv : s->ready)
+        if (!kv.second.empty() && (!best || kv.second.front().t_submit < best->front().t_submit)) best = &kv.second;
+    return *best;
+}
+
 void host_stage(rk_server* s)
 {
     std::unique_lock<std::mutex> lk(s->mu);
@@ -101,139 +122,94 @@ void host_stage(rk_server* s)
             r.ds->index->filter(r.q, r.k, r.cand);
             lk.lock();
         }
-        s->ready.push_back(std::move(r));
-        s->cv_dev.notify_one();
+        push_ready(s, std::move(r));
     }
 }
 
-// One device step over qd.  Each dataset keeps one batch object whose
-// device buffers only grow: refilled every step and launched without a
-// graph, it allocates nothing once warm.  A step over its result budget is
-// split by rk_query_batch.
+rk_batch* batch_for(std::map<rk_dataset*, std::unique_ptr<rk_batch>>& m, rk_dataset* ds)
+{
+    std::unique_ptr<rk_batch>& b = m[ds];
+    if (!b) {
+        b.reset(new rk_batch());
+        b->ds = ds;
+        b->direct = true;
+    }
+    return b.get();
+}
+
+// One host-filtered step over qd.  Each dataset keeps one batch object
+// whose device buffers only grow: refilled every step and launched without
+// a graph, it allocates nothing once warm.
 rk_status run_step(rk_server* s, rk_dataset* ds, const std::vector<rk_query_desc>& qd, std::vector<int32_t>& ids,
                    std::vector<uint64_t>& off)
 {
-    std::unique_ptr<rk_batch>& b = s->batches[ds];
-    if (!b) {
-        b.reset(new rk_batch());
-        b->ds = ds;
-        b->direct = true;
-    }
-    rk_status st = rk::batch_fill(b.get(), qd.data(), qd.size());
+    rk_batch* b = batch_for(s->batches, ds);
+    rk_status st = rk::batch_fill(b, qd.data(), qd.size());
     if (st == RK_OK) {
         std::lock_guard<std::recursive_mutex> lk(ds->ctx->mu);
-        st = rk::batch_tune(b.get());
+        st = rk::batch_tune(b);
     }
-    if (st == RK_OK) st = rk_batch_run(b.get(), nullptr);
-    if (st == RK_OK) {
-        uint64_t total = 0; // the result size is known once the step ran
-        for (size_t i = 0; i < qd.size(); ++i) {
-            const rk::QDev& h = b->hq[i];
-            total += h.trivial ? (h.k > h.nc ? ds->nu : 0) : std::min(b->h_qcount.p[i], b->view.seg_cap);
-        }
-        ids.resize(std::max<uint64_t>(1, total));
-        return rk_batch_fetch(b.get(), ids.data(), ids.size(), off.data());
-    }
-    if (st != RK_ERR_CAPACITY) return st;
-    size_t cap = 1 << 20;
-    while (true) {
-        ids.resize(cap);
-        st = rk_query_batch(ds, qd.data(), qd.size(), ids.data(), ids.size(), off.data(), nullptr);
-        if (st == RK_ERR_CAPACITY && off[qd.size()] > cap) {
-            cap = off[qd.size()];
-            continue;
-        }
-        return st;
-    }
+    if (st == RK_OK) st = rk_batch_run(b, nullptr);
+    if (st == RK_OK || st == RK_ERR_CAPACITY)
+        return rk::batch_collect(b, batch_for(s->rbatches, ds), qd.data(), qd.size(), ids, off);
+    return st;
 }
 
-// One device-filtered step over (q, k): the dataset's device batch, refilled
-// in place and launched without a graph (filter, setup, traversal, sort).  A
-// candidate list beyond its device segment sends the step to rk_query_batch,
-// which filters it on the host.
-rk_status run_step_device(rk_server* s, rk_dataset* ds, const std::vector<uint32_t>& q,
-                          const std::vector<uint32_t>& k, std::vector<int32_t>& ids, std::vector<uint64_t>& off)
+// One device-filtered step over qd (q, k; no candidate lists): the
+// dataset's device batch, refilled in place and launched without a graph
+// (filter, setup, traversal, sort).  The traversal shape is the default (8
+// waves per query): timing shapes needs filtered lists, which only the step
+// itself makes.
+rk_status run_step_device(rk_server* s, rk_dataset* ds, const std::vector<rk_query_desc>& qd,
+                          std::vector<int32_t>& ids, std::vector<uint64_t>& off)
 {
-    std::unique_ptr<rk_batch>& b = s->dbatches[ds];
-    if (!b) {
-        b.reset(new rk_batch());
-        b->ds = ds;
-        b->direct = true;
+    rk_batch* b = batch_for(s->dbatches, ds);
+    std::vector<uint32_t> q(qd.size()), k(qd.size());
+    for (size_t i = 0; i < qd.size(); ++i) {
+        q[i] = qd[i].q;
+        k[i] = qd[i].k;
     }
-    // traversal shape: the default (8 waves per query); timing shapes needs
-    // filtered lists, which only the step itself makes
-    rk_status st = rk::batch_fill_device(b.get(), q.data(), k.data(), q.size());
-    if (st == RK_OK) st = rk_batch_run(b.get(), nullptr);
-    if (st == RK_OK) {
-        uint64_t total = 0;
-        for (size_t i = 0; i < q.size(); ++i) {
-            const rk::QDev& h = b->hq[i];
-            total += h.trivial ? (h.k > h.nc ? ds->nu : 0) : std::min(b->h_qcount.p[i], b->view.seg_cap);
-        }
-        ids.resize(std::max<uint64_t>(1, total));
-        return rk_batch_fetch(b.get(), ids.data(), ids.size(), off.data());
-    }
-    if (st != RK_ERR_CAPACITY) return st;
-    std::vector<rk_query_desc> qd(q.size());
-    for (size_t i = 0; i < q.size(); ++i) qd[i] = rk_query_desc{q[i], k[i], nullptr, 0};
-    size_t cap = 1 << 20;
-    while (true) {
-        ids.resize(cap);
-        st = rk_query_batch(ds, qd.data(), qd.size(), ids.data(), ids.size(), off.data(), nullptr);
-        if (st == RK_ERR_CAPACITY && off[qd.size()] > cap) {
-            cap = off[qd.size()];
-            continue;
-        }
-        return st;
-    }
+    rk_status st = rk::batch_fill_device(b, q.data(), k.data(), q.size());
+    if (st == RK_OK) st = rk_batch_run(b, nullptr);
+    if (st == RK_OK || st == RK_ERR_CAPACITY)
+        return rk::batch_collect(b, batch_for(s->rbatches, ds), qd.data(), qd.size(), ids, off);
+    return st;
 }
 
 void dev_stage(rk_server* s)
 {
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
-        s->cv_dev.wait(lk, [&] { return s->stop || !s->ready.empty(); });
-        if (s->stop && s->ready.empty()) return;
+        s->cv_dev.wait(lk, [&] { return s->stop || s->n_ready > 0; });
+        if (s->stop && s->n_ready == 0) return;
         // let batch-mates gather briefly when the step would be small
-        if (s->ready.size() < s->cfg.max_batch && s->cfg.max_wait_s > 0 && !s->stop) {
-            const double deadline = s->ready.front().t_submit + s->cfg.max_wait_s;
-            while (!s->stop && s->ready.size() < s->cfg.max_batch && !s->waiting.empty() && rk::now_s() < deadline) {
+        if (s->n_ready < s->cfg.max_batch && s->cfg.max_wait_s > 0 && !s->stop) {
+            const double deadline = oldest_ready(s).front().t_submit + s->cfg.max_wait_s;
+            while (!s->stop && s->n_ready < s->cfg.max_batch && !s->waiting.empty() && rk::now_s() < deadline) {
                 s->cv_dev.wait_for(lk, std::chrono::microseconds(200));
             }
         }
-        // one step: requests of the dataset (and filter side) at the head of the queue
-        rk_dataset* ds = s->ready.front().ds;
-        const bool dev = s->ready.front().dev;
+        // one step: the queue (dataset, filter side) whose head waited longest
+        std::deque<Request>& qu = oldest_ready(s);
+        rk_dataset* ds = qu.front().ds;
+        const bool dev = qu.front().dev;
         std::vector<Request> step;
-        for (auto it = s->ready.begin(); it != s->ready.end() && step.size() < s->cfg.max_batch;) {
-            if (it->ds == ds && it->dev == dev) {
-                step.push_back(std::move(*it));
-                it = s->ready.erase(it);
-            } else {
-                ++it;
-            }
+        const size_t take = std::min<size_t>(qu.size(), s->cfg.max_batch);
+        step.reserve(take);
+        for (size_t i = 0; i < take; ++i) {
+            step.push_back(std::move(qu.front()));
+            qu.pop_front();
         }
+        s->n_ready -= take;
         lk.unlock();
         std::vector<uint64_t> off(step.size() + 1);
         std::vector<int32_t> ids;
         rk_status st;
-        if (dev) {
-            std::vector<uint32_t> q(step.size()), k(step.size());
-            for (size_t i = 0; i < step.size(); ++i) {
-                q[i] = step[i].q;
-                k[i] = step[i].k;
-            }
-            st = run_step_device(s, ds, q, k, ids, off);
-        } else {
-            std::vector<rk_query_desc> qd(step.size());
-            for (size_t i = 0; i < step.size(); ++i) {
-                qd[i].q = step[i].q;
-                qd[i].k = step[i].k;
-                qd[i].cand = step[i].cand.data();
-                qd[i].nc = step[i].cand.size();
-            }
-            st = run_step(s, ds, qd, ids, off);
-        }
+        std::vector<rk_query_desc> qd(step.size());
+        for (size_t i = 0; i < step.size(); ++i)
+            qd[i] = rk_query_desc{step[i].q, step[i].k, dev ? nullptr : step[i].cand.data(),
+                                  dev ? 0 : step[i].cand.size()};
+        st = dev ? run_step_device(s, ds, qd, ids, off) : run_step(s, ds, qd, ids, off);
         const double t_end = rk::now_s();
         // results are copied out before the lock is taken: submitters and
         // pollers share it
@@ -302,8 +278,7 @@ rk_status rk_serve_submit(rk_server* s, rk_dataset* ds, uint32_t q, uint32_t k, 
     if (r.dev && s->waiting.empty() && s->reserved + r.reserve <= s->cfg.memory_budget) {
         // nothing to filter on the host: admitted here, straight to the device stage
         s->reserved += r.reserve;
-        s->ready.push_back(std::move(r));
-        s->cv_dev.notify_one();
+        push_ready(s, std::move(r));
         return RK_OK;
     }
     s->waiting.push_back(std::move(r));
