@@ -17,7 +17,8 @@ over exactly --steps timed steps.  The JSON line also carries:
   device_filtered_step  the same step with the influence-zone filter inside it
                 (queries in, results in HBM)
   end_to_end    one call: filter (device, or host cores) + step + download
-  serve         the serving loop under Poisson arrivals (rknn-serve-bench)
+  serve         the serving loop under Poisson arrivals (rknn-serve-bench), k=10
+  serve_datasets  the same with 1M/10k, 10M/100k and 50M/200k resident together, k ~ U{1..50}
 """
 import argparse
 import json
@@ -64,6 +65,9 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-qps", default="100000,300000,500000,1000000")
     ap.add_argument("--serve-seconds", type=float, default=0.5)
+    ap.add_argument("--serve-datasets", default="1000000:10000,10000000:100000,50000000:200000",
+                    help="users:facilities of the datasets resident together for the serve_datasets line")
+    ap.add_argument("--serve-datasets-qps", default="100000,200000,300000")
     ap.add_argument("--table", action="store_true",
                     help="instead of the headline: one JSON line per config 1-3 (plain and batched forms, the CPU "
                          "tree restatement and, where built, the reference's own rknn-inf)")
@@ -212,17 +216,20 @@ def main():
     # instructions per clean test)
     variants = {}
     kernel_names = {"valu4": "k_count_all<4,1>", "valu8": "k_count_all<8,1>", "valu4o8": "k_count_all<4,8>"}
-    for kern in kernel_names:
-        os.environ["RK_COUNT_KERNEL"] = kern
-        kt, flops = [], 0.0
-        for i in range(min(args.roofline_queries, args.queries)):
-            _, t = ds.count_all(qs[i], cands[i], timing=True)
-            kt.append(t.kernel_s)
-            flops += 4.0 * args.users * t.n_candidates
-        variants[kern] = (flops / sum(kt) / 1e12, float(np.mean(kt)), flops / len(kt))
+    kts = {kern: [] for kern in kernel_names}
+    flops = {kern: 0.0 for kern in kernel_names}
+    for _ in range(3):  # the variants interleaved, three rounds each
+        for kern in kernel_names:
+            os.environ["RK_COUNT_KERNEL"] = kern
+            for i in range(min(args.roofline_queries, args.queries)):
+                _, t = ds.count_all(qs[i], cands[i], timing=True)
+                kts[kern].append(t.kernel_s)
+                flops[kern] += 4.0 * args.users * t.n_candidates
+    del os.environ["RK_COUNT_KERNEL"]
+    for kern, kt in kts.items():
+        variants[kern] = (flops[kern] / sum(kt) / 1e12, float(np.mean(kt)), flops[kern] / len(kt))
         log(f"count-all kernel ({kern}): {variants[kern][0]:.2f} TFLOP/s over {len(kt)} launches, "
             f"avg {variants[kern][1] * 1e3:.4f} ms")
-    del os.environ["RK_COUNT_KERNEL"]
     best = max(variants, key=lambda v: variants[v][0])
     achieved, avg_s, flops_launch = variants[best]
     roofline = {"bound": "valu", "achieved": round(achieved, 3),
@@ -289,7 +296,7 @@ def main():
         tree.close()
 
     # ---- serving loop under Poisson arrivals (native load generator)
-    serve = None
+    serve = serve_datasets = None
     sb = os.path.join(ROOT, "rt-rknn_amd", "bin", "rknn-serve-bench")
     if not args.no_serve and os.path.exists(sb):
         batch.close()  # free the step's buffers for the server's own
@@ -304,6 +311,18 @@ def main():
             log(f"serve k={args.k}: offered {pt['offered_qps']:,.0f}/s -> {pt['achieved_qps']:,.0f}/s, "
                 f"p50 {pt['p50_ms']:.3f} ms, p99 {pt['p99_ms']:.3f} ms")
         batch = None
+        # several datasets resident at once, requests (dataset, q, k ~ U{1..50})
+        cmd = [sb, "-c", str(args.config), "-k", "50", "-s", str(args.serve_seconds), "--qps",
+               args.serve_datasets_qps, "--datasets", args.serve_datasets]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            log("serve bench (datasets) failed:", r.stderr[-2000:])
+            sys.exit(1)
+        pts = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+        serve_datasets = {"datasets": args.serve_datasets, "k": "U{1..50}", "points": pts}
+        for pt in pts:
+            log(f"serve {args.serve_datasets}: offered {pt['offered_qps']:,.0f}/s -> "
+                f"{pt['achieved_qps']:,.0f}/s, p50 {pt['p50_ms']:.3f} ms, p99 {pt['p99_ms']:.3f} ms")
 
     out = {
         "metric": "RkNN queries/s @10M users,k=10 batched vs plain; p99 latency; bit-exact sets",
@@ -336,6 +355,7 @@ def main():
         "device_filtered_step": dfilter,
         "end_to_end": end_to_end,
         "serve": serve,
+        "serve_datasets": serve_datasets,
     }
     print(json.dumps(out), flush=True)
     if batch is not None:

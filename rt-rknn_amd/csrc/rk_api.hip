@@ -107,7 +107,7 @@ std::string count_key(rk_dataset* ds, size_t nc)
     return device_arch(ds->ctx) + "/count/u" + std::to_string(u) + "/c" + std::to_string(p);
 }
 
-// RK_COUNT_KERNEL=valu4|valu8|mfma forces a variant; otherwise the
+// RK_COUNT_KERNEL=valu4|valu8|valu4o8 forces a variant; otherwise the
 // autotuner's choice for the class, or -1 when the class has not been timed
 // yet.
 int count_variant(rk_dataset* ds, size_t nc)
