@@ -285,7 +285,8 @@ __device__ __forceinline__ bool clean_for(const CandRec& r, float lx, float ly, 
 }
 
 // Banded path for n_slow candidates s[0 .. n_slow) against U tiles of users
-// (lane = user), counting into cnt.  It runs in double-rate FP32 only:
+// (lane = user), counting into acc (whole numbers, exact in fp32 below
+// 2^24).  It runs in double-rate FP32 only:
 //   c = clamp(4 t - 3/2)
 // is exactly 1 for t >= 5/8 (certainly closer), exactly 0 for t <= 3/8
 // (certainly not), and strictly between 0 and 1 inside the band, where
@@ -297,7 +298,7 @@ __device__ __forceinline__ bool clean_for(const CandRec& r, float lx, float ly, 
 template <int U>
 __device__ __forceinline__ void count_slow(const CandRec* s, const float2* so, uint32_t n_slow, const DsDev& ds,
                                            uint32_t t0, uint32_t nt, const float* wx, const float* wy, float qx,
-                                           float qy, uint32_t* cnt, uint32_t& rech)
+                                           float qy, float* acc, uint32_t& rech)
 {
     float sacc[U], bacc[U];
 #pragma unroll
@@ -313,17 +314,20 @@ __device__ __forceinline__ void count_slow(const CandRec* s, const float2* so, u
         }
     }
     const uint32_t lane = lane_id();
+    // one 64-bit base for the group, the tile as an immediate offset (per-tile
+    // 32-bit indices would each hold a 64-bit address through the group)
+    const uint64_t base = (uint64_t)t0 * kTile + lane;
 #pragma unroll
     for (int j = 0; j < U; ++j) {
         const bool band = bacc[j] > 0.f;
         if (ballot(band) == 0) {
-            cnt[j] += (uint32_t)sacc[j];
+            acc[j] += sacc[j];
             continue;
         }
         // rare: this tile met the band; recount its slow candidates exactly
         uint32_t exact = 0;
         if ((uint32_t)j < nt) {
-            const float px = ds.ux[(t0 + j) * kTile + lane], py = ds.uy[(t0 + j) * kTile + lane];
+            const float px = (ds.ux + base)[j * kTile], py = (ds.uy + base)[j * kTile];
             const double dq = d2_exact(px, py, qx, qy);
             for (uint32_t i = 0; i < n_slow; ++i) {
                 const CandRec c = s[i];
@@ -338,7 +342,7 @@ __device__ __forceinline__ void count_slow(const CandRec* s, const float2* so, u
                 }
             }
         }
-        cnt[j] += band ? exact : (uint32_t)sacc[j];
+        acc[j] += band ? (float)exact : sacc[j];
     }
 }
 
@@ -384,16 +388,16 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
     const uint32_t lane = lane_id();
     const uint32_t t0 = g * U;
     const uint32_t nt = min((uint32_t)U, ds.n_tiles - t0);
-    float wx[U], wy[U], acc[U];
-    uint32_t cnt[U];
+    float wx[U], wy[U], acc[U]; // acc: the count so far, a whole number
 #pragma unroll
     for (int j = 0; j < U; ++j) {
         wx[j] = px[j] - q.qx;
         wy[j] = py[j] - q.qy;
         acc[j] = 0.f;
-        cnt[j] = 0;
     }
-    const float lx = m.x - q.qx, ly = m.y - q.qy, hx = m.z - q.qx, hy = m.w - q.qy;
+    // wave-uniform: kept in scalar registers through the chunks
+    const float lx = uniformf(m.x - q.qx), ly = uniformf(m.y - q.qy), hx = uniformf(m.z - q.qx),
+                hy = uniformf(m.w - q.qy);
     for (uint32_t c0 = 0; c0 < nc; c0 += 64) {
         const uint32_t jc = c0 + lane;
         const bool have = jc < nc;
@@ -420,12 +424,13 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
             for (int j = 0; j < U; ++j) // med3(t,0,1) folds into the FMA's clamp bit
                 acc[j] += __builtin_amdgcn_fmed3f(fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh)), 0.0f, 1.0f);
         }
-        if (n_slow) count_slow<U>(wl + 64, wo, n_slow, ds, t0, nt, wx, wy, q.qx, q.qy, cnt, rech);
+        if (n_slow) count_slow<U>(wl + 64, wo, n_slow, ds, t0, nt, wx, wy, q.qx, q.qy, acc, rech);
         __builtin_amdgcn_wave_barrier();
     }
+    OutT* __restrict__ out = counts + ((uint64_t)t0 * kTile + lane);
 #pragma unroll
     for (int j = 0; j < U; ++j)
-        if ((uint32_t)j < nt) counts[(t0 + j) * kTile + lane] = (OutT)((uint32_t)acc[j] + cnt[j]);
+        if ((uint32_t)j < nt) out[j * kTile] = (OutT)(uint32_t)acc[j];
 }
 
 // The first group's users are requested before the records are staged, so
