@@ -634,8 +634,25 @@ extern "C" rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, siz
     if (!ds || !offsets || (nq && !qs)) return RK_ERR_INVALID_ARG;
     const double t0 = now_s();
     if (t) *t = rk_timing{};
-    rk_batch* b = nullptr;
+    // one run: the kernels are launched directly (a graph pays for its
+    // capture only when replayed); host-filtered batches take the
+    // autotuner's shape for their class (timed once per class), device-
+    // filtered ones the default, as their lists exist only inside the run.
+    // The dataset's batch object for the filter side is refilled in place.
     const bool on_device = device_filterable(qs, nq);
+    const int side = on_device ? 0 : 1;
+    std::unique_lock<std::mutex> own(ds->qb_mu[side], std::try_to_lock);
+    std::unique_ptr<rk_batch> mine;
+    rk_batch* b = nullptr;
+    if (own.owns_lock()) {
+        if (!ds->qb[side]) ds->qb[side].reset(new rk_batch());
+        b = ds->qb[side].get();
+    } else {
+        mine.reset(new rk_batch());
+        b = mine.get();
+    }
+    b->ds = ds;
+    b->direct = true;
     rk_status s;
     if (on_device) {
         std::vector<uint32_t> hq(nq), hk(nq);
@@ -643,18 +660,19 @@ extern "C" rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, siz
             hq[i] = qs[i].q;
             hk[i] = qs[i].k;
         }
-        s = rk_batch_prepare_device(ds, hq.data(), hk.data(), nq, &b);
+        s = batch_fill_device(b, hq.data(), hk.data(), nq);
     } else {
-        s = rk_batch_prepare(ds, qs, nq, &b);
+        s = batch_fill(b, qs, nq);
+        if (s == RK_OK) {
+            std::lock_guard<std::recursive_mutex> lk(ds->ctx->mu);
+            s = batch_tune(b);
+        }
     }
     if (s != RK_OK) return s;
     const double t1 = now_s();
     double dev = 0.0;
     s = rk_batch_run(b, &dev);
-    if (s != RK_OK && s != RK_ERR_CAPACITY) {
-        rk_batch_destroy(b);
-        return s;
-    }
+    if (s != RK_OK && s != RK_ERR_CAPACITY) return s;
     const double t2 = now_s();
     if (s == RK_OK) {
         s = rk_batch_fetch(b, out_ids, cap, offsets);
@@ -684,6 +702,5 @@ extern "C" rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, siz
         if (rk_batch_stats(b, nullptr, 0, nullptr, nullptr, &rech) == RK_OK) t->n_fp64_rechecks = rech;
         t->total_s = now_s() - t0;
     }
-    rk_batch_destroy(b);
-    return s;
+    return s; // every copy out of the batch was synchronous
 }

@@ -128,6 +128,12 @@ struct rk_dataset {
     rk::HostBuf<float2> h_orig;
     rk::HostBuf<int32_t> h_out;
     rk::HostBuf<uint32_t> h_ctr;
+    // rk_query_batch's batch objects ([0] device-filtered, [1] host-filtered),
+    // refilled call after call so that their buffers are allocated once; a
+    // call that finds one busy (another thread) uses a batch of its own
+    std::mutex qb_mu[2];
+    std::unique_ptr<rk_batch> qb[2];
+    ~rk_dataset();
 };
 
 struct rk_batch {
@@ -168,6 +174,8 @@ struct rk_batch {
         if (graph) (void)hipGraphDestroy(graph);
     }
 };
+
+inline rk_dataset::~rk_dataset() = default; // rk_batch is complete here
 
 namespace rk {
 rk_status fail(rk_ctx* c, rk_status s, const std::string& msg);

@@ -5,6 +5,7 @@ only moves numpy buffers across the C ABI.  Every call runs the gfx950
 kernels; errors raise RkError carrying the rk_status code.
 """
 import ctypes as C
+import threading
 
 import numpy as np
 
@@ -64,6 +65,11 @@ class Context:
         self.close()
 
 
+# rk_query_desc (include/rknn.h): uint32 q, uint32 k, const uint32* cand, size_t nc
+_DESC = np.dtype([("q", "<u4"), ("k", "<u4"), ("cand", "<u8"), ("nc", "<u8")])
+assert _DESC.itemsize == C.sizeof(L.rk_query_desc)
+
+
 class Dataset:
     """Device-resident users and facilities (rk_dataset_create)."""
 
@@ -80,6 +86,8 @@ class Dataset:
         ctx.check(st)
         self.h = h
         self.nu, self.nf = len(self.ux), len(self.fx)
+        self._batch_out = None  # query_batch's output buffer, reused while not in use elsewhere
+        self._batch_lock = threading.Lock()
 
     def close(self):
         if self.h:
@@ -146,19 +154,22 @@ class Dataset:
 
     @staticmethod
     def _descs(queries):
+        """rk_query_desc[] as a numpy record array (the struct's layout), the
+        (q, k) columns filled at once; keep holds the candidate arrays."""
+        n = len(queries)
+        d = np.zeros(max(1, n), dtype=_DESC)
         keep = []
-        descs = (L.rk_query_desc * max(1, len(queries)))()
-        for i, qd in enumerate(queries):
-            q, k = int(qd[0]), int(qd[1])
-            cand = qd[2] if len(qd) > 2 else None
-            descs[i].q, descs[i].k = q, k
-            if cand is None:
-                descs[i].cand, descs[i].nc = None, 0
-            else:
-                c = _u32(cand)
-                keep.append(c)
-                descs[i].cand, descs[i].nc = _ptr(c, C.c_uint32), len(c)
-        return descs, keep
+        if n:
+            d["q"][:n] = [int(x[0]) for x in queries]
+            d["k"][:n] = [int(x[1]) for x in queries]
+            for i, x in enumerate(queries):
+                if len(x) > 2 and x[2] is not None:
+                    c = _u32(x[2])
+                    keep.append(c)
+                    d["cand"][i] = c.ctypes.data
+                    d["nc"][i] = len(c)
+        keep.append(d)
+        return d.ctypes.data_as(C.POINTER(L.rk_query_desc)), keep
 
     def query_batch(self, queries, timing=False):
         """queries: iterable of (q, k) or (q, k, cand); cand None runs the filter."""
@@ -166,18 +177,29 @@ class Dataset:
         descs, keep = self._descs(queries)
         nq = len(queries)
         offs = np.zeros(nq + 1, dtype=np.uint64)
-        cap = max(1, self.nu)
         t = L.rk_timing()
-        while True:
-            out = np.empty(cap, dtype=np.int32)
-            st = self._lib.rk_query_batch(self.h, descs, nq, _ptr(out, C.c_int32), cap, _ptr(offs, C.c_uint64),
-                                          C.byref(t))
-            if st == L.RK_ERR_CAPACITY and int(offs[nq]) > cap:
-                cap = int(offs[nq])
-                continue
-            self.ctx.check(st)
-            break
-        res = [out[int(offs[i]): int(offs[i + 1])].copy() for i in range(nq)]
+        # the output buffer is kept between calls (its pages stay mapped); a
+        # concurrent call on the same dataset takes a buffer of its own
+        shared = self._batch_lock.acquire(blocking=False)
+        try:
+            out = self._batch_out if shared else None
+            while True:
+                if out is None:
+                    out = np.empty(max(1, self.nu), dtype=np.int32)
+                st = self._lib.rk_query_batch(self.h, descs, nq, _ptr(out, C.c_int32), len(out),
+                                              _ptr(offs, C.c_uint64), C.byref(t))
+                if st == L.RK_ERR_CAPACITY and int(offs[nq]) > len(out):
+                    out = np.empty(int(offs[nq]), dtype=np.int32)
+                    continue
+                self.ctx.check(st)
+                break
+            total = int(offs[nq])
+            res = np.split(out[:total].copy(), offs[1:nq].astype(np.int64)) if nq else []
+            if shared:
+                self._batch_out = out
+        finally:
+            if shared:
+                self._batch_lock.release()
         return (res, t) if timing else res
 
     def prepare_batch(self, queries, device_filter=False):
