@@ -51,7 +51,17 @@ rk_status build_graph(rk_batch* b)
     hipStream_t st = c->stream;
     RK_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     batch_launch(b, st);
-    hipError_t e = hipStreamEndCapture(st, &b->graph);
+    // the per-query result counts come back inside the graph (pinned memory):
+    // a copy enqueued after the graph starts only after an idle gap of
+    // about 14 us on the device
+    hipError_t e = hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        return fail_hip(c, e, "hipMemcpyAsync (capture)", __LINE__);
+    }
+    e = hipStreamEndCapture(st, &b->graph);
     if (e != hipSuccess) return fail_hip(c, e, "hipStreamEndCapture", __LINE__);
     RK_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     return RK_OK;
@@ -362,31 +372,59 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
     return RK_OK;
 }
 
-rk_status rk_batch_run(rk_batch* b, double* device_seconds)
+rk_status rk_batch_enqueue(rk_batch* b)
 {
     if (!b) return RK_ERR_INVALID_ARG;
     rk_ctx* c = b->ds->ctx;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
+    if (b->pending) return fail(c, RK_ERR_INVALID_ARG, "the batch's last run has not been waited for");
     b->ran = true;
     b->overflow = false;
     const bool direct = !b->exec && b->direct && b->ds->nu && b->nq;
-    if (!b->exec && !direct) {
-        if (device_seconds) *device_seconds = 0.0;
-        return RK_OK;
+    b->pending = true;
+    b->pending_empty = !b->exec && !direct;
+    if (b->pending_empty) return RK_OK;
+    if (!b->ev_start) {
+        RK_HIP(c, hipEventCreate(&b->ev_start));
+        RK_HIP(c, hipEventCreate(&b->ev_end));
+        RK_HIP(c, hipEventCreate(&b->ev_done));
     }
-    RK_HIP(c, hipEventRecord(c->ev0, c->stream));
+    RK_HIP(c, hipEventRecord(b->ev_start, c->stream));
     if (direct)
         batch_launch(b, c->stream);
     else
         RK_HIP(c, hipGraphLaunch(b->exec, c->stream));
     RK_HIP(c, hipGetLastError());
-    RK_HIP(c, hipEventRecord(c->ev1, c->stream));
-    RK_HIP(c, hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                             c->stream));
-    RK_HIP(c, hipStreamSynchronize(c->stream));
+    RK_HIP(c, hipEventRecord(b->ev_end, c->stream));
+    if (direct) // a graph copies the counts itself
+        RK_HIP(c, hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+    RK_HIP(c, hipEventRecord(b->ev_done, c->stream));
+    return RK_OK;
+}
+
+rk_status rk_batch_wait(rk_batch* b, double* device_seconds)
+{
+    if (!b) return RK_ERR_INVALID_ARG;
+    rk_ctx* c = b->ds->ctx;
+    if (!b->pending) return fail(c, RK_ERR_INVALID_ARG, "no run of this batch was enqueued");
+    if (device_seconds) *device_seconds = 0.0;
+    if (!b->pending_empty) {
+        // waited for without the context lock: other batches may enqueue meanwhile
+        const hipError_t e = hipEventSynchronize(b->ev_done);
+        if (e != hipSuccess) {
+            std::lock_guard<std::recursive_mutex> lk(c->mu);
+            b->pending = false;
+            return fail_hip(c, e, "hipEventSynchronize", __LINE__);
+        }
+    }
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    b->pending = false;
+    if (b->pending_empty) return RK_OK;
+    RK_HIP(c, hipSetDevice(c->device));
     float ms = 0.f;
-    RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    RK_HIP(c, hipEventElapsedTime(&ms, b->ev_start, b->ev_end));
     if (device_seconds) *device_seconds = ms * 1e-3;
     if (b->dfilter) {
         const rk_status s = batch_download_views(b);
@@ -397,6 +435,13 @@ rk_status rk_batch_run(rk_batch* b, double* device_seconds)
     for (uint32_t i = 0; i < b->nq; ++i) b->overflow |= b->h_qcount.p[i] > b->view.seg_cap;
     if (b->overflow) return fail(c, RK_ERR_CAPACITY, "a query's result exceeds its segment; split the batch");
     return RK_OK;
+}
+
+rk_status rk_batch_run(rk_batch* b, double* device_seconds)
+{
+    const rk_status s = rk_batch_enqueue(b);
+    if (s != RK_OK) return s;
+    return rk_batch_wait(b, device_seconds);
 }
 
 rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets)
@@ -417,6 +462,7 @@ rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offset
     rk_dataset* ds = b->ds;
     rk_ctx* c = ds->ctx;
     if (!b->ran) return fail(c, RK_ERR_INVALID_ARG, "rk_batch_run has not been called");
+    if (b->pending) return fail(c, RK_ERR_INVALID_ARG, "rk_batch_wait has not been called");
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
     if (b->overflow && !skip)
@@ -563,7 +609,7 @@ extern "C" {
 rk_status rk_batch_stats(const rk_batch* b, uint64_t* items, size_t n_items_cap, size_t* n_levels, uint64_t* results,
                          uint64_t* rechecks)
 {
-    if (!b || !b->ran) return RK_ERR_INVALID_ARG;
+    if (!b || !b->ran || b->pending) return RK_ERR_INVALID_ARG;
     rk_ctx* c = b->ds->ctx;
     const size_t nl = b->ds->view.n_levels;
     if (n_levels) *n_levels = nl;

@@ -168,10 +168,17 @@ struct rk_batch {
     bool cand_overflow = false;  // the last run had a candidate list beyond cand_cap
     rk::DevBuf<uint32_t> dq, dk, dn;
     std::vector<uint32_t> h_dn;
+    // rk_batch_enqueue / rk_batch_wait: a run in flight and its events
+    // (start and end of the device work, and done: after the count copy)
+    bool pending = false;
+    bool pending_empty = false; // nothing was launched (no users or no queries)
+    hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_done = nullptr;
     ~rk_batch()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : {ev_start, ev_end, ev_done})
+            if (e) (void)hipEventDestroy(e);
     }
 };
 

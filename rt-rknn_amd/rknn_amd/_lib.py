@@ -83,6 +83,8 @@ SIGNATURES = {
     "rk_batch_prepare": (ST, [P, C.POINTER(rk_query_desc), C.c_size_t, C.POINTER(P)]),
     "rk_batch_prepare_device": (ST, [P, U32P, U32P, C.c_size_t, C.POINTER(P)]),
     "rk_batch_run": (ST, [P, C.POINTER(C.c_double)]),
+    "rk_batch_enqueue": (ST, [P]),
+    "rk_batch_wait": (ST, [P, C.POINTER(C.c_double)]),
     "rk_batch_fetch": (ST, [P, I32P, C.c_size_t, U64P]),
     "rk_batch_destroy": (ST, [P]),
     "rk_batch_stats": (ST, [P, U64P, C.c_size_t, SZP, U64P, U64P]),

@@ -231,6 +231,16 @@ class Batch:
         self.ds.ctx.check(self._lib.rk_batch_run(self.h, C.byref(s)))
         return s.value
 
+    def enqueue(self):
+        """Puts one run on the context's stream and returns (rk_batch_enqueue)."""
+        self.ds.ctx.check(self._lib.rk_batch_enqueue(self.h))
+
+    def wait(self):
+        """Waits for the enqueued run; its device seconds (rk_batch_wait)."""
+        s = C.c_double()
+        self.ds.ctx.check(self._lib.rk_batch_wait(self.h, C.byref(s)))
+        return s.value
+
     def fetch(self):
         offs = np.zeros(self.nq + 1, dtype=np.uint64)
         cap = max(1, self.ds.nu)

@@ -61,6 +61,42 @@ def test_config1_batch_golden(g1):
         assert np.array_equal(g, res[i % len(res)])
 
 
+def test_batch_enqueue_wait_golden(rk, g1, gties):
+    """rk_batch_enqueue / rk_batch_wait: batches of two datasets, graph-
+    captured and device-filtered, several runs in flight on one stream; every
+    fetch equals the golden sets.  Misuse is an error, not a hang."""
+    bs = []
+    for d, ds, qk, res in (g1, gties):
+        keep = [(q, k) for q, k in qk if k <= 64]
+        exp = [r for (q, k), r in zip(qk, res) if k <= 64]
+        bs.append((ds.prepare_batch([(q, k, ds.filter(q, k)) for q, k in keep]), exp))
+        bs.append((ds.prepare_batch(keep, device_filter=True), exp))
+    for rnd in range(3):
+        for b, _ in bs:
+            b.enqueue()
+        for b, _ in bs:
+            assert b.wait() > 0.0
+        for b, exp in bs:
+            for g, e in zip(b.fetch(), exp):
+                assert np.array_equal(g, e), rnd
+    b = bs[0][0]
+    with pytest.raises(rk.RkError) as e:
+        b.wait()  # nothing enqueued
+    assert e.value.status == 2
+    b.enqueue()
+    with pytest.raises(rk.RkError) as e:
+        b.enqueue()  # one run in flight per batch
+    assert e.value.status == 2
+    with pytest.raises(rk.RkError) as e:
+        b.fetch()  # not waited for
+    assert e.value.status == 2
+    b.wait()
+    for g, e in zip(b.fetch(), bs[0][1]):
+        assert np.array_equal(g, e)
+    for b, _ in bs:
+        b.close()
+
+
 def test_ties_golden_every_form(gties):
     d, ds, qk, res = gties
     rechecks = 0
