@@ -44,7 +44,7 @@ PEAK_HBM_GBS = 8000.0
 # from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known
 # size (profiles/k_count_all_10M_100k.md); the kernel reads every user once and
 # writes one 16-bit count per user, so the figure does not depend on the query
-MEASURED_TRAFFIC = {"valu4": 101.4e6, "valu8": 100.8e6, "valu4o8": 101.4e6}
+MEASURED_TRAFFIC = {"valu4": 101.4e6, "valu8": 100.8e6}
 
 
 def log(*a):
@@ -248,10 +248,10 @@ def main():
         sys.exit(1)
 
     # ---- roofline: dense counting kernel (count-all), HIP events on its stream;
-    # every variant (4 or 8 users per lane, or 4 held to 8 waves per SIMD; 3 VALU
+    # every variant (4, 6 or 8 users per lane; 3 VALU
     # instructions per clean test)
     variants = {}
-    kernel_names = {"valu4": "k_count_all<4,1>", "valu8": "k_count_all<8,1>", "valu4o8": "k_count_all<4,8>"}
+    kernel_names = {"valu4": "k_count_all<4,1>", "valu6": "k_count_all<6,1>", "valu8": "k_count_all<8,1>"}
     kts = {kern: [] for kern in kernel_names}
     flops = {kern: 0.0 for kern in kernel_names}
     for _ in range(3):  # the variants interleaved, three rounds each

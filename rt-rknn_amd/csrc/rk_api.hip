@@ -107,7 +107,7 @@ std::string count_key(rk_dataset* ds, size_t nc)
     return device_arch(ds->ctx) + "/count/u" + std::to_string(u) + "/c" + std::to_string(p);
 }
 
-// RK_COUNT_KERNEL=valu4|valu8|valu4o8 forces a variant; otherwise the
+// RK_COUNT_KERNEL=valu4|valu6|valu8 forces a variant; otherwise the
 // autotuner's choice for the class, or -1 when the class has not been timed
 // yet.
 int count_variant(rk_dataset* ds, size_t nc)
@@ -351,8 +351,11 @@ rk_status rk_dataset_create(rk_ctx* c, const float* ux, const float* uy, const i
         ds->view.n_levels = (uint32_t)lvl + 1;
         // joint rectangles of the count-all kernels' 4- and 8-tile groups
         RK_HIP(c, ds->gmbr4.reserve((ds->n_tiles + 3) / 4));
+        RK_HIP(c, ds->gmbr6.reserve((ds->n_tiles + 5) / 6));
         RK_HIP(c, ds->gmbr8.reserve((ds->n_tiles + 7) / 8));
         launch_group_mbr(ds->mbr[0].p, ds->n_tiles, 4, ds->gmbr4.p, st);
+        RK_HIP(c, hipGetLastError());
+        launch_group_mbr(ds->mbr[0].p, ds->n_tiles, 6, ds->gmbr6.p, st);
         RK_HIP(c, hipGetLastError());
         launch_group_mbr(ds->mbr[0].p, ds->n_tiles, 8, ds->gmbr8.p, st);
         RK_HIP(c, hipGetLastError());
@@ -364,6 +367,7 @@ rk_status rk_dataset_create(rk_ctx* c, const float* ux, const float* uy, const i
     ds->view.uid = ds->uid.p;
     for (int l = 0; l < kMaxLevels; ++l) ds->view.mbr[l] = ds->mbr[l].p;
     ds->view.gmbr4 = ds->gmbr4.p;
+    ds->view.gmbr6 = ds->gmbr6.p;
     ds->view.gmbr8 = ds->gmbr8.p;
     ds->view.nu = (uint32_t)nu;
     ds->view.n_tiles = ds->n_tiles;
@@ -392,7 +396,7 @@ rk_status rk_dataset_info(const rk_dataset* ds, uint64_t* nu, uint64_t* nf, uint
         uint64_t b = ds->ux.bytes() + ds->uy.bytes() + ds->uid.bytes() + ds->perm.bytes() + ds->fx.bytes() +
                      ds->fy.bytes() + ds->out.bytes();
         for (int l = 0; l < kMaxLevels; ++l) b += ds->mbr[l].bytes();
-        b += ds->gmbr4.bytes() + ds->gmbr8.bytes();
+        b += ds->gmbr4.bytes() + ds->gmbr6.bytes() + ds->gmbr8.bytes();
         b += ds->g_start.bytes() + ds->g_fid.bytes() + ds->g_fx.bytes() + ds->g_fy.bytes() + ds->g_dir.bytes();
         *device_bytes = b;
     }

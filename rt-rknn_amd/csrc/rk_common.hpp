@@ -115,6 +115,7 @@ struct DsDev {
     const int32_t* uid;
     const float4* mbr[kMaxLevels]; // mbr[0]: tiles; mbr[l]: groups of 64 level-(l-1) nodes
     const float4* gmbr4;           // groups of 4 consecutive tiles (count-all)
+    const float4* gmbr6;           // groups of 6 consecutive tiles (count-all)
     const float4* gmbr8;           // groups of 8 consecutive tiles (count-all)
     uint32_t nlev[kMaxLevels];     // node count per level
     uint32_t n_levels;             // levels present; nlev[n_levels-1] <= 64

@@ -104,7 +104,7 @@ struct rk_dataset {
     rk::DevBuf<int32_t> uid;
     rk::DevBuf<uint32_t> perm;       // sorted position -> caller's user index
     rk::DevBuf<float4> mbr[rk::kMaxLevels];
-    rk::DevBuf<float4> gmbr4, gmbr8; // joint rectangles of 4- and 8-tile groups
+    rk::DevBuf<float4> gmbr4, gmbr6, gmbr8; // joint rectangles of 4-, 6- and 8-tile groups
     rk::DevBuf<float> fx, fy;        // facilities, caller order
     std::vector<float> hfx, hfy;
     std::vector<int32_t> hfid;

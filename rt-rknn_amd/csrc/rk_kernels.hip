@@ -352,11 +352,11 @@ __device__ __forceinline__ void count_slow(const CandRec* s, const float2* so, u
 template <int U>
 __device__ __forceinline__ void load_group(const DsDev& ds, uint32_t g, float* px, float* py, float4& m)
 {
-    static_assert(U == 4 || U == 8, "group rectangles are kept for 4- and 8-tile groups");
+    static_assert(U == 4 || U == 6 || U == 8, "group rectangles are kept for 4-, 6- and 8-tile groups");
     const uint32_t lane = lane_id();
     const uint32_t t0 = g * U;
     const uint32_t nt = uniform(min((uint32_t)U, ds.n_tiles - t0));
-    m = (U == 4 ? ds.gmbr4 : ds.gmbr8)[g];
+    m = (U == 4 ? ds.gmbr4 : U == 6 ? ds.gmbr6 : ds.gmbr8)[g];
     const uint64_t base = (uint64_t)t0 * kTile + lane;
     const float* ux = ds.ux + base;
     const float* uy = ds.uy + base;
@@ -434,10 +434,12 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
 }
 
 // The first group's users are requested before the records are staged, so
-// the two loads overlap.  (Loading a second group ahead, two groups per
-// wave, measured slower: the tail doubles and occupancy drops.)  W is the
-// occupancy the register allocation is held to (W = 8 with U = 4 spills a
-// few registers per group; W = 1 leaves the allocation free).
+// the two loads overlap.  (A persistent grid that requests each wave's next
+// group while counting the current one measured slower at every grid size:
+// the extra registers cost a wave per SIMD and the strided walk leaves a
+// longer tail than the hardware dispatcher.)  W is the occupancy the
+// register allocation may be held to (W = 1 leaves it free: 53, 62 and 92
+// VGPRs for U = 4, 6 and 8).
 template <int U, int W, typename OutT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W)))
 void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec, const float2* __restrict__ orig,
@@ -1299,9 +1301,9 @@ void launch_count_typed(const DsDev& ds, QDev q, const CandRec* rec, const float
     if (variant == kCountValu8) {
         const int g = grid_for((ds.n_tiles + 7) / 8, kWaves, 1 << 20);
         k_count_all<8, 1, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
-    } else if (variant == kCountValu4Full) {
-        const int g = grid_for((ds.n_tiles + 3) / 4, kWaves, 1 << 20);
-        k_count_all<4, 8, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
+    } else if (variant == kCountValu6) {
+        const int g = grid_for((ds.n_tiles + 5) / 6, kWaves, 1 << 20);
+        k_count_all<6, 1, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
     } else {
         const int g = grid_for((ds.n_tiles + 3) / 4, kWaves, 1 << 20);
         k_count_all<4, 1, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
@@ -1310,7 +1312,7 @@ void launch_count_typed(const DsDev& ds, QDev q, const CandRec* rec, const float
 
 const char* count_variant_name(int variant)
 {
-    return variant == kCountValu8 ? "valu8" : variant == kCountValu4Full ? "valu4o8" : "valu4";
+    return variant == kCountValu8 ? "valu8" : variant == kCountValu6 ? "valu6" : "valu4";
 }
 
 bool count_narrow(uint32_t nc) { return nc < 65536u; }

@@ -65,7 +65,7 @@ void launch_plain(const DsDev& ds, QDev q, const CandRec* rec, const float2* ori
 // instructions per clean test in each)
 constexpr int kCountValu4 = 0;
 constexpr int kCountValu8 = 1;
-constexpr int kCountValu4Full = 2;
+constexpr int kCountValu6 = 2;
 constexpr int kCountVariants = 3;
 const char* count_variant_name(int variant);
 bool count_narrow(uint32_t nc);

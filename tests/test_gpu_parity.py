@@ -112,7 +112,7 @@ def test_ties_golden_every_form(gties):
         assert np.array_equal(g, exp)
 
 
-@pytest.mark.parametrize("kernel", ["valu4", "valu8", "valu4o8", "tuned"])
+@pytest.mark.parametrize("kernel", ["valu4", "valu6", "valu8", "tuned"])
 def test_count_all_matches_oracle(g1, gties, kernel, monkeypatch):
     """Dense per-user counts, every count-all kernel (4 or 8 users per
     lane) and the autotuned choice, against the oracle's counts."""
@@ -293,7 +293,7 @@ def test_autotuned_shapes_agree(rk, ctx, tmp_path, monkeypatch):
         ds.close()
 
 
-@pytest.mark.parametrize("kernel", ["valu4", "valu8", "valu4o8"])
+@pytest.mark.parametrize("kernel", ["valu4", "valu6", "valu8"])
 def test_count_all_large_vs_oracle(rk, ctx, kernel, monkeypatch):
     """Count-all on clustered data with exact bisector ties (config 3) and
     thousands of candidates: every per-user count equals the oracle's."""
