@@ -16,6 +16,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rk_internal.hpp"
@@ -325,6 +326,26 @@ rk_status batch_download_views(rk_batch* b)
 
 rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets, std::vector<uint32_t>* skip);
 
+// memcpy of n ids, split over a few host threads when large (one thread
+// copies ~10 GB/s out of pinned memory; a step's results are megabytes)
+void parallel_copy(int32_t* dst, const int32_t* src, uint64_t n)
+{
+    constexpr uint64_t kPerThread = 1ull << 20; // ids
+    const unsigned t = (unsigned)std::min<uint64_t>(8, n / kPerThread);
+    if (t < 2) {
+        std::memcpy(dst, src, n * sizeof(int32_t));
+        return;
+    }
+    std::vector<std::thread> pool;
+    const uint64_t per = (n + t - 1) / t;
+    for (unsigned k = 1; k < t; ++k) {
+        const uint64_t a = k * per, e = std::min(n, a + per);
+        if (a < e) pool.emplace_back([=] { std::memcpy(dst + a, src + a, (e - a) * sizeof(int32_t)); });
+    }
+    std::memcpy(dst, src, std::min(n, per) * sizeof(int32_t));
+    for (std::thread& th : pool) th.join();
+}
+
 } // namespace rk
 
 extern "C" {
@@ -511,14 +532,22 @@ rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offset
         RK_HIP(c, hipStreamSynchronize(c->stream));
         dev = b->h_out.p;
     }
-    for (uint32_t i = 0; i < nq; ++i) {
-        int32_t* dst = out_ids + offsets[i];
+    // the compacted ids in runs: consecutive queries whose places in the
+    // output and in the compacted array advance together are one copy
+    for (uint32_t i = 0; i < nq;) {
         const uint64_t n = offsets[i + 1] - offsets[i];
         if (b->hq[i].trivial) {
-            if (n) std::memcpy(dst, ds->all_ids_sorted.data(), n * sizeof(int32_t));
-        } else if (n) {
-            std::memcpy(dst, dev + qoff[i], n * sizeof(int32_t)); // sorted on the device
+            if (n) std::memcpy(out_ids + offsets[i], ds->all_ids_sorted.data(), n * sizeof(int32_t));
+            ++i;
+            continue;
         }
+        uint32_t j = i + 1;
+        while (j < nq && !b->hq[j].trivial && !out_of_seg[j] && !out_of_seg[j - 1] &&
+               offsets[j + 1] - offsets[j] == qoff[j + 1] - qoff[j])
+            ++j;
+        if (!out_of_seg[i] && offsets[j] > offsets[i]) // sorted on the device
+            parallel_copy(out_ids + offsets[i], dev + qoff[i], offsets[j] - offsets[i]);
+        i = j;
     }
     return RK_OK;
 }

@@ -43,10 +43,13 @@ struct Request {
     std::vector<uint32_t> cand; // host-filtered requests: filled by the host stage
 };
 
+// A finished request: its ids are [off, off + n) of its step's result
+// array, which the step's requests share (copied out once, by the poller).
 struct Done {
-    std::vector<int32_t> ids;
-    double latency;
-    rk_status status;
+    std::shared_ptr<const std::vector<int32_t>> buf;
+    uint64_t off = 0, n = 0;
+    double latency = 0.0;
+    rk_status status = RK_OK;
 };
 
 } // namespace
@@ -211,13 +214,18 @@ void dev_stage(rk_server* s)
                                   dev ? 0 : step[i].cand.size()};
         st = dev ? run_step_device(s, ds, qd, ids, off) : run_step(s, ds, qd, ids, off);
         const double t_end = rk::now_s();
-        // results are copied out before the lock is taken: submitters and
-        // pollers share it
+        // the step's ids become one shared array, each request a range of it
+        // (set up before the lock is taken: submitters and pollers share it)
+        const auto shared = std::make_shared<const std::vector<int32_t>>(std::move(ids));
         std::vector<Done> fin(step.size());
         for (size_t i = 0; i < step.size(); ++i) {
             fin[i].status = st;
             fin[i].latency = t_end - step[i].t_submit;
-            if (st == RK_OK) fin[i].ids.assign(ids.begin() + off[i], ids.begin() + off[i + 1]);
+            if (st == RK_OK) {
+                fin[i].buf = shared;
+                fin[i].off = off[i];
+                fin[i].n = off[i + 1] - off[i];
+            }
         }
         lk.lock();
         bool freed = false;
@@ -289,23 +297,23 @@ rk_status rk_serve_submit(rk_server* s, rk_dataset* ds, uint32_t q, uint32_t k, 
 rk_status rk_serve_poll(rk_server* s, uint64_t ticket, int32_t* out_ids, size_t cap, size_t* n_out, double* latency_s)
 {
     if (!s || !n_out || (cap && !out_ids)) return RK_ERR_INVALID_ARG;
-    std::lock_guard<std::mutex> lk(s->mu);
+    std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->done.find(ticket);
     if (it == s->done.end()) {
         if (ticket == 0 || ticket >= s->next_ticket) return RK_ERR_INVALID_ARG;
         return RK_ERR_NOT_READY;
     }
-    Done& d = it->second;
-    *n_out = d.ids.size();
+    Done d = it->second;
+    *n_out = d.n;
     if (latency_s) *latency_s = d.latency;
     if (d.status != RK_OK) {
-        rk_status st = d.status;
         s->done.erase(it);
-        return st;
+        return d.status;
     }
-    if (d.ids.size() > cap) return RK_ERR_CAPACITY; // keep the result for a larger buffer
-    if (!d.ids.empty()) std::memcpy(out_ids, d.ids.data(), d.ids.size() * sizeof(int32_t));
+    if (d.n > cap) return RK_ERR_CAPACITY; // keep the result for a larger buffer
     s->done.erase(it);
+    lk.unlock(); // the copy runs outside the server's lock
+    if (d.n) std::memcpy(out_ids, d.buf->data() + d.off, d.n * sizeof(int32_t));
     return RK_OK;
 }
 
