@@ -141,8 +141,14 @@ typedef struct rk_query_desc {
     uint64_t nc;
 } rk_query_desc;
 
-/* One call: prepare + run + fetch.  offsets has nq+1 entries; the ids of
- * query i are out_ids[offsets[i] .. offsets[i+1]), sorted ascending. */
+/* One call: fill + run + fetch.  offsets has nq+1 entries; the ids of
+ * query i are out_ids[offsets[i] .. offsets[i+1]), sorted ascending.  The
+ * kernels are launched directly on a batch object the dataset keeps per
+ * filter side (allocated once).  A query whose result outgrows its share of
+ * the batch's result pool, or whose device candidate list outgrows its
+ * segment, is run again on its own (exact-size segments, or the host
+ * filter); the others' results are kept.  RK_ERR_CAPACITY only when cap is
+ * below offsets[nq]. */
 rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq,
                          int32_t* out_ids, size_t cap, uint64_t* offsets, rk_timing* t);
 
@@ -152,7 +158,9 @@ rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq,
  * own data; RK_AUTOTUNE=0 disables, =8 or =4 forces that many waves per
  * query, RK_TUNE_FILE names the JSON cache) and
  * captures the hipGraph; run replays it (results stay in HBM); fetch copies
- * them out. */
+ * them out.  run returns RK_ERR_CAPACITY when a query's result outgrows its
+ * share of the result pool (2^26 ids per batch, RK_BATCH_POOL_IDS); fewer
+ * queries per batch give each a larger share. */
 rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, rk_batch** out);
 rk_status rk_batch_run(rk_batch* b, double* device_seconds);
 rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets);
@@ -171,10 +179,10 @@ rk_status rk_batch_destroy(rk_batch* b);
  * each run filters on the device (rk_filter_device's filter) before the
  * traversal, all in one graph, so a run goes from queries to result ids
  * without the host.  run returns RK_ERR_CAPACITY if a candidate list
- * exceeds its device segment (4096 facilities); rk_query_batch then filters
- * on the host instead.  rk_query_batch uses this form for queries without
- * candidate lists and k <= 64 unless RK_FILTER=host.  No reference
- * counterpart. */
+ * exceeds its device segment (4096 facilities); rk_query_batch then runs
+ * that query again through the host filter.  rk_query_batch uses this form
+ * for queries without candidate lists and k <= 64 unless RK_FILTER=host.  No
+ * reference counterpart. */
 rk_status rk_batch_prepare_device(rk_dataset* ds, const uint32_t* q, const uint32_t* k, size_t nq,
                                   rk_batch** out);
 /* Work counters of the last run: items[l] = tree nodes of level l that
