@@ -161,6 +161,31 @@ __device__ bool df_relevant(const double* ex, const double* ey, const double* bo
     return false;
 }
 
+// Up to this k the sector lanes keep their k smallest exit distances in
+// registers, sorted (an insertion is kDfRegK independent compare/select
+// pairs); beyond it in the LDS heaps (a sift walks dependent LDS accesses).
+constexpr uint32_t kDfRegK = 16;
+
+// Inserts v into h[0 .. kDfRegK), sorted ascending (h[i] becomes the larger
+// of h[i-1] and min(h[i], v): every new value depends only on old ones).
+__device__ __forceinline__ void reg_insert(float* h, float v)
+{
+#pragma unroll
+    for (int i = kDfRegK - 1; i > 0; --i) h[i] = v < h[i - 1] ? h[i - 1] : fminf(h[i], v);
+    h[0] = fminf(h[0], v);
+}
+
+// h[k - 1] of the sorted (non-negative) list, as the largest of its first k:
+// a reduction, so the array is never indexed at run time (which would move
+// it to scratch memory)
+__device__ __forceinline__ float reg_kth(const float* h, uint32_t k)
+{
+    float t = 0.f;
+#pragma unroll
+    for (int i = 0; i < (int)kDfRegK; ++i) t = fmaxf(t, (uint32_t)i < k ? h[i] : 0.f);
+    return t;
+}
+
 // fp32 value >= v
 __device__ __forceinline__ float round_up_f(double v)
 {
@@ -239,6 +264,10 @@ __global__ __launch_bounds__(64) void k_dfilter(FacGridDev g, const uint32_t* __
         s_bound[lane] = bound;
         double R = wave_max_d(bound);
         uint32_t hs = 0; // heap size of this lane's sector
+        const bool in_regs = k <= kDfRegK; // this query's sector lists in registers
+        float h[kDfRegK];
+#pragma unroll
+        for (int i = 0; i < (int)kDfRegK; ++i) h[i] = INFINITY;
         __syncthreads();
         const bool have_users = bx0 <= bx1 && by0 <= by1;
         const int cx = min(max((int)floor((qx - g.x0) / g.cs), 0), G - 1);
@@ -310,7 +339,13 @@ __global__ __launch_bounds__(64) void k_dfilter(FacGridDev g, const uint32_t* __
                         if (!(ca > 0.0 && cb > 0.0)) continue;
                         const double rmax = e.z * (1.0 + e.w) * (1.0 + 1e-12) / (2.0 * fmin(ca, cb));
                         const float v = round_up_f(rmax);
-                        if (hs < k) { // push, sift up
+                        if (in_regs) { // the k smallest: h[k - 1] is the LDS heap's top
+                            if (v < h[kDfRegK - 1]) {
+                                reg_insert(h, v);
+                                hs += hs < k ? 1u : 0u;
+                                upd = true;
+                            }
+                        } else if (hs < k) { // push, sift up
                             uint32_t j = hs++;
                             while (j > 0) {
                                 const uint32_t p = (j - 1) >> 1;
@@ -341,7 +376,7 @@ __global__ __launch_bounds__(64) void k_dfilter(FacGridDev g, const uint32_t* __
                         }
                     }
                     if (upd && hs == k) {
-                        const double bnd = fmin(capv, (double)s_heap[0][lane]);
+                        const double bnd = fmin(capv, (double)(in_regs ? reg_kth(h, k) : s_heap[0][lane]));
                         if (bnd < bound) {
                             bound = bnd;
                             s_bound[lane] = bnd;
