@@ -6,6 +6,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <algorithm>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -39,9 +40,12 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // contents are not kept; a regrowth takes at least 1.5x (buffers refilled
+    // step after step settle after a few allocations, which synchronise)
     hipError_t reserve(size_t want)
     {
         if (want <= n && p) return hipSuccess;
+        if (p) want = std::max(want, n + n / 2);
         release();
         if (want == 0) want = 1;
         hipError_t e = hipMalloc((void**)&p, want * sizeof(T));
@@ -67,9 +71,11 @@ struct HostBuf {
     {
         if (p) (void)hipHostFree(p);
     }
+    // as DevBuf::reserve: contents not kept, a regrowth takes at least 1.5x
     hipError_t reserve(size_t want)
     {
         if (want <= n && p) return hipSuccess;
+        if (p) want = std::max(want, n + n / 2);
         if (p) (void)hipHostFree(p);
         p = nullptr;
         n = 0;

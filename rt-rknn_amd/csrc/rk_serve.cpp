@@ -54,7 +54,22 @@ struct Done {
 
 } // namespace
 
+// Result arrays of finished steps, returned by the last poller instead of
+// freed: the device stage refills them with their pages still mapped (a
+// fresh multi-megabyte array per step page-faults on the device stage while
+// pollers unmap the old ones, and both wait on the process's memory map).
+struct IdPool {
+    std::mutex mu;
+    std::vector<std::vector<int32_t>*> free;
+    ~IdPool()
+    {
+        for (auto* v : free) delete v;
+    }
+};
+
 struct rk_server {
+    // first member: destroyed last, after the finished results that return to it
+    std::shared_ptr<IdPool> pool = std::make_shared<IdPool>();
     rk_ctx* ctx = nullptr;
     rk_serve_config cfg{};
     std::mutex mu;
@@ -206,7 +221,16 @@ void dev_stage(rk_server* s)
         s->n_ready -= take;
         lk.unlock();
         std::vector<uint64_t> off(step.size() + 1);
-        std::vector<int32_t> ids;
+        std::vector<int32_t>* ids_buf = nullptr;
+        {
+            std::lock_guard<std::mutex> pl(s->pool->mu);
+            if (!s->pool->free.empty()) {
+                ids_buf = s->pool->free.back();
+                s->pool->free.pop_back();
+            }
+        }
+        if (!ids_buf) ids_buf = new std::vector<int32_t>();
+        std::vector<int32_t>& ids = *ids_buf; // keeps its size: a step refills it without zeroing
         rk_status st;
         std::vector<rk_query_desc> qd(step.size());
         for (size_t i = 0; i < step.size(); ++i)
@@ -216,7 +240,13 @@ void dev_stage(rk_server* s)
         const double t_end = rk::now_s();
         // the step's ids become one shared array, each request a range of it
         // (set up before the lock is taken: submitters and pollers share it)
-        const auto shared = std::make_shared<const std::vector<int32_t>>(std::move(ids));
+        const std::shared_ptr<IdPool> pool = s->pool;
+        const std::shared_ptr<const std::vector<int32_t>> shared(ids_buf, [pool](const std::vector<int32_t>* v) {
+            auto* w = const_cast<std::vector<int32_t>*>(v);
+            std::lock_guard<std::mutex> pl(pool->mu);
+            if (pool->free.size() < 8) pool->free.push_back(w);
+            else delete w;
+        });
         std::vector<Done> fin(step.size());
         for (size_t i = 0; i < step.size(); ++i) {
             fin[i].status = st;
