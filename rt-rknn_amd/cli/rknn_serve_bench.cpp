@@ -142,15 +142,20 @@ int main(int argc, char** argv)
         rk_serve_config cfg{max_batch, 0, wait};
         rk_server* srv = nullptr;
         if ((st = rk_serve_start(ctx, &cfg, &srv)) != RK_OK) return die("rk_serve_start", ctx, st);
-        for (const Resident& r : res) { // untimed warm-up: a full step's worth per dataset grows the buffers
-            std::vector<uint64_t> wt(std::min<size_t>(max_batch, r.qs.size()));
-            for (size_t i = 0; i < wt.size(); ++i)
-                if (rk_serve_submit(srv, r.ds, r.qs[i], fixed_k ? (uint32_t)kmax : 1u + (uint32_t)(i % kmax),
-                                    &wt[i]) != RK_OK)
-                    wt[i] = 0;
+        // untimed warm-up: four rounds of a full step's worth per dataset, all
+        // in flight together, grow the server's buffers to their working size
+        for (int round = 0; round < 4; ++round) {
+            std::vector<uint64_t> wt;
+            for (const Resident& r : res)
+                for (size_t i = 0; i < std::min<size_t>(max_batch, r.qs.size()); ++i) {
+                    uint64_t t = 0;
+                    const uint32_t k = fixed_k ? (uint32_t)kmax : 1u + (uint32_t)((i + round) % kmax);
+                    if (rk_serve_submit(srv, r.ds, r.qs[(i * 7 + round) % r.qs.size()], k, &t) == RK_OK)
+                        wt.push_back(t);
+                }
             for (uint64_t t : wt) {
                 size_t n = 0;
-                while (t && rk_serve_poll(srv, t, out.data(), out.size(), &n, nullptr) == RK_ERR_NOT_READY)
+                while (rk_serve_poll(srv, t, out.data(), out.size(), &n, nullptr) == RK_ERR_NOT_READY)
                     std::this_thread::yield();
             }
         }

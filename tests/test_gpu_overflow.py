@@ -114,18 +114,20 @@ def test_large_results_sorted_on_the_device(rk, ctx, filt, negative_ids, monkeyp
         uid = uid.astype(np.int32)
     ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
     try:
-        qs = [int(q) for q in rk.generate_queries(400, 16, seed)]
-        ks = [8 + 4 * i for i in range(16)]
+        # 64 queries: over 2M result ids, so the host copy of the fetch is split over threads
+        qs = [int(q) for q in rk.generate_queries(400, 64, seed)]
+        ks = [8 + (i % 16) * 4 for i in range(64)]
         got = ds.query_batch(list(zip(qs, ks)))
         sizes = [len(g) for g in got]
-        print(f"{filt} negative={negative_ids}: sizes {min(sizes)}..{max(sizes)}, "
+        print(f"{filt} negative={negative_ids}: sizes {min(sizes)}..{max(sizes)}, total {sum(sizes)}, "
               f"{sum(s > 8192 for s in sizes)} beyond 8192", flush=True)
-        assert sum(s > 8192 for s in sizes) >= 8
+        assert sum(s > 8192 for s in sizes) >= 32
+        assert sum(sizes) > 2 << 20
         for q, k, g in zip(qs, ks, got):
             assert np.all(g[1:] > g[:-1])
             assert np.array_equal(g, ds.query(q, k)), (q, k)
         O.set_threads(16)
-        for i in (3, 15):
+        for i in (3, 15, 63):
             assert np.array_equal(got[i], O.rknn(ux, uy, uid, fx, fy, qs[i], ks[i]))
     finally:
         ds.close()
