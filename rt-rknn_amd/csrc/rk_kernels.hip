@@ -848,15 +848,40 @@ __device__ void descend(const DsDev& ds, const BatchDev& b, const QDev& Q, uint3
 // left untouched for the radix-sort fallback.
 constexpr uint32_t kMaxBucket = 32;
 
+// Wave-wide prefix operations on DPP: row shifts 1, 2, 4, 8 within each row
+// of 16 lanes, then row 15 broadcast into rows 1 and 3 and row 31 into rows
+// 2 and 3.  No lane-address registers are needed (a shuffle network keeps one
+// per step live, which spilled in the batched traversal kernel).  A lane
+// reading past its row, or in a row the mask leaves out, takes `id`.
+constexpr int kDppShr1 = 0x111, kDppShr2 = 0x112, kDppShr4 = 0x114, kDppShr8 = 0x118;
+constexpr int kDppBcast15 = 0x142, kDppBcast31 = 0x143;
+
+template <typename Op>
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t x, uint32_t id, Op op)
+{
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, kDppShr1, 0xf, 0xf, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, kDppShr2, 0xf, 0xf, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, kDppShr4, 0xf, 0xf, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, kDppShr8, 0xf, 0xf, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, kDppBcast15, 0xa, 0xf, false));
+    x = op(x, (uint32_t)__builtin_amdgcn_update_dpp((int)id, (int)x, kDppBcast31, 0xc, 0xf, false));
+    return x; // lane i: op over lanes 0..i
+}
+
+// inclusive prefix sum over the wave
+__device__ __forceinline__ uint32_t wave_scan_incl_u32(uint32_t v)
+{
+    return wave_prefix(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+}
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v)
 {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
-    return v;
+    const uint32_t p = wave_prefix(v, 0xffffffffu, [](uint32_t a, uint32_t b) { return min(a, b); });
+    return (uint32_t)__builtin_amdgcn_readlane((int)p, 63);
 }
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
 {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
-    return v;
+    const uint32_t p = wave_prefix(v, 0u, [](uint32_t a, uint32_t b) { return max(a, b); });
+    return (uint32_t)__builtin_amdgcn_readlane((int)p, 63);
 }
 
 // Sorts seg[0, n), n <= kThreads * kItems, in place.  LDS: out[kThreads *
@@ -911,11 +936,7 @@ __device__ bool bucket_sort(int32_t* __restrict__ seg, uint32_t n, uint32_t* out
         local[p] = cnt[tid * kPer + p];
         sum += local[p];
     }
-    uint32_t inc = sum;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t v = (uint32_t)__shfl_up((int)inc, o);
-        if ((int)lane >= o) inc += v;
-    }
+    const uint32_t inc = wave_scan_incl_u32(sum);
     if (lane == 63) red[wave] = inc;
     __syncthreads();
     uint32_t run = inc - sum;
