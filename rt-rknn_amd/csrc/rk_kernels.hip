@@ -296,7 +296,8 @@ __device__ __forceinline__ bool clean_for(const CandRec& r, float lx, float ly, 
 // zero unless a band value occurred.  A tile with one re-walks its slow
 // candidates with the oracle's binary64 comparison in the band.
 template <int U>
-__device__ __forceinline__ void count_slow(const CandRec* s, const float2* so, uint32_t n_slow, const DsDev& ds,
+__device__ __forceinline__ void count_slow(const CandRec* s, const float2* __restrict__ orig, uint32_t n_slow,
+                                           const DsDev& ds,
                                            uint32_t t0, uint32_t nt, const float* wx, const float* wy, float qx,
                                            float qy, float* acc, uint32_t& rech)
 {
@@ -335,9 +336,11 @@ __device__ __forceinline__ void count_slow(const CandRec* s, const float2* so, u
                 exact += (t >= kCertainHi) ? 1u : 0u;
                 const bool inband = in_band(t);
                 const uint64_t bm = ballot(inband);
-                if (bm) {
-                    const float2 f = so[i];
-                    if (inband && d2_exact(px, py, f.x, f.y) < dq) ++exact;
+                if (bm) { // the record's pad holds its candidate index
+                    if (inband) {
+                        const float2 f = orig[__float_as_uint(c.pad)];
+                        if (d2_exact(px, py, f.x, f.y) < dq) ++exact;
+                    }
                     rech += (uint32_t)__popcll(bm);
                 }
             }
@@ -382,7 +385,7 @@ template <int U, typename OutT>
 __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint32_t nc, bool lds_recs,
                                             const CandRec* __restrict__ srec, const CandRec* __restrict__ rec,
                                             const float2* __restrict__ orig, OutT* __restrict__ counts,
-                                            CandRec* wl, float2* wo, uint32_t g, const float* px, const float* py,
+                                            CandRec* wl, uint32_t g, const float* px, const float* py,
                                             float4 m, uint32_t& rech)
 {
     const uint32_t lane = lane_id();
@@ -411,9 +414,9 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
         const bool slow = have && !clean;
         const uint64_t mc = ballot(clean), ms = ballot(slow);
         if (clean) wl[rank_below(mc)] = r;
-        if (slow) {
+        if (slow) { // the facility's coordinates are read only if a band value occurs
+            r.pad = __uint_as_float(jc);
             wl[64 + rank_below(ms)] = r;
-            wo[rank_below(ms)] = orig[jc];
         }
         __builtin_amdgcn_wave_barrier();
         const uint32_t n_clean = (uint32_t)__popcll(mc), n_slow = (uint32_t)__popcll(ms);
@@ -424,7 +427,7 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
             for (int j = 0; j < U; ++j) // med3(t,0,1) folds into the FMA's clamp bit
                 acc[j] += __builtin_amdgcn_fmed3f(fmaf(wx[j], c.gx, fmaf(wy[j], c.gy, -c.sh)), 0.0f, 1.0f);
         }
-        if (n_slow) count_slow<U>(wl + 64, wo, n_slow, ds, t0, nt, wx, wy, q.qx, q.qy, acc, rech);
+        if (n_slow) count_slow<U>(wl + 64, orig, n_slow, ds, t0, nt, wx, wy, q.qx, q.qy, acc, rech);
         __builtin_amdgcn_wave_barrier();
     }
     OutT* __restrict__ out = counts + ((uint64_t)t0 * kTile + lane);
@@ -434,12 +437,13 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
 }
 
 // The first group's users are requested before the records are staged, so
-// the two loads overlap.  (A persistent grid that requests each wave's next
-// group while counting the current one measured slower at every grid size:
-// the extra registers cost a wave per SIMD and the strided walk leaves a
-// longer tail than the hardware dispatcher.)  W is the occupancy the
-// register allocation may be held to (W = 1 leaves it free: 53, 62 and 92
-// VGPRs for U = 4, 6 and 8).
+// the two loads overlap.  (Waves that request their next group's users while
+// counting the current one measured slower: over a persistent grid, the
+// strided walk leaves a longer tail than the hardware dispatcher; over runs
+// of 2 or 4 adjacent groups per wave, 41.7 and 35.1 TFLOP/s against 45.8 for
+// one group per wave at k = 10.)  W is the occupancy the register
+// allocation may be held to (W = 1 leaves it free: 53, 60 and 90 VGPRs for
+// U = 4, 6 and 8).
 template <int U, int W, typename OutT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(W)))
 void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec, const float2* __restrict__ orig,
@@ -447,7 +451,6 @@ void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec, const float2
 {
     extern __shared__ CandRec s_recs[];       // the query's records (nc <= kCountLdsRecs)
     __shared__ CandRec s_w[kWaves][128];      // per chunk: clean ones at [0, 64), the others at [64, 128)
-    __shared__ float2 s_o[kWaves][64];        // facility coordinates of the others
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nc = uniform(q.nc);
     const bool lds_recs = nc <= kCountLdsRecs;
@@ -463,8 +466,7 @@ void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec, const float2
     }
     uint32_t rech = 0;
     while (g < n_groups) {
-        count_group<U, OutT>(ds, q, nc, lds_recs, s_recs, rec, orig, counts, s_w[wave], s_o[wave], g, px, py, m,
-                             rech);
+        count_group<U, OutT>(ds, q, nc, lds_recs, s_recs, rec, orig, counts, s_w[wave], g, px, py, m, rech);
         g += stride;
         if (g < n_groups) load_group<U>(ds, g, px, py, m);
     }
