@@ -164,12 +164,14 @@ rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq,
 rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, rk_batch** out);
 rk_status rk_batch_run(rk_batch* b, double* device_seconds);
 rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets);
-/* rk_batch_run in two halves: enqueue puts one run on the context's stream
- * and returns; wait blocks until that run is done, then reports its device
- * time and checks its segments as rk_batch_run does (RK_ERR_CAPACITY).
- * Runs of several batches may be in flight at once (they execute in enqueue
- * order), so the device need not idle while the host waits; one batch holds
- * at most one run in flight, and fetch and stats need its wait first.  No
+/* rk_batch_run in two halves: enqueue puts one run on the batch's own
+ * stream and returns; wait blocks until that run is done, then reports its
+ * device time (first to last kernel of the run) and checks its segments as
+ * rk_batch_run does (RK_ERR_CAPACITY).  Runs of several batches may be in
+ * flight at once and overlap on the device, so the device need not idle
+ * while the host waits, and one run's blocks fill the SIMDs that another's
+ * finished blocks free.  One batch holds at most one run in flight; fetch
+ * and stats need its wait first.  No
  * reference counterpart (the reference's kernel call is synchronous,
  * rknn-inf-gpu/inf_kernel.cu:107). */
 rk_status rk_batch_enqueue(rk_batch* b);

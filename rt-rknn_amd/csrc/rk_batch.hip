@@ -140,6 +140,7 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
 {
     rk_dataset* ds = b->ds;
     rk_ctx* c = ds->ctx;
+    if (b->pending) return fail(c, RK_ERR_INVALID_ARG, "the batch's last run has not been waited for");
     if (nq > (1u << 24)) return fail(c, RK_ERR_INVALID_ARG, "too many queries in one batch");
     b->nq = (uint32_t)nq;
     b->ran = false;
@@ -244,6 +245,7 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
 {
     rk_dataset* ds = b->ds;
     rk_ctx* c = ds->ctx;
+    if (b->pending) return fail(c, RK_ERR_INVALID_ARG, "the batch's last run has not been waited for");
     if (nq > (1u << 24)) return fail(c, RK_ERR_INVALID_ARG, "too many queries in one batch");
     for (size_t i = 0; i < nq; ++i) {
         if (q[i] >= ds->nf) return fail(c, RK_ERR_INVALID_ARG, "query index q must be < number of facilities");
@@ -410,18 +412,22 @@ rk_status rk_batch_enqueue(rk_batch* b)
         RK_HIP(c, hipEventCreate(&b->ev_start));
         RK_HIP(c, hipEventCreate(&b->ev_end));
         RK_HIP(c, hipEventCreate(&b->ev_done));
+        RK_HIP(c, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)); // default priority
     }
-    RK_HIP(c, hipEventRecord(b->ev_start, c->stream));
+    // The fills finished on the context stream before they returned, and
+    // fetch runs only after the wait, so the batch's stream needs no ordering
+    // against the context stream.
+    hipStream_t st = b->stream;
+    RK_HIP(c, hipEventRecord(b->ev_start, st));
     if (direct)
-        batch_launch(b, c->stream);
+        batch_launch(b, st);
     else
-        RK_HIP(c, hipGraphLaunch(b->exec, c->stream));
+        RK_HIP(c, hipGraphLaunch(b->exec, st));
     RK_HIP(c, hipGetLastError());
-    RK_HIP(c, hipEventRecord(b->ev_end, c->stream));
+    RK_HIP(c, hipEventRecord(b->ev_end, st));
     if (direct) // a graph copies the counts itself
-        RK_HIP(c, hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                 c->stream));
-    RK_HIP(c, hipEventRecord(b->ev_done, c->stream));
+        RK_HIP(c, hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    RK_HIP(c, hipEventRecord(b->ev_done, st));
     return RK_OK;
 }
 

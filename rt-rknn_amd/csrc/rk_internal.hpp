@@ -179,8 +179,16 @@ struct rk_batch {
     bool pending = false;
     bool pending_empty = false; // nothing was launched (no users or no queries)
     hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_done = nullptr;
+    // the batch's own stream for its runs (created with the events), so that
+    // runs of different batches overlap on the device: the next batch's query
+    // blocks take the SIMDs that the previous batch's finished blocks free
+    hipStream_t stream = nullptr;
     ~rk_batch()
     {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         for (hipEvent_t e : {ev_start, ev_end, ev_done})

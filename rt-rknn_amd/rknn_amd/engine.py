@@ -232,7 +232,8 @@ class Batch:
         return s.value
 
     def enqueue(self):
-        """Puts one run on the context's stream and returns (rk_batch_enqueue)."""
+        """Puts one run on the batch's own stream and returns (rk_batch_enqueue):
+        runs of different batches overlap on the device."""
         self.ds.ctx.check(self._lib.rk_batch_enqueue(self.h))
 
     def wait(self):

@@ -63,8 +63,9 @@ def test_config1_batch_golden(g1):
 
 def test_batch_enqueue_wait_golden(rk, g1, gties):
     """rk_batch_enqueue / rk_batch_wait: batches of two datasets, graph-
-    captured and device-filtered, several runs in flight on one stream; every
-    fetch equals the golden sets.  Misuse is an error, not a hang."""
+    captured and device-filtered, four runs in flight at once, each on its
+    batch's own stream; every fetch equals the golden sets.  Misuse is an
+    error, not a hang."""
     bs = []
     for d, ds, qk, res in (g1, gties):
         keep = [(q, k) for q, k in qk if k <= 64]
