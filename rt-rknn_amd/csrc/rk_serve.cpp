@@ -58,6 +58,7 @@ struct Done {
 // freed: the device stage refills them with their pages still mapped (a
 // fresh multi-megabyte array per step page-faults on the device stage while
 // pollers unmap the old ones, and both wait on the process's memory map).
+constexpr size_t kPoolKeep = 8; // arrays kept
 struct IdPool {
     std::mutex mu;
     std::vector<std::vector<int32_t>*> free;
@@ -87,8 +88,10 @@ struct rk_server {
     std::vector<std::thread> host_threads; // filter workers
     std::thread dev_thread;
     std::condition_variable cv_dev;
-    std::map<rk_dataset*, std::unique_ptr<rk_batch>> batches;  // device stage only: host-filtered steps
-    std::map<rk_dataset*, std::unique_ptr<rk_batch>> dbatches; // device-filtered steps
+    // device stage only, two of each kind per dataset: a step is filled and
+    // enqueued on one while the previous step's run, on the other, finishes
+    std::map<rk_dataset*, std::unique_ptr<rk_batch>> batches[2];  // host-filtered steps
+    std::map<rk_dataset*, std::unique_ptr<rk_batch>> dbatches[2]; // device-filtered steps
     std::map<rk_dataset*, std::unique_ptr<rk_batch>> rbatches; // re-runs of the queries that outgrew a segment
     bool host_filter_only = false;                             // RK_FILTER=host
 };
@@ -155,53 +158,119 @@ rk_batch* batch_for(std::map<rk_dataset*, std::unique_ptr<rk_batch>>& m, rk_data
     return b.get();
 }
 
-// One host-filtered step over qd.  Each dataset keeps one batch object
-// whose device buffers only grow: refilled every step and launched without
-// a graph, it allocates nothing once warm.
-rk_status run_step(rk_server* s, rk_dataset* ds, const std::vector<rk_query_desc>& qd, std::vector<int32_t>& ids,
-                   std::vector<uint64_t>& off)
+// A step from its enqueue to its publication.
+struct Step {
+    std::vector<Request> req; // kept until published: re-runs use their candidate lists
+    std::vector<rk_query_desc> qd;
+    rk_dataset* ds = nullptr;
+    bool dev = false;
+    rk_batch* b = nullptr;
+    bool enqueued = false;
+    rk_status st = RK_OK;
+};
+
+// Fills the step's batch object and enqueues its run on the batch's stream.
+// The batch objects' device buffers only grow: refilled every step and
+// launched without a graph, they allocate nothing once warm.  Host-filtered
+// steps time the traversal shapes for a new workload class first;
+// device-filtered steps take the default shape (8 waves per query), since
+// timing shapes needs filtered lists, which only the step itself makes.
+void start_step(rk_server* s, Step& p, int slot)
 {
-    rk_batch* b = batch_for(s->batches, ds);
-    rk_status st = rk::batch_fill(b, qd.data(), qd.size());
-    if (st == RK_OK) {
-        std::lock_guard<std::recursive_mutex> lk(ds->ctx->mu);
-        st = rk::batch_tune(b);
+    if (p.dev) {
+        p.b = batch_for(s->dbatches[slot], p.ds);
+        std::vector<uint32_t> q(p.qd.size()), k(p.qd.size());
+        for (size_t i = 0; i < p.qd.size(); ++i) {
+            q[i] = p.qd[i].q;
+            k[i] = p.qd[i].k;
+        }
+        p.st = rk::batch_fill_device(p.b, q.data(), k.data(), q.size());
+    } else {
+        p.b = batch_for(s->batches[slot], p.ds);
+        p.st = rk::batch_fill(p.b, p.qd.data(), p.qd.size());
+        if (p.st == RK_OK) {
+            std::lock_guard<std::recursive_mutex> lk(p.ds->ctx->mu);
+            p.st = rk::batch_tune(p.b);
+        }
     }
-    if (st == RK_OK) st = rk_batch_run(b, nullptr);
-    if (st == RK_OK || st == RK_ERR_CAPACITY)
-        return rk::batch_collect(b, batch_for(s->rbatches, ds), qd.data(), qd.size(), ids, off);
-    return st;
+    if (p.st == RK_OK) p.st = rk_batch_enqueue(p.b);
+    p.enqueued = p.st == RK_OK;
 }
 
-// One device-filtered step over qd (q, k; no candidate lists): the
-// dataset's device batch, refilled in place and launched without a graph
-// (filter, setup, traversal, sort).  The traversal shape is the default (8
-// waves per query): timing shapes needs filtered lists, which only the step
-// itself makes.
-rk_status run_step_device(rk_server* s, rk_dataset* ds, const std::vector<rk_query_desc>& qd,
-                          std::vector<int32_t>& ids, std::vector<uint64_t>& off)
+// Waits for the step's run, collects its ids (queries that outgrew a
+// segment run again alone) and publishes them.
+void finish_step(rk_server* s, Step& p)
 {
-    rk_batch* b = batch_for(s->dbatches, ds);
-    std::vector<uint32_t> q(qd.size()), k(qd.size());
-    for (size_t i = 0; i < qd.size(); ++i) {
-        q[i] = qd[i].q;
-        k[i] = qd[i].k;
+    std::vector<uint64_t> off(p.req.size() + 1);
+    std::vector<int32_t>* ids_buf = nullptr;
+    {
+        std::lock_guard<std::mutex> pl(s->pool->mu);
+        if (!s->pool->free.empty()) {
+            ids_buf = s->pool->free.back();
+            s->pool->free.pop_back();
+        }
     }
-    rk_status st = rk::batch_fill_device(b, q.data(), k.data(), q.size());
-    if (st == RK_OK) st = rk_batch_run(b, nullptr);
-    if (st == RK_OK || st == RK_ERR_CAPACITY)
-        return rk::batch_collect(b, batch_for(s->rbatches, ds), qd.data(), qd.size(), ids, off);
-    return st;
+    if (!ids_buf) ids_buf = new std::vector<int32_t>();
+    std::vector<int32_t>& ids = *ids_buf; // keeps its size: a step refills it without zeroing
+    rk_status st = p.st;
+    if (p.enqueued) {
+        st = rk_batch_wait(p.b, nullptr);
+        if (st == RK_OK || st == RK_ERR_CAPACITY)
+            st = rk::batch_collect(p.b, batch_for(s->rbatches, p.ds), p.qd.data(), p.qd.size(), ids, off);
+    }
+    const double t_end = rk::now_s();
+    // the step's ids become one shared array, each request a range of it
+    // (set up before the lock is taken: submitters and pollers share it)
+    const std::shared_ptr<IdPool> pool = s->pool;
+    const std::shared_ptr<const std::vector<int32_t>> shared(ids_buf, [pool](const std::vector<int32_t>* v) {
+        auto* w = const_cast<std::vector<int32_t>*>(v);
+        std::lock_guard<std::mutex> pl(pool->mu);
+        if (pool->free.size() < kPoolKeep) pool->free.push_back(w);
+        else delete w;
+    });
+    std::vector<Done> fin(p.req.size());
+    for (size_t i = 0; i < p.req.size(); ++i) {
+        fin[i].status = st;
+        fin[i].latency = t_end - p.req[i].t_submit;
+        if (st == RK_OK) {
+            fin[i].buf = shared;
+            fin[i].off = off[i];
+            fin[i].n = off[i + 1] - off[i];
+        }
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    bool freed = false;
+    for (size_t i = 0; i < p.req.size(); ++i) {
+        s->done[p.req[i].ticket] = std::move(fin[i]);
+        s->reserved -= p.req[i].reserve;
+        freed |= p.req[i].reserve > 0;
+    }
+    if (freed && !s->waiting.empty()) s->cv.notify_all();
+    s->cv_done.notify_all();
 }
 
+// The device stage keeps up to two steps alive: it fills and enqueues the
+// next step, then finishes the previous one, so the device runs a step
+// while the host collects and publishes the one before.  With nothing ready
+// to overlap, the step in flight is finished at once.
 void dev_stage(rk_server* s)
 {
+    std::unique_ptr<Step> prev; // enqueued, not yet published
+    int slot = 0;
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
+        if (prev && s->n_ready == 0) {
+            lk.unlock();
+            finish_step(s, *prev);
+            prev.reset();
+            lk.lock();
+            continue;
+        }
         s->cv_dev.wait(lk, [&] { return s->stop || s->n_ready > 0; });
         if (s->stop && s->n_ready == 0) return;
-        // let batch-mates gather briefly when the step would be small
-        if (s->n_ready < s->cfg.max_batch && s->cfg.max_wait_s > 0 && !s->stop) {
+        // with the device idle, let batch-mates gather briefly when the step
+        // would be small
+        if (!prev && s->n_ready < s->cfg.max_batch && s->cfg.max_wait_s > 0 && !s->stop) {
             const double deadline = oldest_ready(s).front().t_submit + s->cfg.max_wait_s;
             while (!s->stop && s->n_ready < s->cfg.max_batch && !s->waiting.empty() && rk::now_s() < deadline) {
                 s->cv_dev.wait_for(lk, std::chrono::microseconds(200));
@@ -209,63 +278,27 @@ void dev_stage(rk_server* s)
         }
         // one step: the queue (dataset, filter side) whose head waited longest
         std::deque<Request>& qu = oldest_ready(s);
-        rk_dataset* ds = qu.front().ds;
-        const bool dev = qu.front().dev;
-        std::vector<Request> step;
+        std::unique_ptr<Step> cur(new Step());
+        cur->ds = qu.front().ds;
+        cur->dev = qu.front().dev;
         const size_t take = std::min<size_t>(qu.size(), s->cfg.max_batch);
-        step.reserve(take);
+        cur->req.reserve(take);
         for (size_t i = 0; i < take; ++i) {
-            step.push_back(std::move(qu.front()));
+            cur->req.push_back(std::move(qu.front()));
             qu.pop_front();
         }
         s->n_ready -= take;
         lk.unlock();
-        std::vector<uint64_t> off(step.size() + 1);
-        std::vector<int32_t>* ids_buf = nullptr;
-        {
-            std::lock_guard<std::mutex> pl(s->pool->mu);
-            if (!s->pool->free.empty()) {
-                ids_buf = s->pool->free.back();
-                s->pool->free.pop_back();
-            }
+        cur->qd.resize(take);
+        for (size_t i = 0; i < take; ++i) {
+            const Request& r = cur->req[i];
+            cur->qd[i] = rk_query_desc{r.q, r.k, cur->dev ? nullptr : r.cand.data(), cur->dev ? 0 : r.cand.size()};
         }
-        if (!ids_buf) ids_buf = new std::vector<int32_t>();
-        std::vector<int32_t>& ids = *ids_buf; // keeps its size: a step refills it without zeroing
-        rk_status st;
-        std::vector<rk_query_desc> qd(step.size());
-        for (size_t i = 0; i < step.size(); ++i)
-            qd[i] = rk_query_desc{step[i].q, step[i].k, dev ? nullptr : step[i].cand.data(),
-                                  dev ? 0 : step[i].cand.size()};
-        st = dev ? run_step_device(s, ds, qd, ids, off) : run_step(s, ds, qd, ids, off);
-        const double t_end = rk::now_s();
-        // the step's ids become one shared array, each request a range of it
-        // (set up before the lock is taken: submitters and pollers share it)
-        const std::shared_ptr<IdPool> pool = s->pool;
-        const std::shared_ptr<const std::vector<int32_t>> shared(ids_buf, [pool](const std::vector<int32_t>* v) {
-            auto* w = const_cast<std::vector<int32_t>*>(v);
-            std::lock_guard<std::mutex> pl(pool->mu);
-            if (pool->free.size() < 8) pool->free.push_back(w);
-            else delete w;
-        });
-        std::vector<Done> fin(step.size());
-        for (size_t i = 0; i < step.size(); ++i) {
-            fin[i].status = st;
-            fin[i].latency = t_end - step[i].t_submit;
-            if (st == RK_OK) {
-                fin[i].buf = shared;
-                fin[i].off = off[i];
-                fin[i].n = off[i + 1] - off[i];
-            }
-        }
+        slot ^= 1; // never the batch object of the step in flight
+        start_step(s, *cur, slot);
+        if (prev) finish_step(s, *prev);
+        prev = std::move(cur);
         lk.lock();
-        bool freed = false;
-        for (size_t i = 0; i < step.size(); ++i) {
-            s->done[step[i].ticket] = std::move(fin[i]);
-            s->reserved -= step[i].reserve;
-            freed |= step[i].reserve > 0;
-        }
-        if (freed && !s->waiting.empty()) s->cv.notify_all();
-        s->cv_done.notify_all();
     }
 }
 
