@@ -44,7 +44,7 @@ PEAK_HBM_GBS = 8000.0
 # from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known
 # size (profiles/k_count_all_10M_100k.md); the kernel reads every user once and
 # writes one 16-bit count per user, so the figure does not depend on the query
-MEASURED_TRAFFIC = {"valu4": 101.4e6, "valu6": 101.0e6, "valu8": 100.8e6}
+MEASURED_TRAFFIC = {"valu4": 101.4e6, "valu6": 101.1e6, "valu8": 100.8e6}
 
 
 def log(*a):
