@@ -200,8 +200,13 @@ rk_status rk_batch_config(const rk_batch* b, uint32_t* waves_per_query);
 typedef struct rk_serve_config {
     uint32_t max_batch;          /* queries per device step (default 256)          */
     uint64_t memory_budget;      /* device bytes for in-flight work (default 4 GiB) */
-    double max_wait_s;           /* longest a request waits for batch-mates         */
+    double max_wait_s;           /* longest a request waits for batch-mates while   *
+                                  * no step is in flight                            */
 } rk_serve_config;
+/* The server's device thread keeps up to two steps alive: it fills and
+ * enqueues the next step, then finishes (waits for, collects, publishes) the
+ * previous one, so the device runs a step while the host finishes the one
+ * before.  A step in flight is finished at once when nothing else is ready. */
 
 rk_status rk_serve_start(rk_ctx* ctx, const rk_serve_config* cfg, rk_server** out);
 /* Thread-safe.  Returns a ticket; RK_ERR_ADMISSION if the request can never
