@@ -19,6 +19,7 @@
 // from its own candidates.
 #include <algorithm>
 #include <condition_variable>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -167,7 +168,19 @@ struct Step {
     rk_batch* b = nullptr;
     bool enqueued = false;
     rk_status st = RK_OK;
+    double t_start = 0, t_filled = 0, t_enqueued = 0; // phase clocks for RK_SERVE_TRACE
 };
+
+// RK_SERVE_TRACE=1: a line on stderr for every step with a phase longer than
+// a millisecond (fill, enqueue, wait for the run, collect, publish).
+bool serve_trace()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("RK_SERVE_TRACE");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
 
 // Fills the step's batch object and enqueues its run on the batch's stream.
 // The batch objects' device buffers only grow: refilled every step and
@@ -177,6 +190,7 @@ struct Step {
 // timing shapes needs filtered lists, which only the step itself makes.
 void start_step(rk_server* s, Step& p, int slot)
 {
+    p.t_start = rk::now_s();
     if (p.dev) {
         p.b = batch_for(s->dbatches[slot], p.ds);
         std::vector<uint32_t> q(p.qd.size()), k(p.qd.size());
@@ -193,8 +207,10 @@ void start_step(rk_server* s, Step& p, int slot)
             p.st = rk::batch_tune(p.b);
         }
     }
+    p.t_filled = rk::now_s();
     if (p.st == RK_OK) p.st = rk_batch_enqueue(p.b);
     p.enqueued = p.st == RK_OK;
+    p.t_enqueued = rk::now_s();
 }
 
 // Waits for the step's run, collects its ids (queries that outgrew a
@@ -213,12 +229,25 @@ void finish_step(rk_server* s, Step& p)
     if (!ids_buf) ids_buf = new std::vector<int32_t>();
     std::vector<int32_t>& ids = *ids_buf; // keeps its size: a step refills it without zeroing
     rk_status st = p.st;
+    const double t_finish = rk::now_s();
+    double t_waited = t_finish;
     if (p.enqueued) {
         st = rk_batch_wait(p.b, nullptr);
+        t_waited = rk::now_s();
         if (st == RK_OK || st == RK_ERR_CAPACITY)
             st = rk::batch_collect(p.b, batch_for(s->rbatches, p.ds), p.qd.data(), p.qd.size(), ids, off);
     }
     const double t_end = rk::now_s();
+    if (serve_trace()) {
+        const double ms[4] = {1e3 * (p.t_filled - p.t_start), 1e3 * (p.t_enqueued - p.t_filled),
+                              1e3 * (t_waited - t_finish), 1e3 * (t_end - t_waited)};
+        if (ms[0] > 1.0 || ms[1] > 1.0 || ms[2] > 1.0 || ms[3] > 1.0)
+            std::fprintf(stderr,
+                         "rk_serve: slow step: %zu requests, %zu ids; fill %.3f ms, enqueue %.3f ms, "
+                         "wait %.3f ms, collect %.3f ms; %.3f ms from enqueue to finish\n",
+                         p.req.size(), (size_t)off[p.req.size()], ms[0], ms[1], ms[2], ms[3],
+                         1e3 * (t_finish - p.t_enqueued));
+    }
     // the step's ids become one shared array, each request a range of it
     // (set up before the lock is taken: submitters and pollers share it)
     const std::shared_ptr<IdPool> pool = s->pool;
