@@ -312,7 +312,7 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
 rk_status batch_download_views(rk_batch* b)
 {
     rk_ctx* c = b->ds->ctx;
-    hipStream_t st = b->stream ? b->stream : c->stream; // the run's stream, idle after its wait
+    hipStream_t st = c->stream;
     if (!b->nq) return RK_OK;
     RK_HIP(c, hipMemcpyAsync(b->hq.data(), b->q.p, b->nq * sizeof(QDev), hipMemcpyDeviceToHost, st));
     RK_HIP(c, hipMemcpyAsync(b->h_dn.data(), b->dn.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -332,7 +332,7 @@ rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offset
 // copies ~10 GB/s out of pinned memory; a step's results are megabytes)
 void parallel_copy(int32_t* dst, const int32_t* src, uint64_t n)
 {
-    constexpr uint64_t kPerThread = 1ull << 18; // ids (1 MiB): a few threads pay off from about 2 MiB
+    constexpr uint64_t kPerThread = 1ull << 20; // ids
     const unsigned t = (unsigned)std::min<uint64_t>(8, n / kPerThread);
     if (t < 2) {
         std::memcpy(dst, src, n * sizeof(int32_t));
@@ -506,16 +506,14 @@ rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offset
     }
     std::vector<uint64_t> qoff(nq + 1, 0);
     const bool dev_ran = b->exec || (b->direct && ds->nu && nq);
-    // On the batch's own stream, idle since the run was waited for: the
-    // context stream may share a hardware queue with another batch's run in
-    // flight, and the fetch would queue behind it.  (A null-stream copy
-    // would not wait for the scan.)
-    hipStream_t st = b->stream ? b->stream : c->stream;
     if (dev_ran) {
-        launch_batch_scan(b->view, st);
+        launch_batch_scan(b->view, c->stream);
         RK_HIP(c, hipGetLastError());
-        RK_HIP(c, hipMemcpyAsync(qoff.data(), b->qoff.p, (nq + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        RK_HIP(c, hipStreamSynchronize(st));
+        // on the context's (non-blocking) stream: a null-stream copy would not
+        // wait for the scan
+        RK_HIP(c, hipMemcpyAsync(qoff.data(), b->qoff.p, (nq + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+        RK_HIP(c, hipStreamSynchronize(c->stream));
         for (uint32_t i = 0; i < nq; ++i)
             if (qoff[i + 1] - qoff[i] != std::min(b->h_qcount.p[i], b->view.seg_cap))
                 return fail(c, RK_ERR_INTERNAL, "result offsets disagree with the per-query counts");
@@ -532,11 +530,12 @@ rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offset
     if (dev_ran && qoff[nq]) {
         RK_HIP(c, b->out.reserve(qoff[nq]));
         b->view.out = b->out.p; // compaction is outside the graph, so the pointer may change
-        launch_batch_compact(b->view, st);
+        launch_batch_compact(b->view, c->stream);
         RK_HIP(c, hipGetLastError());
         RK_HIP(c, b->h_out.reserve(qoff[nq])); // pinned: a direct DMA, no zero-filled staging vector
-        RK_HIP(c, hipMemcpyAsync(b->h_out.p, b->out.p, qoff[nq] * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-        RK_HIP(c, hipStreamSynchronize(st));
+        RK_HIP(c, hipMemcpyAsync(b->h_out.p, b->out.p, qoff[nq] * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+        RK_HIP(c, hipStreamSynchronize(c->stream));
         dev = b->h_out.p;
     }
     // the compacted ids in runs: consecutive queries whose places in the
