@@ -402,32 +402,53 @@ rk_status rk_batch_enqueue(rk_batch* b)
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
     if (b->pending) return fail(c, RK_ERR_INVALID_ARG, "the batch's last run has not been waited for");
-    b->ran = true;
+    // results of an earlier run are void from here on; the batch is pending
+    // only once everything of this run is on its stream, so a failed call
+    // leaves it free for the next fill or enqueue
+    b->ran = false;
     b->overflow = false;
+    b->cand_overflow = false;
     const bool direct = !b->exec && b->direct && b->ds->nu && b->nq;
-    b->pending = true;
-    b->pending_empty = !b->exec && !direct;
-    if (b->pending_empty) return RK_OK;
-    if (!b->ev_start) {
-        RK_HIP(c, hipEventCreate(&b->ev_start));
-        RK_HIP(c, hipEventCreate(&b->ev_end));
-        RK_HIP(c, hipEventCreate(&b->ev_done));
-        RK_HIP(c, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)); // default priority
+    if (!b->exec && !direct) {
+        b->ran = true;
+        b->pending = true;
+        b->pending_empty = true;
+        return RK_OK;
     }
+    // each object is created on its own, so one that failed is retried
+    if (!b->ev_start) RK_HIP(c, hipEventCreate(&b->ev_start));
+    if (!b->ev_end) RK_HIP(c, hipEventCreate(&b->ev_end));
+    if (!b->ev_done) RK_HIP(c, hipEventCreate(&b->ev_done));
+    if (!b->stream) RK_HIP(c, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)); // default priority
     // The fills finished on the context stream before they returned, and
     // fetch runs only after the wait, so the batch's stream needs no ordering
     // against the context stream.
     hipStream_t st = b->stream;
-    RK_HIP(c, hipEventRecord(b->ev_start, st));
-    if (direct)
-        batch_launch(b, st);
-    else
-        RK_HIP(c, hipGraphLaunch(b->exec, st));
-    RK_HIP(c, hipGetLastError());
-    RK_HIP(c, hipEventRecord(b->ev_end, st));
-    if (direct) // a graph copies the counts itself
-        RK_HIP(c, hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
-    RK_HIP(c, hipEventRecord(b->ev_done, st));
+    hipError_t e = hipEventRecord(b->ev_start, st);
+    const char* what = "hipEventRecord";
+    if (e == hipSuccess) {
+        if (direct) {
+            batch_launch(b, st);
+            e = hipGetLastError();
+            what = "kernel launch";
+        } else {
+            e = hipGraphLaunch(b->exec, st);
+            what = "hipGraphLaunch";
+        }
+    }
+    if (e == hipSuccess) e = hipEventRecord(b->ev_end, st);
+    if (e == hipSuccess && direct) { // a graph copies the counts itself
+        e = hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+        what = "hipMemcpyAsync (counts)";
+    }
+    if (e == hipSuccess) e = hipEventRecord(b->ev_done, st);
+    if (e != hipSuccess) {
+        // whatever reached the stream finishes before the buffers are refilled
+        (void)hipStreamSynchronize(st);
+        return fail_hip(c, e, what, __LINE__);
+    }
+    b->pending = true;
+    b->pending_empty = false;
     return RK_OK;
 }
 
@@ -456,10 +477,14 @@ rk_status rk_batch_wait(rk_batch* b, double* device_seconds)
     if (b->dfilter) {
         const rk_status s = batch_download_views(b);
         if (s != RK_OK) return s;
-        if (b->cand_overflow)
-            return fail(c, RK_ERR_CAPACITY, "a query's device candidate list exceeds its segment; filter on the host");
     }
+    // both checks before either is reported: fetch refuses a run with
+    // either kind of overflow (a truncated candidate list gives too many
+    // results, a truncated segment too few)
     for (uint32_t i = 0; i < b->nq; ++i) b->overflow |= b->h_qcount.p[i] > b->view.seg_cap;
+    b->ran = true;
+    if (b->cand_overflow)
+        return fail(c, RK_ERR_CAPACITY, "a query's device candidate list exceeds its segment; filter on the host");
     if (b->overflow) return fail(c, RK_ERR_CAPACITY, "a query's result exceeds its segment; split the batch");
     return RK_OK;
 }
@@ -492,6 +517,8 @@ rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offset
     if (b->pending) return fail(c, RK_ERR_INVALID_ARG, "rk_batch_wait has not been called");
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
+    if (b->cand_overflow && !skip)
+        return fail(c, RK_ERR_CAPACITY, "the last run overflowed a device candidate segment; filter on the host");
     if (b->overflow && !skip)
         return fail(c, RK_ERR_CAPACITY, "the last run overflowed a result segment; split the batch");
     const uint32_t nq = b->nq;

@@ -1,0 +1,131 @@
+"""GPU parity at the sizes the configurations run: the rknn-inf configuration
+in full (1M users, 10k facilities, 100 queries, k=10), and a dataset deep
+enough for a 4-level tile tree (more than 64^4 = 16,777,216 users), as the
+50M-user serving dataset has.  Every form -- plain, graph-captured batch,
+device-filtered batch, one call, served -- against the FP64 oracle over the
+host filter's candidate lists (the lists are sufficient: the filter tests
+pin that against brute force)."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_sets(arrs, qs, ks, cands):
+    ux, uy, uid, fx, fy, fid = arrs
+    O.set_threads(16)
+    return [O.rknn(ux, uy, uid, fx, fy, q, k, cand=c) for q, k, c in zip(qs, ks, cands)]
+
+
+def served(rk, ctx, ds, qs, ks):
+    srv = rk.Server(ctx, max_batch=64, max_wait_s=0.002)
+    try:
+        tickets = [srv.submit(ds, q, k) for q, k in zip(qs, ks)]
+        res = {}
+        deadline = time.time() + 90
+        while len(res) < len(tickets) and time.time() < deadline:
+            for t in tickets:
+                if t not in res:
+                    got = srv.poll(t, ds.nu)
+                    if got is not None:
+                        res[t] = got[0]
+            time.sleep(0.001)
+        assert len(res) == len(tickets)
+        return [res[t] for t in tickets]
+    finally:
+        srv.stop()
+
+
+def test_config2_full_size(rk, ctx):
+    """rknn-inf form: 1,000,000 users, 10,000 facilities, 100 queries, k=10."""
+    seed = rk.default_seed(2)
+    arrs = rk.generate(2, 1_000_000, 10_000, seed=seed)
+    qs = [int(q) for q in rk.generate_queries(10_000, 100, seed)]
+    ks = [10] * 100
+    ds = rk.Dataset(ctx, *arrs)
+    try:
+        cands = [ds.filter(q, k) for q, k in zip(qs, ks)]
+        exp = oracle_sets(arrs, qs, ks, cands)
+        print("config 2 full size: |R| mean", np.mean([len(e) for e in exp]), flush=True)
+        for i, (q, k, c) in enumerate(zip(qs, ks, cands)):
+            assert np.array_equal(ds.query_plain(q, k, c), exp[i]), (q, k)
+        b = ds.prepare_batch(list(zip(qs, ks, cands)))
+        try:
+            b.run()
+            for i, g in enumerate(b.fetch()):
+                assert np.array_equal(g, exp[i]), qs[i]
+        finally:
+            b.close()
+        for i, g in enumerate(ds.query_batch(list(zip(qs, ks)))):
+            assert np.array_equal(g, exp[i]), qs[i]
+    finally:
+        ds.close()
+
+
+@pytest.fixture(scope="module")
+def deep(rk, ctx):
+    """17M users over 20k facilities (config-4 generator: half uniform, half
+    clustered with exact ties): a 4-level tile tree, results of ~850*k ids."""
+    nu, nf = 17_000_000, 20_000
+    seed = rk.default_seed(4)
+    t0 = time.time()
+    arrs = rk.generate(4, nu, nf, seed=seed, nq_ties=12, seed_ties=seed)
+    qs = [int(q) for q in rk.generate_queries(nf, 12, seed)]
+    ks = [1, 10, 50] * 4
+    ds = rk.Dataset(ctx, *arrs)
+    cands = [ds.filter(q, k) for q, k in zip(qs, ks)]
+    exp = oracle_sets(arrs, qs, ks, cands)
+    print(f"deep dataset: {nu} users, {nf} facilities, |C| {[len(c) for c in cands]}, "
+          f"|R| {[len(e) for e in exp]} ({time.time() - t0:.1f}s)", flush=True)
+    yield arrs, ds, qs, ks, cands, exp
+    ds.close()
+
+
+def test_deep_tree_has_four_levels(deep):
+    arrs, ds, qs, ks, cands, exp = deep
+    b = ds.prepare_batch(list(zip(qs, ks, cands)))
+    try:
+        b.run()
+        items, res, _ = b.stats()
+        assert len(items) == 4, items
+        assert all(n > 0 for n in items), items  # every level was descended
+        assert res == sum(len(e) for e in exp)
+        for i, g in enumerate(b.fetch()):
+            assert np.array_equal(g, exp[i]), (qs[i], ks[i])
+    finally:
+        b.close()
+
+
+def test_deep_plain(deep):
+    arrs, ds, qs, ks, cands, exp = deep
+    for q, k, c, e in zip(qs, ks, cands, exp):
+        assert np.array_equal(ds.query_plain(q, k, c), e), (q, k)
+
+
+def test_deep_device_filtered_batch(deep):
+    arrs, ds, qs, ks, cands, exp = deep
+    b = ds.prepare_batch(list(zip(qs, ks)), device_filter=True)
+    try:
+        b.run()
+        for i, g in enumerate(b.fetch()):
+            assert np.array_equal(g, exp[i]), (qs[i], ks[i])
+    finally:
+        b.close()
+
+
+@pytest.mark.parametrize("filt", ["device", "host"])
+def test_deep_one_call(deep, filt, monkeypatch):
+    arrs, ds, qs, ks, cands, exp = deep
+    monkeypatch.setenv("RK_FILTER", filt)
+    for i, g in enumerate(ds.query_batch(list(zip(qs, ks)))):
+        assert np.array_equal(g, exp[i]), (qs[i], ks[i])
+
+
+def test_deep_served(rk, ctx, deep):
+    arrs, ds, qs, ks, cands, exp = deep
+    for i, g in enumerate(served(rk, ctx, ds, qs, ks)):
+        assert np.array_equal(g, exp[i]), (qs[i], ks[i])

@@ -131,3 +131,31 @@ def test_large_results_sorted_on_the_device(rk, ctx, filt, negative_ids, monkeyp
             assert np.array_equal(got[i], O.rknn(ux, uy, uid, fx, fy, qs[i], ks[i]))
     finally:
         ds.close()
+
+
+def test_fetch_refuses_a_candidate_overflow(rk, data, monkeypatch):
+    """A device-filtered batch whose candidate list outgrew its segment: the
+    run reports RK_ERR_CAPACITY and a plain fetch of that run is refused (its
+    truncated lists would give too many results); a later run that fits is
+    fetched normally."""
+    arrs, ds, qs, ks, plain = data
+    monkeypatch.setenv("RK_DFILTER_SLOTS", "24")
+    b = ds.prepare_batch(list(zip(qs, ks)), device_filter=True)
+    try:
+        with pytest.raises(rk.RkError) as e:
+            b.run()
+        assert e.value.status == 4
+        with pytest.raises(rk.RkError) as e:
+            b.fetch()
+        assert e.value.status == 4
+    finally:
+        b.close()
+    monkeypatch.delenv("RK_DFILTER_SLOTS")
+    idx = list(range(8))
+    b = ds.prepare_batch([(qs[i], ks[i]) for i in idx], device_filter=True)
+    try:
+        b.run()
+        for i, g in zip(idx, b.fetch()):
+            assert np.array_equal(g, plain[i]), (qs[i], ks[i])
+    finally:
+        b.close()
