@@ -116,6 +116,27 @@ rk_status rk_query_plain(rk_dataset* ds, uint32_t q, uint32_t k,
                          const uint32_t* cand, size_t nc,
                          int32_t* out_ids, size_t cap, size_t* n_out, rk_timing* t);
 
+/* ---- allocation-free plain form.  No counterpart in the reference: it is
+ *      for callers that cannot allocate device memory (code that must not
+ *      call hipMalloc: serving loops with a fixed footprint, harnesses that
+ *      forbid allocation).  Same semantics and result as rk_query_plain; the
+ *      kernels' scratch (candidate indices, their records, counters, the
+ *      staging of the result ids) is one fixed array per device, set up by
+ *      rk_init and held until the process ends (about 4.3 MB).  No call
+ *      allocates device or pinned memory.  Limits: nc <= 4096
+ *      (RK_ERR_INVALID_ARG beyond); a result counted on the device holds at
+ *      most 1,048,576 ids (RK_ERR_CAPACITY with *n_out = |R| beyond, as when
+ *      cap < |R|; a query with fewer live candidates than k returns every
+ *      user from the host copy, at any size).  Calls on one device
+ *      share the array and the library serialises them (a per-device lock);
+ *      each call clears what it needs, so a call never sees an earlier call's
+ *      contents. ---- */
+#define RK_NOALLOC_MAX_CANDIDATES 4096u
+#define RK_NOALLOC_MAX_IDS 1048576u
+rk_status rk_query_plain_noalloc(rk_dataset* ds, uint32_t q, uint32_t k,
+                                 const uint32_t* cand, size_t nc,
+                                 int32_t* out_ids, size_t cap, size_t* n_out);
+
 /* ---- brute-force form: C = all facilities but q; replaces gpuReverseKNN
  *      (rknn-naive-gpu/rknn_kernel.cuh:40-46) without its N_u x N_f matrix. ---- */
 rk_status rk_query_bruteforce(rk_dataset* ds, uint32_t q, uint32_t k,

@@ -47,6 +47,53 @@ rk_status clean_candidates(rk_dataset* ds, uint32_t q, const uint32_t* cand, siz
 
 using namespace rk;
 
+// ------------------------------------------------- allocation-free scratch
+//
+// rk_query_plain_noalloc's device memory beyond its inputs and outputs: the
+// query's candidate indices, their records, the plain kernel's counters and
+// the staging of the result ids.  One array per device, held until the
+// process ends; each device's copy lives at its own address, which
+// hipGetSymbolAddress reports (once per device, at rk_init) and which host
+// code uses in place of the array's name.
+namespace {
+
+constexpr uint32_t kNaMaxCand = RK_NOALLOC_MAX_CANDIDATES;
+constexpr uint32_t kNaMaxIds = RK_NOALLOC_MAX_IDS;
+constexpr size_t kNaCandOff = 0;
+constexpr size_t kNaRecOff = kNaCandOff + kNaMaxCand * sizeof(uint32_t);
+constexpr size_t kNaOrigOff = kNaRecOff + kNaMaxCand * sizeof(CandRec);
+constexpr size_t kNaCtrOff = kNaOrigOff + kNaMaxCand * sizeof(float2);
+constexpr size_t kNaIdsOff = kNaCtrOff + 256 * sizeof(uint32_t);
+constexpr size_t kNaBytes = kNaIdsOff + (size_t)kNaMaxIds * sizeof(int32_t);
+static_assert(kPlainCounters <= 256, "counter slots");
+static_assert(kNaRecOff % 16 == 0 && kNaOrigOff % 16 == 0 && kNaIdsOff % 16 == 0, "scratch alignment");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;                            // guards g_na_ptr
+unsigned char* g_na_ptr[kNaMaxDevices] = {};   // per device: the scratch array's device address
+std::mutex g_na_call[kNaMaxDevices];           // per device: one allocation-free call at a time
+
+// With `device` current: the device address of its scratch array, looked
+// up once per device.
+hipError_t scratch_address(int device, unsigned char** out)
+{
+    *out = nullptr;
+    if (device < 0 || device >= kNaMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_na_ptr[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_na_ptr[device];
+    return hipSuccess;
+}
+
+} // namespace
+
 namespace {
 
 bool finite_bounded(const float* v, size_t n)
@@ -154,7 +201,8 @@ rk_status plain_core(rk_dataset* ds, uint32_t q, uint32_t k, const std::vector<u
     const double t1 = now_s();
     QDev qd = make_qdev(ds, q, k, cl.size());
     RK_HIP(c, hipEventRecord(c->ev0, c->stream));
-    launch_plain(ds->view, qd, ds->rec.p, ds->orig.p, ds->out.p, ds->ctr.p, device_grid(c->n_cu, 32), c->stream);
+    launch_plain(ds->view, qd, ds->rec.p, ds->orig.p, ds->out.p, ds->n_pad, ds->ctr.p, device_grid(c->n_cu, 32),
+                 c->stream);
     RK_HIP(c, hipGetLastError());
     RK_HIP(c, hipEventRecord(c->ev1, c->stream));
     RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, kPlainCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -230,6 +278,13 @@ rk_status rk_init(int device, rk_ctx** out)
         return RK_ERR_HIP;
     }
     if (c->n_cu <= 0) c->n_cu = 1;
+    if (scratch_address(device, &c->scratch) != hipSuccess) {
+        (void)hipEventDestroy(c->ev0);
+        (void)hipEventDestroy(c->ev1);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return RK_ERR_HIP;
+    }
     *out = c;
     return RK_OK;
 }
@@ -466,6 +521,68 @@ rk_status rk_query(rk_dataset* ds, uint32_t q, uint32_t k, int32_t* out_ids, siz
     s = plain_core(ds, q, k, cl, out_ids, cap, n_out, t, t0);
     if (t) t->filter_s = t1 - t0;
     return s;
+}
+
+rk_status rk_query_plain_noalloc(rk_dataset* ds, uint32_t q, uint32_t k, const uint32_t* cand, size_t nc,
+                                 int32_t* out_ids, size_t cap, size_t* n_out)
+{
+    rk_status s = check_query(ds, q, k);
+    if (s != RK_OK) return s;
+    rk_ctx* c = ds->ctx;
+    if (!n_out || (cap && !out_ids)) return fail(c, RK_ERR_INVALID_ARG, "NULL output");
+    if (nc && !cand) return fail(c, RK_ERR_INVALID_ARG, "candidate list is NULL with nc > 0");
+    if (nc > kNaMaxCand)
+        return fail(c, RK_ERR_INVALID_ARG,
+                    "the allocation-free form takes at most RK_NOALLOC_MAX_CANDIDATES (4096) candidates");
+    if (!c->scratch) return fail(c, RK_ERR_INTERNAL, "no scratch address for this device");
+    // the candidate list cleaned as clean_candidates does, counted in place
+    const float qx = ds->hfx[q], qy = ds->hfy[q];
+    size_t live = 0;
+    for (size_t i = 0; i < nc; ++i) {
+        const uint32_t f = cand[i];
+        if (f >= ds->nf) return fail(c, RK_ERR_INVALID_ARG, "candidate index out of range");
+        live += (f == q || (ds->hfx[f] == qx && ds->hfy[f] == qy)) ? 0 : 1;
+    }
+    *n_out = 0;
+    if (ds->nu == 0) return RK_OK;
+    if (k > live) // fewer candidates than k: every user qualifies
+        return deliver(ds, ds->all_ids_sorted.data(), ds->all_ids_sorted.size(), out_ids, cap, n_out);
+    double wx = 0, wy = 0;
+    axis_bounds(ds->ubox[0], ds->ubox[1], ds->ubox[2], ds->ubox[3], qx, qy, &wx, &wy);
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    std::lock_guard<std::mutex> one(g_na_call[c->device]); // contexts sharing the device share the array
+    RK_HIP(c, hipSetDevice(c->device));
+    unsigned char* base = c->scratch;
+    uint32_t* d_cand = reinterpret_cast<uint32_t*>(base + kNaCandOff);
+    CandRec* d_rec = reinterpret_cast<CandRec*>(base + kNaRecOff);
+    float2* d_orig = reinterpret_cast<float2*>(base + kNaOrigOff);
+    uint32_t* d_ctr = reinterpret_cast<uint32_t*>(base + kNaCtrOff);
+    int32_t* d_ids = reinterpret_cast<int32_t*>(base + kNaIdsOff);
+    hipStream_t st = c->stream;
+    if (nc) RK_HIP(c, hipMemcpyAsync(d_cand, cand, nc * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    // records of every listed candidate (dead ones for q and its position)
+    // and the counters cleared by the same launch
+    launch_build_records(d_cand, (uint32_t)nc, q, qx, qy, wx, wy, ds->fx.p, ds->fy.p, d_rec, d_orig, d_ctr,
+                         kPlainCounters, st);
+    RK_HIP(c, hipGetLastError());
+    QDev qd = make_qdev(ds, q, k, nc);
+    launch_plain(ds->view, qd, d_rec, d_orig, d_ids, kNaMaxIds, d_ctr, device_grid(c->n_cu, 32), st);
+    RK_HIP(c, hipGetLastError());
+    uint32_t n = 0;
+    RK_HIP(c, hipMemcpyAsync(&n, d_ctr, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    RK_HIP(c, hipStreamSynchronize(st));
+    *n_out = n;
+    if (n > kNaMaxIds)
+        return fail(c, RK_ERR_CAPACITY,
+                    "the result exceeds the allocation-free form's staging of RK_NOALLOC_MAX_IDS (1048576) ids; "
+                    "*n_out holds its size");
+    if (n > cap) return fail(c, RK_ERR_CAPACITY, "output buffer too small; *n_out holds the size needed");
+    if (n) {
+        RK_HIP(c, hipMemcpyAsync(out_ids, d_ids, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        RK_HIP(c, hipStreamSynchronize(st));
+    }
+    std::sort(out_ids, out_ids + n); // in place: no allocation
+    return RK_OK;
 }
 
 rk_status rk_count_all(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc, uint32_t* counts, rk_timing* t)

@@ -100,6 +100,10 @@ struct rk_ctx {
     std::recursive_mutex mu; // serialises device work on this context
     std::string err;
     std::string arch;        // gcnArchName without feature suffixes (autotuner classes)
+    // device address of the allocation-free form's scratch array on this
+    // context's device (hipGetSymbolAddress at rk_init; the array's name in
+    // host code is not a device address)
+    unsigned char* scratch = nullptr;
 };
 
 struct rk_dataset {

@@ -207,10 +207,12 @@ __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, ui
 
 constexpr uint32_t kPlainRecLds = 512;
 
+// Kept users go to out[] through ctr[0]; ids past out_cap are counted but
+// not stored (the caller sees the count and reports the shortfall).
 template <bool kShared>
 __device__ void plain_tiles(const DsDev& ds, const QDev& q, const CandRec* __restrict__ rec,
                             const float2* __restrict__ orig, WaveLds* L, int32_t* __restrict__ out,
-                            uint32_t* __restrict__ ctr, uint32_t& rech)
+                            uint32_t out_cap, uint32_t* __restrict__ ctr, uint32_t& rech)
 {
     const uint32_t lane = lane_id();
     const uint32_t stride = gridDim.x * kWaves;
@@ -222,14 +224,15 @@ __device__ void plain_tiles(const DsDev& ds, const QDev& q, const CandRec* __res
             uint32_t pos = 0;
             if (lane == 0) pos = atomicAdd(&ctr[0], (uint32_t)__popcll(r.keep));
             pos = uniform(pos);
-            if ((r.keep >> lane) & 1ull) out[pos + rank_below(r.keep)] = ds.uid[tile * kTile + lane];
+            const uint32_t at = pos + rank_below(r.keep);
+            if (((r.keep >> lane) & 1ull) && at < out_cap) out[at] = ds.uid[tile * kTile + lane];
         }
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_plain(DsDev ds, QDev q, const CandRec* __restrict__ rec,
                                                   const float2* __restrict__ orig, int32_t* __restrict__ out,
-                                                  uint32_t* __restrict__ ctr)
+                                                  uint32_t out_cap, uint32_t* __restrict__ ctr)
 {
     __shared__ union {
         SharedRecs<kPlainRecLds> s;
@@ -243,11 +246,41 @@ __global__ __launch_bounds__(kBlock) void k_plain(DsDev ds, QDev q, const CandRe
             lds.s.orig[j] = orig[j];
         }
         __syncthreads();
-        plain_tiles<true>(ds, q, lds.s.rec, lds.s.orig, nullptr, out, ctr, rech);
+        plain_tiles<true>(ds, q, lds.s.rec, lds.s.orig, nullptr, out, out_cap, ctr, rech);
     } else {
-        plain_tiles<false>(ds, q, rec, orig, &lds.w[wave], out, ctr, rech);
+        plain_tiles<false>(ds, q, rec, orig, &lds.w[wave], out, out_cap, ctr, rech);
     }
     if (lane_id() == 0 && rech) atomicAdd(&ctr[1 + (blockIdx.x * kWaves + wave) % kBuckets], rech);
+}
+
+// Candidate records of one query built on the device from facility indices
+// (binary64 -> fp32 as the host does it).  A candidate that can never be
+// strictly closer than q -- q itself, or a facility at q's exact position --
+// gets a record that misses every user (t = -1 everywhere: never counted,
+// never in the band).  The first block also clears the plain form's
+// counters, so the launch that follows needs no memset.
+__global__ void k_build_records(const uint32_t* __restrict__ cand, uint32_t nc, uint32_t q, float qx, float qy,
+                                double wx, double wy, const float* __restrict__ fx, const float* __restrict__ fy,
+                                CandRec* __restrict__ rec, float2* __restrict__ orig, uint32_t* __restrict__ ctr,
+                                uint32_t n_ctr)
+{
+    if (blockIdx.x == 0)
+        for (uint32_t i = threadIdx.x; i < n_ctr; i += blockDim.x) ctr[i] = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+        const uint32_t f = cand[j];
+        const float x = fx[f], y = fy[f];
+        CandRec r;
+        if (f == q || (x == qx && y == qy)) {
+            r.gx = 0.f;
+            r.gy = 0.f;
+            r.sh = 1.f;
+            r.pad = 0.f;
+        } else {
+            r = make_record(qx, qy, x, y, wx, wy);
+        }
+        rec[j] = r;
+        orig[j] = make_float2(x, y);
+    }
 }
 // ------------------------------------------------------------ count-all form
 //
@@ -1309,9 +1342,17 @@ void launch_group_mbr(const float4* tiles, uint32_t n_tiles, uint32_t fan, float
 }
 
 void launch_plain(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, int32_t* out_ids,
-                  uint32_t* counters, int grid, hipStream_t st)
+                  uint32_t out_cap, uint32_t* counters, int grid, hipStream_t st)
 {
-    k_plain<<<grid_for(ds.n_tiles, kWaves, grid), kBlock, 0, st>>>(ds, q, rec, orig, out_ids, counters);
+    k_plain<<<grid_for(ds.n_tiles, kWaves, grid), kBlock, 0, st>>>(ds, q, rec, orig, out_ids, out_cap, counters);
+}
+
+void launch_build_records(const uint32_t* cand, uint32_t nc, uint32_t q, float qx, float qy, double wx, double wy,
+                          const float* fx, const float* fy, CandRec* rec, float2* orig, uint32_t* ctr,
+                          uint32_t n_ctr, hipStream_t st)
+{
+    k_build_records<<<grid_for(nc, 256, 64), 256, 0, st>>>(cand, nc, q, qx, qy, wx, wy, fx, fy, rec, orig, ctr,
+                                                           n_ctr);
 }
 
 // Count-all grid: a block for every kWaves groups (capped; the kernel loops

@@ -55,8 +55,14 @@ void launch_group_mbr(const float4* tiles, uint32_t n_tiles, uint32_t fan, float
 // plain form: one launch over every tile for one query
 // counters: [0] = ids written, [1 .. kBuckets] = binary64 tests per wave slot
 constexpr int kPlainCounters = 1 + kBuckets;
+// out_ids holds out_cap ids; counters[0] counts every kept user, stored or not
 void launch_plain(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, int32_t* out_ids,
-                  uint32_t* counters, int grid, hipStream_t st);
+                  uint32_t out_cap, uint32_t* counters, int grid, hipStream_t st);
+// records of nc candidate indices for query q (dead records for q and for
+// facilities at q's position); also zeroes counters[0 .. n_ctr)
+void launch_build_records(const uint32_t* cand, uint32_t nc, uint32_t q, float qx, float qy, double wx, double wy,
+                          const float* fx, const float* fy, CandRec* rec, float2* orig, uint32_t* counters,
+                          uint32_t n_ctr, hipStream_t st);
 
 // count-all form: exact per-user count (sorted order), no early exit.
 // rechecks has kBuckets slots.  Counts are 16-bit when count_narrow(nc)

@@ -21,6 +21,10 @@ RK_ERR_ADMISSION = 7
 RK_ERR_NOT_READY = 8
 RK_ERR_INTERNAL = 9
 
+# limits of the allocation-free plain form (include/rknn.h)
+RK_NOALLOC_MAX_CANDIDATES = 4096
+RK_NOALLOC_MAX_IDS = 1048576
+
 
 class rk_timing(C.Structure):
     _fields_ = [
@@ -76,6 +80,7 @@ SIGNATURES = {
                               C.c_size_t, SZP]),
     "rk_filter_device": (ST, [P, U32P, U32P, C.c_size_t, U32P, C.c_size_t, U64P]),
     "rk_query_plain": (ST, [P, C.c_uint32, C.c_uint32, U32P, C.c_size_t, I32P, C.c_size_t, SZP, TP]),
+    "rk_query_plain_noalloc": (ST, [P, C.c_uint32, C.c_uint32, U32P, C.c_size_t, I32P, C.c_size_t, SZP]),
     "rk_query_bruteforce": (ST, [P, C.c_uint32, C.c_uint32, I32P, C.c_size_t, SZP, TP]),
     "rk_query": (ST, [P, C.c_uint32, C.c_uint32, I32P, C.c_size_t, SZP, TP]),
     "rk_count_all": (ST, [P, C.c_uint32, U32P, C.c_size_t, U32P, TP]),

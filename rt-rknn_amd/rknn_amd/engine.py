@@ -135,6 +135,19 @@ class Dataset:
         ids, t = self._ids_call(self._lib.rk_query_plain, q, k, _ptr(cand, C.c_uint32), len(cand))
         return (ids, t) if timing else ids
 
+    def query_plain_noalloc(self, q, k, cand, cap=None):
+        """rk_query_plain_noalloc: the plain form on the library's fixed
+        per-device scratch (no device allocation).  cap bounds the output."""
+        cand = _u32(cand)
+        cap = max(1, self.nu) if cap is None else cap
+        out = np.empty(max(1, cap), dtype=np.int32)
+        n = C.c_size_t()
+        st = self._lib.rk_query_plain_noalloc(self.h, q, k, _ptr(cand, C.c_uint32), len(cand),
+                                              _ptr(out, C.c_int32), cap, C.byref(n))
+        if st != L.RK_OK:
+            raise RkError(st, self._lib.rk_last_error(self.ctx.h).decode() + f" (n_out={n.value})")
+        return out[: n.value].copy()
+
     def query_bruteforce(self, q, k, timing=False):
         ids, t = self._ids_call(self._lib.rk_query_bruteforce, q, k)
         return (ids, t) if timing else ids
