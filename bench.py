@@ -3,23 +3,31 @@ batched (one graph-captured launch sequence per step) versus plain (one
 counting launch, sync and readback per query), bit-exact sets.
 
 A step = one pass of the batched form over one batch of NQ queries whose
-candidate lists (from the host influence-zone filter) are already resident
-in HBM; the users are Hilbert-sorted and resident.  `value` counts queries/s
-over exactly --steps timed steps, two batch objects of the same queries
-alternating so that one is enqueued while the other runs
-(batched_sync_ms_per_step: one run at a time).  The JSON line also carries:
+candidate lists (from the host influence-zone filter) are resident in HBM;
+the users are Hilbert-sorted and resident.  The step ends with every query's
+sorted result ids in pinned host memory: its last kernel writes them there.
+`value` counts queries/s over exactly --steps timed steps that rotate over
+--batches distinct batches (8 by default: 8,192 distinct queries), two in
+flight, each step's results taken in place (rk_batch_results) after its wait;
+`value_fetch` is the same with every step's ids copied into a caller buffer
+(rk_batch_fetch).  The JSON line also carries:
   roofline      dense counting kernel (count-all: every user x every candidate,
                 4 fp32 flops per test) timed with HIP events on its stream
   cpu_baseline  the reference's tree-culled verification (rknn-inf) restated
                 in C, exact in binary64, on the step's queries and candidate
                 lists, one host thread (cpu_baseline_mt: every host core;
                 cpu_bruteforce: rknn-naive semantics on two queries)
-  plain_*       the one-launch-per-query form on the same queries
-  mixed_k       the same step with k drawn uniformly from 1..50 per query
-  device_filtered_step  the same step with the influence-zone filter inside it
-                (queries in, results in HBM)
+  cpu_reference the reference's own CPU programs, built from its sources into
+                oracle/_ref: rknn-inf on the rknn-inf configuration (1M users,
+                10k facilities), rknn-naive on the 10k/1k configuration; host
+                core count and CPU model
+  plain_*       the one-launch-per-query form on batch 0's queries
+  mixed_k       the step with k drawn uniformly from 1..50 per query
+  device_filtered_step  the step with the influence-zone filter inside it
+                (queries in, results in host memory)
   end_to_end    one rk_query_batch call: filter (device, or host cores) + step +
-                download, on the library's own clock (Python wrapper time beside)
+                the copy into the caller's buffer, on the library's own clock,
+                at k=10 and at k ~ U{1..50}
   serve         the serving loop under Poisson arrivals (rknn-serve-bench), k=10
   serve_datasets  the same with 1M/10k, 10M/100k and 50M/200k resident together, k ~ U{1..50}
 """
@@ -94,7 +102,11 @@ def main():
     ap.add_argument("--roofline-queries", type=int, default=8)
     ap.add_argument("--cpu-queries", type=int, default=2, help="queries of the brute-force CPU context run")
     ap.add_argument("--inflight", type=int, default=2,
-                    help="batch objects of the same queries alternating in the timed step, each on its own stream")
+                    help="batches in flight in the timed steps, each on its own stream")
+    ap.add_argument("--batches", type=int, default=8,
+                    help="distinct batches of --queries queries rotated through the timed steps")
+    ap.add_argument("--ref-queries", type=int, default=2,
+                    help="queries per reference CPU program (cpu_reference; one process each)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-qps", default="100000,300000,500000,1000000")
@@ -130,62 +142,81 @@ def main():
     ds = R.Dataset(ctx, ux, uy, uid, fx, fy, fid)
     log(f"dataset resident in {time.time() - t0:.2f}s ({ds.device_bytes() / 2**20:.0f} MiB)")
 
-    # host influence-zone filter (outside the timed region; reported)
+    # host influence-zone filter for every query of the rotation (outside the
+    # timed region; reported)
+    nb = max(1, args.batches)
+    nq = args.queries
+    qs_all = [int(q) for q in R.generate_queries(args.facilities, nb * nq, seed)]
     t0 = time.time()
-    cands = [ds.filter(q, args.k) for q in qs]
+    cands_all = [ds.filter(q, args.k) for q in qs_all]
     filter_s = time.time() - t0
-    ncs = np.array([len(c) for c in cands])
-    log(f"filter: {args.queries} queries in {filter_s:.2f}s, |C| mean {ncs.mean():.1f} "
+    ncs = np.array([len(c) for c in cands_all])
+    log(f"filter: {nb * nq} queries in {filter_s:.2f}s, |C| mean {ncs.mean():.1f} "
         f"min {ncs.min()} max {ncs.max()}")
+    qs, cands = qs_all[:nq], cands_all[:nq]  # batch 0: the other sections' queries
 
-    # ---- batched form: prepared once, replayed per step.  Batch objects of
-    # the same queries (--inflight, 2 by default) alternate, each enqueued
-    # while the others run (rk_batch_enqueue / rk_batch_wait).  Each batch
-    # runs on a stream of its own, so the runs overlap on the device: the next
-    # run's query blocks take the SIMDs that the previous run's finished
-    # blocks free.  Every
-    # step is a full pass over the 1,024 queries, its counts checked on the
-    # host.  The synchronous form (one run at a time, rk_batch_run) is timed
-    # beside it as the per-step latency, and its device time (HIP events
-    # around the graph) is the step's device time alone.
-    n_inflight = max(1, args.inflight)
-    batches = [ds.prepare_batch([(q, args.k, c) for q, c in zip(qs, cands)]) for _ in range(n_inflight)]
+    # ---- batched form: nb batches of distinct queries, each prepared once
+    # (candidate lists uploaded, graph captured) and rotated step after step,
+    # two in flight (rk_batch_enqueue / rk_batch_wait, each batch on its own
+    # stream); a step is done when its results are in host memory.  Their
+    # first run sizes each batch's landing area (and captures its graph again).
+    batches = [ds.prepare_batch([(q, args.k, c) for q, c in zip(qs_all[j * nq:(j + 1) * nq],
+                                                                 cands_all[j * nq:(j + 1) * nq])])
+               for j in range(nb)]
+    for b in batches:
+        b.run()
     batch = batches[0]
     for _ in range(args.warmup):
         batch.run()
     t0 = time.perf_counter()
     dev_sync = [batch.run() for _ in range(args.steps)]
     sync_ms = (time.perf_counter() - t0) / args.steps * 1e3
-    for _ in range(args.warmup):
-        for b in batches:
+    n_inflight = max(1, args.inflight)
+
+    def rotate(steps, take):
+        """Seconds per step over the rotation; take(b) after each wait."""
+        pending = []
+        for s in range(max(steps, 0)):
+            b = batches[s % nb]
+            if len(pending) == n_inflight:
+                c = pending.pop(0)
+                c.wait()
+                take(c)
             b.enqueue()
-        for b in batches:
-            b.wait()
-    dev, pending = [], []
+            pending.append(b)
+        while pending:
+            c = pending.pop(0)
+            c.wait()
+            take(c)
+
+    rotate(args.warmup * nb, lambda b: b.results_raw())
     t0 = time.perf_counter()
-    for s in range(args.steps):
-        b = batches[s % n_inflight]
-        if len(pending) == n_inflight:
-            dev.append(pending.pop(0).wait())
-        b.enqueue()
-        pending.append(b)
-    while pending:
-        dev.append(pending.pop(0).wait())
+    rotate(args.steps, lambda b: b.results_raw())
     wall = time.perf_counter() - t0
-    batched = batch.fetch()
-    for b in batches[1:]:
-        if not all(np.array_equal(a, c) for a, c in zip(batched, b.fetch())):
-            log("ERROR: batch objects of the same queries differ")
+    rotate(nb, lambda b: b.fetch_flat())
+    t0 = time.perf_counter()
+    rotate(args.steps, lambda b: b.fetch_flat())
+    wall_fetch = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    value = nq * args.steps / wall
+    value_fetch = nq * args.steps / wall_fetch
+    # every batch's results, in place and fetched, must agree (checked after the timing)
+    batched_all = []
+    for b in batches:
+        b.run()
+        got = b.fetch()
+        view = b.results()
+        if not all(np.array_equal(a, v) for a, v in zip(got, view)):
+            log("ERROR: a batch's in-place results differ from its fetched ones")
             sys.exit(1)
-        b.close()
+        batched_all.append(got)
+    batched = batched_all[0]
     items, n_res, n_rech = batch.stats()
     log(f"batched work per step: items per level (leaf first) {items}, results {n_res}, fp64 re-checks {n_rech}; "
         f"autotuned block: {batch.waves_per_query()} waves per query")
-    ms_step = wall / args.steps * 1e3
-    value = args.queries * args.steps / wall
-    log(f"batched: {ms_step:.3f} ms/step (one run at a time {sync_ms:.3f} ms, device {np.mean(dev_sync) * 1e3:.3f} ms; "
-        f"{np.mean(dev) * 1e3:.3f} ms from start to end of a run overlapping another), "
-        f"{value:,.0f} queries/s")
+    log(f"batched, {nb} batches rotating, results in host memory: {ms_step:.3f} ms/step, {value:,.0f} queries/s "
+        f"(ids also copied to a caller buffer: {nq * args.steps / wall_fetch:,.0f}); one run at a time "
+        f"{sync_ms:.3f} ms, device {np.mean(dev_sync) * 1e3:.3f} ms")
 
     # ---- mixed k (1..50 per query, the batched workload's k mix), same step
     rng = np.random.default_rng(seed)
@@ -241,22 +272,27 @@ def main():
     # (the library's own clock around the call, rk_timing.total_s, is the
     # C-ABI figure; the wall time of the Python wrapper is reported beside it)
     end_to_end = {}
-    for filt in ("device", "host"):
+    for name, filt, kk, ref in (("device_filter", "device", [args.k] * nq, batched),
+                                ("host_filter", "host", [args.k] * nq, batched),
+                                ("mixed_k_device_filter", "device", [int(k) for k in ks], mixed_res),
+                                ("mixed_k_host_filter", "host", [int(k) for k in ks], mixed_res)):
         os.environ["RK_FILTER"] = filt
-        ds.query_batch([(q, args.k) for q in qs])  # warm: the dataset's batch buffers grow once
+        ds.query_batch(list(zip(qs, kk)))  # warm: the dataset's batch buffers grow once
         calls, e2e_ok = [], True
         for _ in range(3):
             t0 = time.perf_counter()
-            e2e_res, tm = ds.query_batch([(q, args.k) for q in qs], timing=True)
-            calls.append((tm.total_s, time.perf_counter() - t0, tm.filter_s, tm.kernel_s, tm.download_s))
-            e2e_ok = e2e_ok and all(np.array_equal(a, b) for a, b in zip(e2e_res, batched))
-        lib_s, py_s, f_s, k_s, d_s = (float(np.median([c[j] for c in calls])) for j in range(5))
-        end_to_end[f"{filt}_filter"] = {"qps": round(args.queries / lib_s, 1), "ms": round(lib_s * 1e3, 3),
-                                        "filter_ms": round(f_s * 1e3, 3), "device_ms": round(k_s * 1e3, 3),
-                                        "download_ms": round(d_s * 1e3, 3),
-                                        "python_wall_ms": round(py_s * 1e3, 3), "matches": bool(e2e_ok)}
-        log(f"end to end, {filt} filter (filter + step + download): {args.queries / lib_s:,.0f} queries/s "
-            f"(library clock, median of 3; {py_s * 1e3:.2f} ms through the Python wrapper); same sets: {e2e_ok}")
+            e2e_res, tm = ds.query_batch(list(zip(qs, kk)), timing=True)
+            calls.append((tm.total_s, time.perf_counter() - t0, tm.filter_s, tm.kernel_s, tm.download_s,
+                          tm.upload_s))
+            e2e_ok = e2e_ok and all(np.array_equal(a, b) for a, b in zip(e2e_res, ref))
+        lib_s, py_s, f_s, k_s, d_s, u_s = (float(np.median([c[j] for c in calls])) for j in range(6))
+        end_to_end[name] = {"qps": round(nq / lib_s, 1), "ms": round(lib_s * 1e3, 3),
+                            "filter_ms": round(f_s * 1e3, 3), "fill_ms": round(u_s * 1e3, 3),
+                            "device_ms": round(k_s * 1e3, 3), "download_ms": round(d_s * 1e3, 3),
+                            "python_wall_ms": round(py_s * 1e3, 3), "matches": bool(e2e_ok)}
+        log(f"end to end, {name} (filter + step + copy to the caller): {nq / lib_s:,.0f} queries/s "
+            f"(library clock, median of 3: filter {f_s * 1e3:.3f}, fill {u_s * 1e3:.3f}, device {k_s * 1e3:.3f}, "
+            f"copy {d_s * 1e3:.3f} ms; {py_s * 1e3:.2f} ms through the Python wrapper); same sets: {e2e_ok}")
         if not e2e_ok:
             break
     del os.environ["RK_FILTER"]
@@ -279,8 +315,16 @@ def main():
     plain_wall = time.perf_counter() - t0
     plain_qps = n_plain / plain_wall
     lat_ms = np.array(lat) * 1e3
+    # the rotation's other batches: 32 queries each against the plain form
+    checked = n_plain
+    for j in range(1, nb):
+        for i in range(min(32, nq)):
+            q = qs_all[j * nq + i]
+            mismatches += int(not np.array_equal(ds.query_plain(q, args.k, cands_all[j * nq + i]),
+                                                 batched_all[j][i]))
+            checked += 1
     log(f"plain: {plain_qps:,.0f} queries/s, p50 {np.percentile(lat_ms, 50):.3f} ms, "
-        f"p99 {np.percentile(lat_ms, 99):.3f} ms; batched == plain on {n_plain - mismatches}/{n_plain}")
+        f"p99 {np.percentile(lat_ms, 99):.3f} ms; batched == plain on {checked - mismatches}/{checked}")
     if mismatches:
         log("ERROR: batched and plain result sets differ")
         sys.exit(1)
@@ -369,6 +413,8 @@ def main():
             sys.exit(1)
         tree.close()
 
+    cpu_reference = None if args.no_cpu else reference_programs(R, args.ref_queries)
+
     # ---- serving loop under Poisson arrivals (native load generator)
     serve = serve_datasets = None
     sb = os.path.join(ROOT, "rt-rknn_amd", "bin", "rknn-serve-bench")
@@ -412,22 +458,25 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64 generator, config %d)" % args.config,
         "config": {"workload": f"batched RkNN, {args.users} users, {args.facilities} facilities, "
-                               f"{args.queries} queries per step, k={args.k}",
+                               f"{args.queries} queries per step, k={args.k}, results in host memory",
                    "users": args.users, "facilities": args.facilities, "queries_per_step": args.queries,
                    "k": args.k, "candidates_mean": round(float(ncs.mean()), 2), "generator_config": args.config,
-                   "batches_in_flight": n_inflight},
+                   "distinct_batches": nb, "batches_in_flight": n_inflight},
+        "value_fetch": round(value_fetch, 1),
         "plain_qps": round(plain_qps, 1),
         "batched_vs_plain": round(value / plain_qps, 2),
+        "batched_vs_plain_basis": "both with their sorted result ids in host memory: each step's last kernel "
+                                  "writes them to pinned memory; each plain call reads its ids back",
         "plain_latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 4),
                              "p99": round(float(np.percentile(lat_ms, 99)), 4)},
         "batched_device_ms": round(float(np.mean(dev_sync)) * 1e3, 4),
-        "batched_overlapped_run_ms": round(float(np.mean(dev)) * 1e3, 4),
         "batched_sync_ms_per_step": round(sync_ms, 4),
         "filter_qps": round(args.queries / filter_s, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baseline_mt": cpu_mt,
         "cpu_bruteforce": cpu_bf,
+        "cpu_reference": cpu_reference,
         "mixed_k": mixed,
         "device_filtered_step": dfilter,
         "end_to_end": end_to_end,
@@ -439,6 +488,101 @@ def main():
         batch.close()
     ds.close()
     ctx.close()
+
+
+def host_cpu():
+    """Core count and model of this host (the CPU baselines run here)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)), "cpu_model": model}
+
+
+def _ref_times(binary, path, q, k):
+    """One run of a reference CPU program: its printed phase times and result count."""
+    out = subprocess.run([binary, "-if", path, "-k", str(k), "-q", str(q)], capture_output=True, text=True,
+                         timeout=600).stdout
+    res = {}
+    for line in out.splitlines():
+        for key, tag in (("tree_build_s", "R*-tree is built in "), ("filter_s", "Filtering time: "),
+                         ("verify_s", "Verification time: "), ("total_s", "Total time: "),
+                         ("search_s", "RkNN search completed in ")):
+            if line.startswith(tag):
+                res[key] = float(line[len(tag):].split("[")[0])
+        if line.startswith("Found "):
+            res["found"] = int(line.split()[1])
+    return res
+
+
+def reference_programs(R, n):
+    """The reference's own CPU programs, built from its sources into
+    oracle/_ref, on this host (one process per query, one thread each, as
+    they are written): rknn-inf on the rknn-inf configuration (1M users, 10k
+    facilities, k=10; its R*-tree build reported apart, as it prints it,
+    rknn-inf/inf.cpp:179,584-601) and rknn-naive on the 10k/1k configuration
+    with integer coordinates (it parses integers, rknn-naive/rknn.cpp:306-328;
+    its search time, :347-352).  The engine's counts on the same files beside
+    them."""
+    import tempfile
+
+    import oracle
+
+    out = {"host": host_cpu(), "threads_per_program": 1}
+    inf, naive = oracle.ref_binary("rknn-inf"), oracle.ref_binary("rknn-naive")
+    if not inf and not naive:
+        out["absent"] = "oracle/_ref holds no reference programs on this host (they are built where the " \
+                        "reference sources are, by oracle/Makefile ref)"
+        return out
+    ctx = R.Context(0)
+    try:
+        with tempfile.TemporaryDirectory() as td:
+            if inf:
+                seed = R.default_seed(2)
+                arrs = R.generate(2, 1_000_000, 10_000, seed=seed)
+                path = os.path.join(td, "config2.txt")
+                R.write_text(path, *arrs)
+                qs = [int(q) for q in R.generate_queries(10_000, n, seed)]
+                runs = [_ref_times(inf, path, q, 10) for q in qs]
+                ds = R.Dataset(ctx, *arrs)
+                ours = [len(ds.query(q, 10)) for q in qs]
+                ds.close()
+                tot = float(np.mean([r["total_s"] for r in runs]))
+                out["rknn_inf"] = {"workload": "1,000,000 users, 10,000 facilities, uniform, k=10", "queries": len(qs),
+                                   "tree_build_s": round(float(np.mean([r["tree_build_s"] for r in runs])), 3),
+                                   "filter_s": round(float(np.mean([r["filter_s"] for r in runs])), 6),
+                                   "verify_s": round(float(np.mean([r["verify_s"] for r in runs])), 6),
+                                   "total_s": round(tot, 6), "qps": round(1.0 / tot, 1),
+                                   "found": [r.get("found") for r in runs], "engine_found": ours}
+                log(f"reference rknn-inf (1 thread): {1.0 / tot:,.1f} queries/s over {len(qs)} queries "
+                    f"(filter + verification; R*-tree build {out['rknn_inf']['tree_build_s']}s apart); "
+                    f"found {out['rknn_inf']['found']}, engine {ours}")
+            if naive:
+                seed = R.default_seed(1)
+                ux, uy, uid, fx, fy, fid = R.generate(1, 10_000, 1_000, seed=seed)
+                ux, uy, fx, fy = (np.round(v).astype(np.float32) for v in (ux, uy, fx, fy))
+                path = os.path.join(td, "config1_int.txt")
+                R.write_text(path, ux, uy, uid, fx, fy, fid)
+                qs = [int(q) for q in R.generate_queries(1_000, n, seed)]
+                runs = [_ref_times(naive, path, q, 5) for q in qs]
+                ds = R.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+                ours = [len(ds.query_bruteforce(q, 5)) for q in qs]
+                ds.close()
+                sec = float(np.mean([r["search_s"] for r in runs]))
+                out["rknn_naive"] = {"workload": "10,000 users, 1,000 facilities, integer coordinates in [0,1000]^2, "
+                                                 "k=5", "queries": len(qs), "search_s": round(sec, 6),
+                                     "qps": round(1.0 / sec, 3), "found": [r.get("found") for r in runs],
+                                     "engine_found": ours}
+                log(f"reference rknn-naive (1 thread): {1.0 / sec:,.3f} queries/s over {len(qs)} queries; "
+                    f"found {out['rknn_naive']['found']}, engine {ours}")
+    finally:
+        ctx.close()
+    return out
 
 
 # configs 1-3: (users, facilities, queries, k values)

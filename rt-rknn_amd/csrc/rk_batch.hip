@@ -3,10 +3,12 @@
 //
 // A step is two launches, one block per query.  k_batch_query: candidate
 // records (binary64 -> fp32), the breadth-first descent of the bounding-
-// rectangle tree through LDS frontiers, the per-tile dominance count, the
-// query's result segment; k_batch_sort: its radix sort.  Blocks share no
-// counter, so queries never wait on each other.  The fetch compacts the segments
-// (offsets scan + copy) outside the graph.
+// rectangle tree through LDS frontiers, the per-tile dominance count, and
+// the query's result ids collected, sorted and written to the batch's
+// landing area in pinned host memory; k_batch_land: the same for results
+// too large for the block's LDS.  Queries share only the landing area's
+// reservation counter (one atomic per query).  When the run is done, its
+// results are in host memory: the fetch is a host copy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -52,21 +54,125 @@ rk_status build_graph(rk_batch* b)
     hipStream_t st = c->stream;
     RK_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     batch_launch(b, st);
-    // the per-query result counts come back inside the graph (pinned memory):
-    // a copy enqueued after the graph starts only after an idle gap of
-    // about 14 us on the device
-    hipError_t e = hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
+    // (the results, counts and landing offsets reach pinned host memory from
+    // the step's last kernel: the graph holds no copy node)
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(st, &g);
         if (g) (void)hipGraphDestroy(g);
-        return fail_hip(c, e, "hipMemcpyAsync (capture)", __LINE__);
+        return fail_hip(c, e, "kernel launch (capture)", __LINE__);
     }
     e = hipStreamEndCapture(st, &b->graph);
     if (e != hipSuccess) return fail_hip(c, e, "hipStreamEndCapture", __LINE__);
     RK_HIP(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     return RK_OK;
 }
+
+// A prepared batch's graph captured again (its landing area or copy size
+// changed).  Caller holds the context lock; no run in flight.
+rk_status recapture(rk_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    return build_graph(b);
+}
+
+// The landing area holds at least `ids` ids (host and device twins, capped
+// at kMaxLandIds); its reservation counters start at zero.  Caller holds the
+// context lock; the batch has no run in flight.
+rk_status ensure_landing(rk_batch* b, uint64_t ids)
+{
+    rk_ctx* c = b->ds->ctx;
+    ids = std::min<uint64_t>(std::max<uint64_t>(ids, 1), kMaxLandIds);
+    if (!b->land_ctr.p) {
+        RK_HIP(c, b->land_ctr.reserve(2));
+        RK_HIP(c, hipMemsetAsync(b->land_ctr.p, 0, 2 * sizeof(unsigned long long), c->stream));
+        RK_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    if (ids <= b->land_cap) return RK_OK;
+    b->land_cap = 0;
+    b->hland_dev = nullptr;
+    // coherent: the kernel's stores reach host memory as they retire, and the
+    // host reads them after the run's event without any cache maintenance
+    RK_HIP(c, b->h_land.reserve(ids, hipHostMallocCoherent));
+    RK_HIP(c, b->dland.reserve(ids));
+    void* dp = nullptr;
+    RK_HIP(c, hipHostGetDevicePointer(&dp, b->h_land.p, 0));
+    b->hland_dev = static_cast<int32_t*>(dp);
+    b->land_cap = std::min<uint64_t>(std::min<uint64_t>(b->h_land.n, b->dland.n), kMaxLandIds);
+    return RK_OK;
+}
+
+// Room for the counts and landing offsets of nq queries in pinned host
+// memory (written by the step's last kernel).  Caller holds the context lock.
+rk_status ensure_qres(rk_batch* b, uint64_t nq)
+{
+    rk_ctx* c = b->ds->ctx;
+    const uint64_t want = std::max<uint64_t>(2 * nq, 2);
+    if (want <= b->h_qres.n && b->hqres_dev) return RK_OK;
+    b->hqres_dev = nullptr;
+    RK_HIP(c, b->h_qres.reserve(want, hipHostMallocCoherent));
+    void* dp = nullptr;
+    RK_HIP(c, hipHostGetDevicePointer(&dp, b->h_qres.p, 0));
+    b->hqres_dev = static_cast<uint32_t*>(dp);
+    return RK_OK;
+}
+
+// The landed extent of a finished run: every query that fit its segment has
+// its ids at [qland, qland + count), densely packed from 0.  *short_area:
+// some such query found the landing area full; *need: ids those queries
+// need in all.
+uint64_t landed_extent(const rk_batch* b, bool* short_area, uint64_t* need)
+{
+    uint64_t ext = 0, nd = 0;
+    bool sh = false;
+    const uint32_t* cnt = b->h_qcount();
+    const uint32_t* land = b->h_qland();
+    for (uint32_t i = 0; i < b->nq; ++i) {
+        if (b->hq[i].trivial || cnt[i] > b->view.seg_cap || !cnt[i]) continue;
+        nd += cnt[i];
+        if (land[i] == kNotLanded)
+            sh = true;
+        else
+            ext = std::max<uint64_t>(ext, (uint64_t)land[i] + cnt[i]);
+    }
+    if (short_area) *short_area = sh;
+    if (need) *need = nd;
+    return ext;
+}
+
+// The kernels' view of a filled batch.
+void set_view(rk_batch* b, uint64_t seg_cap)
+{
+    rk_dataset* ds = b->ds;
+    BatchDev& v = b->view;
+    v.q = b->q.p;
+    v.nq = b->nq;
+    v.seg_cap = (uint32_t)seg_cap;
+    v.cand = b->cand.p;
+    v.fx = ds->fx.p;
+    v.fy = ds->fy.p;
+    v.wbound = b->wb.p;
+    v.rec = b->rec.p;
+    v.orig = b->orig.p;
+    v.seg = b->seg.p;
+    v.qcount = b->qres.p;
+    v.qland = b->qres.p + b->nq;
+    v.qstat = b->qstat.p;
+    v.dland = b->dland.p;
+    v.hland = b->hland_dev;
+    v.hqres = b->hqres_dev;
+    v.land_cap = b->land_cap;
+    v.land_ctr = b->land_ctr.p;
+    v.id_bits = ds->id_bits;
+}
+
+// First landing area of a batch: room for 2,048 ids per query (a run that
+// needs more grows it and is repeated, rk_batch_wait).
+uint64_t initial_landing(uint64_t nq) { return std::max<uint64_t>(nq * 2048, 1u << 16); }
 
 uint32_t pow2_ceil(uint64_t v)
 {
@@ -196,37 +302,24 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
     RK_HIP(c, b->cand.reserve(all.size()));
     RK_HIP(c, b->rec.reserve(all.size()));
     RK_HIP(c, b->orig.reserve(all.size()));
-    RK_HIP(c, b->qcount.reserve(nq));
+    RK_HIP(c, b->qres.reserve(2 * (uint64_t)nq));
     RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
-    RK_HIP(c, b->qoff.reserve(nq + 1));
-    RK_HIP(c, b->h_qcount.reserve(nq));
     // result segment per query: the whole budget split evenly, never more than the users
     uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? pool_budget() / nq : 1));
     seg_cap = std::max<uint64_t>(seg_cap, std::min<uint64_t>(ds->nu, b->seg_min));
     seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
     RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
+    {
+        rk_status ls = ensure_landing(b, initial_landing(nq));
+        if (ls == RK_OK) ls = ensure_qres(b, nq);
+        if (ls != RK_OK) return ls;
+    }
     if (nq) RK_HIP(c, hipMemcpyAsync(b->q.p, b->hq.data(), nq * sizeof(QDev), hipMemcpyHostToDevice, st));
     if (nq) RK_HIP(c, hipMemcpyAsync(b->wb.p, wb.data(), nq * sizeof(double2), hipMemcpyHostToDevice, st));
     if (!all.empty())
         RK_HIP(c, hipMemcpyAsync(b->cand.p, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     RK_HIP(c, hipStreamSynchronize(st));
-
-    BatchDev& v = b->view;
-    v.q = b->q.p;
-    v.nq = b->nq;
-    v.seg_cap = (uint32_t)seg_cap;
-    v.cand = b->cand.p;
-    v.fx = ds->fx.p;
-    v.fy = ds->fy.p;
-    v.wbound = b->wb.p;
-    v.rec = b->rec.p;
-    v.orig = b->orig.p;
-    v.seg = b->seg.p;
-    v.qcount = b->qcount.p;
-    v.qstat = b->qstat.p;
-    v.qoff = b->qoff.p;
-    v.out = b->out.p;
-    v.id_bits = ds->id_bits;
+    set_view(b, seg_cap);
     return RK_OK;
 }
 
@@ -259,6 +352,7 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
     b->total_cand = 0;
     b->hq.assign(nq, QDev{});
     b->h_dn.assign(nq, 0);
+
     // candidate segment per query: every facility at most, 4096 slots at most
     b->cand_cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ds->nf, dfilter_slots()));
     const uint64_t slots = std::max<uint64_t>(1, (uint64_t)nq * b->cand_cap);
@@ -274,36 +368,23 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
     RK_HIP(c, b->cand.reserve(slots));
     RK_HIP(c, b->rec.reserve(slots));
     RK_HIP(c, b->orig.reserve(slots));
-    RK_HIP(c, b->qcount.reserve(nq));
+    RK_HIP(c, b->qres.reserve(2 * (uint64_t)nq));
     RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
-    RK_HIP(c, b->qoff.reserve(nq + 1));
-    RK_HIP(c, b->h_qcount.reserve(nq));
     uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? pool_budget() / nq : 1));
     seg_cap = std::max<uint64_t>(seg_cap, std::min<uint64_t>(ds->nu, b->seg_min));
     seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
     RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
+    {
+        rk_status ls = ensure_landing(b, initial_landing(nq));
+        if (ls == RK_OK) ls = ensure_qres(b, nq);
+        if (ls != RK_OK) return ls;
+    }
     if (nq) {
         RK_HIP(c, hipMemcpyAsync(b->dq.p, q, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
         RK_HIP(c, hipMemcpyAsync(b->dk.p, k, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     }
     RK_HIP(c, hipStreamSynchronize(st));
-
-    BatchDev& v = b->view;
-    v.q = b->q.p;
-    v.nq = b->nq;
-    v.seg_cap = (uint32_t)seg_cap;
-    v.cand = b->cand.p;
-    v.fx = ds->fx.p;
-    v.fy = ds->fy.p;
-    v.wbound = b->wb.p;
-    v.rec = b->rec.p;
-    v.orig = b->orig.p;
-    v.seg = b->seg.p;
-    v.qcount = b->qcount.p;
-    v.qstat = b->qstat.p;
-    v.qoff = b->qoff.p;
-    v.out = b->out.p;
-    v.id_bits = ds->id_bits;
+    set_view(b, seg_cap);
     return RK_OK;
 }
 
@@ -328,25 +409,79 @@ rk_status batch_download_views(rk_batch* b)
 
 rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets, std::vector<uint32_t>* skip);
 
-// memcpy of n ids, split over a few host threads when large (one thread
-// copies ~10 GB/s out of pinned memory; a step's results are megabytes)
-void parallel_copy(int32_t* dst, const int32_t* src, uint64_t n)
+// One run of a filled batch on its own stream: events around the device
+// work, the kernels (or the captured graph), the copy of the counts and
+// landing offsets (inside the graph, or after the kernels), and the done
+// event.  Caller holds the context lock.  On failure whatever reached the
+// stream is finished before returning.
+rk_status run_on_stream(rk_batch* b)
 {
-    constexpr uint64_t kPerThread = 1ull << 20; // ids
-    const unsigned t = (unsigned)std::min<uint64_t>(8, n / kPerThread);
+    rk_ctx* c = b->ds->ctx;
+    const bool direct = !b->exec;
+    // each object is created on its own, so one that failed is retried
+    if (!b->ev_start) RK_HIP(c, hipEventCreate(&b->ev_start));
+    if (!b->ev_end) RK_HIP(c, hipEventCreate(&b->ev_end));
+    if (!b->ev_done) RK_HIP(c, hipEventCreate(&b->ev_done));
+    if (!b->stream) RK_HIP(c, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)); // default priority
+    // The fills finished on the context stream before they returned, and
+    // fetch runs only after the wait, so the batch's stream needs no ordering
+    // against the context stream.
+    hipStream_t st = b->stream;
+    hipError_t e = hipEventRecord(b->ev_start, st);
+    const char* what = "hipEventRecord";
+    if (e == hipSuccess) {
+        if (direct) {
+            batch_launch(b, st);
+            e = hipGetLastError();
+            what = "kernel launch";
+        } else {
+            e = hipGraphLaunch(b->exec, st);
+            what = "hipGraphLaunch";
+        }
+    }
+    if (e == hipSuccess) e = hipEventRecord(b->ev_end, st);
+    if (e == hipSuccess) e = hipEventRecord(b->ev_done, st);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        return fail_hip(c, e, what, __LINE__);
+    }
+    return RK_OK;
+}
+
+// Copies of (dst, src, n) id ranges, split over a few host threads when
+// large (one thread copies ~10 GB/s; a step's results are megabytes).
+struct IdRange {
+    int32_t* dst;
+    const int32_t* src;
+    uint64_t n;
+};
+
+void copy_ranges(const std::vector<IdRange>& r, uint64_t total)
+{
+    constexpr uint64_t kPerThread = 1ull << 18; // ids
+    const unsigned t = (unsigned)std::min<uint64_t>(8, total / kPerThread);
     if (t < 2) {
-        std::memcpy(dst, src, n * sizeof(int32_t));
+        for (const IdRange& x : r) std::memcpy(x.dst, x.src, x.n * sizeof(int32_t));
         return;
     }
+    // thread k copies the ids [k, k+1) * total / t of the concatenated ranges
+    auto part = [&](unsigned k) {
+        const uint64_t lo = total * k / t, hi = total * (k + 1) / t;
+        uint64_t at = 0;
+        for (const IdRange& x : r) {
+            const uint64_t a = std::max(lo, at), e = std::min(hi, at + x.n);
+            if (a < e) std::memcpy(x.dst + (a - at), x.src + (a - at), (e - a) * sizeof(int32_t));
+            at += x.n;
+            if (at >= hi) break;
+        }
+    };
     std::vector<std::thread> pool;
-    const uint64_t per = (n + t - 1) / t;
-    for (unsigned k = 1; k < t; ++k) {
-        const uint64_t a = k * per, e = std::min(n, a + per);
-        if (a < e) pool.emplace_back([=] { std::memcpy(dst + a, src + a, (e - a) * sizeof(int32_t)); });
-    }
-    std::memcpy(dst, src, std::min(n, per) * sizeof(int32_t));
+    for (unsigned k = 1; k < t; ++k) pool.emplace_back(part, k);
+    part(0);
     for (std::thread& th : pool) th.join();
 }
+
+rk_status result_ranges(rk_batch* b, std::vector<IdRange>& ranges, std::vector<uint32_t>* skip);
 
 } // namespace rk
 
@@ -415,38 +550,8 @@ rk_status rk_batch_enqueue(rk_batch* b)
         b->pending_empty = true;
         return RK_OK;
     }
-    // each object is created on its own, so one that failed is retried
-    if (!b->ev_start) RK_HIP(c, hipEventCreate(&b->ev_start));
-    if (!b->ev_end) RK_HIP(c, hipEventCreate(&b->ev_end));
-    if (!b->ev_done) RK_HIP(c, hipEventCreate(&b->ev_done));
-    if (!b->stream) RK_HIP(c, hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)); // default priority
-    // The fills finished on the context stream before they returned, and
-    // fetch runs only after the wait, so the batch's stream needs no ordering
-    // against the context stream.
-    hipStream_t st = b->stream;
-    hipError_t e = hipEventRecord(b->ev_start, st);
-    const char* what = "hipEventRecord";
-    if (e == hipSuccess) {
-        if (direct) {
-            batch_launch(b, st);
-            e = hipGetLastError();
-            what = "kernel launch";
-        } else {
-            e = hipGraphLaunch(b->exec, st);
-            what = "hipGraphLaunch";
-        }
-    }
-    if (e == hipSuccess) e = hipEventRecord(b->ev_end, st);
-    if (e == hipSuccess && direct) { // a graph copies the counts itself
-        e = hipMemcpyAsync(b->h_qcount.p, b->qcount.p, b->nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st);
-        what = "hipMemcpyAsync (counts)";
-    }
-    if (e == hipSuccess) e = hipEventRecord(b->ev_done, st);
-    if (e != hipSuccess) {
-        // whatever reached the stream finishes before the buffers are refilled
-        (void)hipStreamSynchronize(st);
-        return fail_hip(c, e, what, __LINE__);
-    }
+    const rk_status s = rk::run_on_stream(b);
+    if (s != RK_OK) return s;
     b->pending = true;
     b->pending_empty = false;
     return RK_OK;
@@ -473,15 +578,39 @@ rk_status rk_batch_wait(rk_batch* b, double* device_seconds)
     RK_HIP(c, hipSetDevice(c->device));
     float ms = 0.f;
     RK_HIP(c, hipEventElapsedTime(&ms, b->ev_start, b->ev_end));
-    if (device_seconds) *device_seconds = ms * 1e-3;
-    if (b->dfilter) {
-        const rk_status s = batch_download_views(b);
+    double dev_s = ms * 1e-3;
+    for (int attempt = 0;; ++attempt) {
+        if (b->dfilter) {
+            const rk_status s = batch_download_views(b);
+            if (s != RK_OK) return s;
+        }
+        // every query that fit its segment has its sorted ids in the landing
+        // area, unless the area was too small: then it grows to 1.5x what the
+        // run needed and the run is repeated (once; a repeat on an area of
+        // the exact size cannot fall short)
+        uint64_t need = 0;
+        bool short_area = false;
+        (void)landed_extent(b, &short_area, &need);
+        if (!short_area) break;
+        if (attempt > 0) return fail(c, RK_ERR_INTERNAL, "a repeated run fell short of its landing area");
+        rk_status s = ensure_landing(b, need + need / 2);
         if (s != RK_OK) return s;
+        set_view(b, b->view.seg_cap);
+        if (b->exec) { // the graph holds the old area's addresses: capture it again
+            s = recapture(b);
+            if (s != RK_OK) return s;
+        }
+        s = run_on_stream(b);
+        if (s != RK_OK) return s;
+        RK_HIP(c, hipEventSynchronize(b->ev_done));
+        RK_HIP(c, hipEventElapsedTime(&ms, b->ev_start, b->ev_end));
+        dev_s += ms * 1e-3;
     }
-    // both checks before either is reported: fetch refuses a run with
-    // either kind of overflow (a truncated candidate list gives too many
-    // results, a truncated segment too few)
-    for (uint32_t i = 0; i < b->nq; ++i) b->overflow |= b->h_qcount.p[i] > b->view.seg_cap;
+    if (device_seconds) *device_seconds = dev_s;
+    // both overflow checks before either is reported: fetch refuses a run
+    // with either kind (a truncated candidate list gives too many results, a
+    // truncated segment too few)
+    for (uint32_t i = 0; i < b->nq; ++i) b->overflow |= b->h_qcount()[i] > b->view.seg_cap;
     b->ran = true;
     if (b->cand_overflow)
         return fail(c, RK_ERR_CAPACITY, "a query's device candidate list exceeds its segment; filter on the host");
@@ -502,86 +631,83 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
     return rk::fetch_impl(b, out_ids, cap, offsets, nullptr);
 }
 
+rk_status rk_batch_results(rk_batch* b, const int32_t** ids, uint64_t* n)
+{
+    if (!b || (b->nq && (!ids || !n))) return RK_ERR_INVALID_ARG;
+    rk_ctx* c = b->ds->ctx;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    std::vector<IdRange> ranges;
+    const rk_status s = rk::result_ranges(b, ranges, nullptr);
+    if (s != RK_OK) return s;
+    for (uint32_t i = 0; i < b->nq; ++i) {
+        ids[i] = ranges[i].src;
+        n[i] = ranges[i].n;
+    }
+    return RK_OK;
+}
+
 } // extern "C"
 
 namespace rk {
 
-// The results of the last run.  With skip == nullptr every query must have
-// fit its segments; otherwise the queries that did not get empty ranges and
-// are listed in *skip.
-rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets, std::vector<uint32_t>* skip)
+// Where the last run's results are: per query a range of the landing area
+// (or of the dataset's sorted id list, for queries every user qualifies
+// for).  With skip == nullptr every query must have fit its segments;
+// otherwise the queries that did not get empty ranges and are listed in
+// *skip.  Caller holds the context lock.
+rk_status result_ranges(rk_batch* b, std::vector<IdRange>& ranges, std::vector<uint32_t>* skip)
 {
     rk_dataset* ds = b->ds;
     rk_ctx* c = ds->ctx;
     if (!b->ran) return fail(c, RK_ERR_INVALID_ARG, "rk_batch_run has not been called");
     if (b->pending) return fail(c, RK_ERR_INVALID_ARG, "rk_batch_wait has not been called");
-    std::lock_guard<std::recursive_mutex> lk(c->mu);
-    RK_HIP(c, hipSetDevice(c->device));
     if (b->cand_overflow && !skip)
         return fail(c, RK_ERR_CAPACITY, "the last run overflowed a device candidate segment; filter on the host");
     if (b->overflow && !skip)
         return fail(c, RK_ERR_CAPACITY, "the last run overflowed a result segment; split the batch");
     const uint32_t nq = b->nq;
-    std::vector<uint8_t> out_of_seg(nq, 0);
-    if (skip) {
-        skip->clear();
-        for (uint32_t i = 0; i < nq; ++i) {
-            const bool lost = b->h_qcount.p[i] > b->view.seg_cap || (b->dfilter && b->h_dn[i] > b->cand_cap);
-            out_of_seg[i] = lost ? 1 : 0;
-            if (lost) skip->push_back(i);
+    const uint32_t* cnt = b->h_qcount();
+    const uint32_t* land = b->h_qland();
+    if (skip) skip->clear();
+    ranges.assign(nq, IdRange{nullptr, nullptr, 0});
+    for (uint32_t i = 0; i < nq; ++i) {
+        const QDev& h = b->hq[i];
+        // a truncated device candidate list first: its view may say trivial
+        if (cnt[i] > b->view.seg_cap || (b->dfilter && b->h_dn[i] > b->cand_cap)) {
+            skip->push_back(i); // (without skip the overflow was refused above)
+            continue;
         }
+        if (h.trivial) {
+            if (h.k > h.nc) ranges[i] = {nullptr, ds->all_ids_sorted.data(), ds->nu};
+            continue;
+        }
+        if (!cnt[i]) continue;
+        if (land[i] == kNotLanded || (uint64_t)land[i] + cnt[i] > b->land_cap)
+            return fail(c, RK_ERR_INTERNAL, "a result is missing from the landing area");
+        ranges[i] = {nullptr, b->h_land.p + land[i], cnt[i]};
     }
-    std::vector<uint64_t> qoff(nq + 1, 0);
-    const bool dev_ran = b->exec || (b->direct && ds->nu && nq);
-    if (dev_ran) {
-        launch_batch_scan(b->view, c->stream);
-        RK_HIP(c, hipGetLastError());
-        // on the context's (non-blocking) stream: a null-stream copy would not
-        // wait for the scan
-        RK_HIP(c, hipMemcpyAsync(qoff.data(), b->qoff.p, (nq + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                 c->stream));
-        RK_HIP(c, hipStreamSynchronize(c->stream));
-        for (uint32_t i = 0; i < nq; ++i)
-            if (qoff[i + 1] - qoff[i] != std::min(b->h_qcount.p[i], b->view.seg_cap))
-                return fail(c, RK_ERR_INTERNAL, "result offsets disagree with the per-query counts");
-    }
+    return RK_OK;
+}
+
+// The results of the last run, copied from the landing area (see
+// result_ranges for skip).
+rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offsets, std::vector<uint32_t>* skip)
+{
+    rk_ctx* c = b->ds->ctx;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    std::vector<IdRange> ranges;
+    const rk_status s = result_ranges(b, ranges, skip);
+    if (s != RK_OK) return s;
+    const uint32_t nq = b->nq;
     uint64_t total = 0;
     for (uint32_t i = 0; i < nq; ++i) {
         offsets[i] = total;
-        if (out_of_seg[i]) continue;
-        total += b->hq[i].trivial ? (b->hq[i].k > b->hq[i].nc ? ds->nu : 0) : (qoff[i + 1] - qoff[i]);
+        ranges[i].dst = out_ids + total;
+        total += ranges[i].n;
     }
     offsets[nq] = total;
     if (total > cap) return fail(c, RK_ERR_CAPACITY, "output buffer too small; offsets[nq] holds the size needed");
-    const int32_t* dev = nullptr;
-    if (dev_ran && qoff[nq]) {
-        RK_HIP(c, b->out.reserve(qoff[nq]));
-        b->view.out = b->out.p; // compaction is outside the graph, so the pointer may change
-        launch_batch_compact(b->view, c->stream);
-        RK_HIP(c, hipGetLastError());
-        RK_HIP(c, b->h_out.reserve(qoff[nq])); // pinned: a direct DMA, no zero-filled staging vector
-        RK_HIP(c, hipMemcpyAsync(b->h_out.p, b->out.p, qoff[nq] * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                 c->stream));
-        RK_HIP(c, hipStreamSynchronize(c->stream));
-        dev = b->h_out.p;
-    }
-    // the compacted ids in runs: consecutive queries whose places in the
-    // output and in the compacted array advance together are one copy
-    for (uint32_t i = 0; i < nq;) {
-        const uint64_t n = offsets[i + 1] - offsets[i];
-        if (b->hq[i].trivial) {
-            if (n) std::memcpy(out_ids + offsets[i], ds->all_ids_sorted.data(), n * sizeof(int32_t));
-            ++i;
-            continue;
-        }
-        uint32_t j = i + 1;
-        while (j < nq && !b->hq[j].trivial && !out_of_seg[j] && !out_of_seg[j - 1] &&
-               offsets[j + 1] - offsets[j] == qoff[j + 1] - qoff[j])
-            ++j;
-        if (!out_of_seg[i] && offsets[j] > offsets[i]) // sorted on the device
-            parallel_copy(out_ids + offsets[i], dev + qoff[i], offsets[j] - offsets[i]);
-        i = j;
-    }
+    copy_ranges(ranges, total);
     return RK_OK;
 }
 
@@ -612,7 +738,7 @@ rk_status batch_collect(rk_batch* b, rk_batch* r, const rk_query_desc* qd, size_
     for (size_t j = 0; j < redo.size(); ++j) {
         const uint32_t i = redo[j];
         rq[j] = qd[i];
-        need = std::max<uint64_t>(need, b->h_qcount.p[i]);
+        need = std::max<uint64_t>(need, b->h_qcount()[i]);
         host |= dfilter && b->h_dn[i] > b->cand_cap;
     }
     if (host) // candidate lists from the host filter for every re-run query without one
@@ -690,7 +816,7 @@ rk_status rk_batch_stats(const rk_batch* b, uint64_t* items, size_t n_items_cap,
         const uint32_t* s = st.data() + (size_t)i * kQueryStats;
         rech += s[0];
         for (size_t l = 0; l < nl; ++l) lv[l] += s[1 + l];
-        res += b->h_qcount.p[i];
+        res += b->h_qcount()[i];
     }
     for (size_t l = 0; l < nl && l < n_items_cap && items; ++l) items[l] = lv[l];
     if (results) *results = res;

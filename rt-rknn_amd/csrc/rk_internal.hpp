@@ -72,7 +72,7 @@ struct HostBuf {
         if (p) (void)hipHostFree(p);
     }
     // as DevBuf::reserve: contents not kept, a regrowth takes at least 1.5x
-    hipError_t reserve(size_t want)
+    hipError_t reserve(size_t want, unsigned flags = hipHostMallocDefault)
     {
         if (want <= n && p) return hipSuccess;
         if (p) want = std::max(want, n + n / 2);
@@ -80,7 +80,7 @@ struct HostBuf {
         p = nullptr;
         n = 0;
         if (want == 0) want = 1;
-        hipError_t e = hipHostMalloc((void**)&p, want * sizeof(T), hipHostMallocDefault);
+        hipError_t e = hipHostMalloc((void**)&p, want * sizeof(T), flags);
         if (e != hipSuccess) {
             p = nullptr;
             return e;
@@ -156,14 +156,23 @@ struct rk_batch {
     rk::DevBuf<double2> wb;
     rk::DevBuf<rk::CandRec> rec;
     rk::DevBuf<float2> orig;
-    rk::DevBuf<int32_t> seg;     // per-query result segments
-    rk::DevBuf<uint32_t> qcount;
+    rk::DevBuf<int32_t> seg;     // per-query result segments (spill of results beyond the kernel's LDS)
+    rk::DevBuf<uint32_t> qres;   // [0, nq): results per query, [nq, 2 nq): landing offsets
     rk::DevBuf<uint32_t> qstat;
-    rk::DevBuf<uint64_t> qoff;
-    rk::DevBuf<int32_t> out;
-    rk::HostBuf<uint32_t> h_qcount;
-    rk::HostBuf<int32_t> h_out;  // pinned staging of the compacted results (fetch)
+    // landing area: the kernels sort each query's ids into dland (HBM) at
+    // the offset qres[nq + i], and the run's last kernel writes them to
+    // h_land (pinned, coherent host memory), with the counts and offsets to
+    // h_qres; land_ctr: the run's reservation counters
+    rk::HostBuf<uint32_t> h_qres;
+    rk::HostBuf<int32_t> h_land;
+    uint32_t* hqres_dev = nullptr; // h_qres and h_land as the device addresses them
+    int32_t* hland_dev = nullptr;
+    rk::DevBuf<int32_t> dland;
+    rk::DevBuf<unsigned long long> land_ctr;
+    uint64_t land_cap = 0;
     rk::BatchDev view{};
+    uint32_t* h_qcount() const { return h_qres.p; }
+    uint32_t* h_qland() const { return h_qres.p + nq; }
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool ran = false;

@@ -82,6 +82,13 @@ struct SharedRecs {
     float2 orig[N];
 };
 
+// The batched form's shared records: the facility index in each record's
+// spare word instead of an orig[] array (8 bytes of LDS less per record).
+template <uint32_t N>
+struct SharedRecsIdx {
+    CandRec rec[N];
+};
+
 // One query against one tile of 64 users.  Phase A puts candidates on lanes
 // and classifies each against the tile's bounding rectangle m: certainly
 // covering the whole tile (counted once for every user), certainly missing
@@ -95,11 +102,16 @@ struct SharedRecs {
 //   kPre:    the users' coordinates are given (pux, puy); otherwise they are
 //            loaded on the first crossing candidate, so culled tiles cost no
 //            user traffic
-template <bool kShared, bool kPre>
+//   kIdx:    (with kShared) the records carry their facility index in the
+//            spare word, and a band test reads the facility's coordinates
+//            from fxg/fyg instead of orig[]
+template <bool kShared, bool kPre, bool kIdx = false>
 __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, uint32_t nc,
                              const CandRec* __restrict__ rec, const float2* __restrict__ orig, uint32_t tile,
-                             float4 m, float pux, float puy, WaveLds* L)
+                             float4 m, float pux, float puy, WaveLds* L, const float* __restrict__ fxg = nullptr,
+                             const float* __restrict__ fyg = nullptr)
 {
+    static_assert(!kIdx || kShared, "indexed records are the shared-LDS layout");
     const uint32_t lane = lane_id();
     k = uniform(k);
     nc = uniform(nc);
@@ -146,7 +158,7 @@ __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, ui
                 loaded = true;
             }
             const CandRec* cr = rec + c0;
-            const float2* co = orig + c0;
+            const float2* co = kIdx ? nullptr : orig + c0;
             if (!kShared) {
                 L->rec[lane] = r;
                 L->orig[lane] = o;
@@ -183,7 +195,13 @@ __device__ LeafOut leaf_tile(const DsDev& ds, float qx, float qy, uint32_t k, ui
                     for (int g = 0; g < 4; ++g) {
                         const uint64_t bm = ballot(band[g]);
                         if (bm) {
-                            const float2 f = co[b[g]];
+                            float2 f;
+                            if (kIdx) {
+                                const uint32_t fi = __float_as_uint(cr[b[g]].pad);
+                                f = make_float2(fxg[fi], fyg[fi]);
+                            } else {
+                                f = co[b[g]];
+                            }
                             if (band[g] && d2_exact(ux, uy, f.x, f.y) < dq) ++cnt;
                             fp64 += (uint32_t)__popcll(bm & ballot(valid));
                         }
@@ -656,15 +674,20 @@ struct QueryCtl {
     uint32_t qcur[4];           // placement cursors per quadrant
     uint32_t quad[5];           // quadrant q holds records [quad[q], quad[q+1])
     uint32_t red[2 * kQWaves + 1]; // the segment sort's reductions and skew flag
-    uint32_t pad[2];
+    uint32_t land;              // the query's landing offset (block broadcast)
+    uint32_t pad[1];
 };
 constexpr uint32_t kCtlWords = sizeof(QueryCtl) / 4;
 constexpr size_t kCtlBytes = (sizeof(QueryCtl) + 15) / 16 * 16;
 constexpr size_t kCoverBytes = kPhaseItems * 64 * sizeof(uint32_t);
-// candidate area: the shared records, or one streaming slot per wave
-constexpr size_t kCandBytes = sizeof(SharedRecs<kRecLds>) > kQWaves * sizeof(WaveLds)
-                                  ? sizeof(SharedRecs<kRecLds>)
+// candidate area: the shared records, or one streaming slot per wave; after
+// the descent it holds the in-block sort of the result ids
+constexpr size_t kCandBytes = sizeof(SharedRecsIdx<kRecLds>) > kQWaves * sizeof(WaveLds)
+                                  ? sizeof(SharedRecsIdx<kRecLds>)
                                   : kQWaves * sizeof(WaveLds);
+constexpr size_t kResBytes = kResLds * sizeof(uint32_t); // the query's result ids
+constexpr uint32_t kResBuckets = 1024;                    // buckets of the in-block sort
+static_assert((kResLds + kResBuckets) * sizeof(uint32_t) <= kCandBytes, "in-block sort storage");
 
 // Sign quadrant of a record: bit 0 gx < 0, bit 1 gy < 0 (the complement of
 // the corner choice gx >= 0 ? low : high of the node test).
@@ -759,10 +782,13 @@ __device__ __forceinline__ uint32_t* frontier(uint32_t* front, uint32_t l) { ret
 
 // The descent of one query (all threads of the block call it).  The first
 // phase expands a virtual root whose children are the top-level nodes.
+// Kept users go to the block's LDS result buffer res[] (the first kResLds of
+// them) and past it to the query's segment in HBM at the same positions.
+// Shared records (kShared) carry their facility index for the band tests.
 template <uint32_t kW, bool kShared>
 __device__ void descend(const DsDev& ds, const BatchDev& b, const QDev& Q, uint32_t qi, QueryCtl& ctl,
                         uint32_t* front, uint32_t* cover, const CandRec* __restrict__ rec,
-                        const float2* __restrict__ orig, WaveLds* L)
+                        const float2* __restrict__ orig, WaveLds* L, uint32_t* res)
 {
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
     const uint32_t top = uniform(ds.n_levels - 1);
@@ -808,7 +834,8 @@ __device__ void descend(const DsDev& ds, const BatchDev& b, const QDev& Q, uint3
                     uxn = ds.ux[tn * kTile + lane];
                     uyn = ds.uy[tn * kTile + lane];
                 }
-                const LeafOut r = leaf_tile<kShared, true>(ds, Q.qx, Q.qy, Q.k, Q.nc, rec, orig, tile, m, ux, uy, L);
+                const LeafOut r = leaf_tile<kShared, true, kShared>(ds, Q.qx, Q.qy, Q.k, Q.nc, rec, orig, tile, m, ux,
+                                                                    uy, L, b.fx, b.fy);
                 fp64 += r.fp64;
                 if (r.keep) {
                     uint32_t pos = 0;
@@ -816,7 +843,11 @@ __device__ void descend(const DsDev& ds, const BatchDev& b, const QDev& Q, uint3
                     pos = uniform(pos);
                     if ((r.keep >> lane) & 1ull) {
                         const uint32_t rk = pos + rank_below(r.keep);
-                        if (rk < b.seg_cap) seg[rk] = ds.uid[tile * kTile + lane];
+                        const int32_t id = ds.uid[tile * kTile + lane];
+                        if (rk < kResLds)
+                            res[rk] = (uint32_t)id;
+                        else if (rk < b.seg_cap)
+                            seg[rk] = id;
                     }
                 }
                 tile = tn;
@@ -919,23 +950,25 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v)
     return (uint32_t)__builtin_amdgcn_readlane((int)p, 63);
 }
 
-// Sorts seg[0, n), n <= kThreads * kItems, in place.  LDS: out[kThreads *
-// kItems], cnt[kNB], red[2 * kThreads / 64 + 1].  All threads call it.
+// Sorts n distinct keys (unsigned order) held in registers -- key[i] of
+// thread t is item t + i * kThreads; items >= n are ignored -- into out[0, n)
+// in LDS.  LDS: out[n], cnt[kNB], red[2 * kThreads / 64 + 1].  All threads
+// call it.  Returns false, out[] then undefined, when a bucket held more than
+// kMaxBucket keys (ids bunched very unevenly in their range).
 template <int kThreads, int kItems, uint32_t kNB>
-__device__ bool bucket_sort(int32_t* __restrict__ seg, uint32_t n, uint32_t* out, uint32_t* cnt, uint32_t* red)
+__device__ bool bucket_sort_regs(const uint32_t (&key)[kItems], uint32_t n, uint32_t* out, uint32_t* cnt,
+                                 uint32_t* red)
 {
     constexpr uint32_t kW = kThreads / 64;
     constexpr uint32_t kPer = kNB / kThreads;
     static_assert(kPer * kThreads == kNB, "buckets per thread");
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
-    uint32_t key[kItems], pos[kItems];
+    uint32_t pos[kItems];
     uint32_t lo = 0xffffffffu, hi = 0;
 #pragma unroll
-    for (int i = 0; i < kItems; ++i) { // keys mapped to unsigned order
-        const uint32_t j = tid + i * kThreads;
-        key[i] = j < n ? ((uint32_t)seg[j] ^ 0x80000000u) : 0u;
+    for (int i = 0; i < kItems; ++i) {
         pos[i] = 0;
-        if (j < n) {
+        if (tid + i * kThreads < n) {
             lo = min(lo, key[i]);
             hi = max(hi, key[i]);
         }
@@ -1003,55 +1036,40 @@ __device__ bool bucket_sort(int32_t* __restrict__ seg, uint32_t n, uint32_t* out
         }
     }
     __syncthreads();
-    if (red[2 * kW]) return false;
-#pragma unroll
-    for (int i = 0; i < kItems; ++i) {
-        const uint32_t j = tid + i * kThreads;
-        if (j < n) seg[j] = (int32_t)(out[j] ^ 0x80000000u);
-    }
-    return true;
+    return red[2 * kW] == 0;
 }
 
-// Segments of up to 8 ids per thread (kQuerySortMax for the largest block)
-// are sorted by the block that wrote them; its candidate area (free once the
-// descent ends) holds the keys and the bucket counts.
-constexpr uint32_t kQuerySortMax = kQueryBlock * 8;
-constexpr uint32_t kQuerySortBuckets = 2048;
-static_assert((kQuerySortMax + kQuerySortBuckets) * sizeof(uint32_t) <= kCandBytes, "sort storage");
-
-// Radix sort fallback (rocPRIM's block radix sort via hipCUB, keys in
-// registers, only the bits below the largest id).
-template <class Sorter, int kItems>
-__device__ void sort_segment(int32_t* seg, uint32_t n, uint32_t id_bits, typename Sorter::TempStorage& tmp)
+// Takes n ids of the landing area for one query: the offset, or kNotLanded
+// when the area cannot hold them (the run is then repeated with a larger
+// area).
+__device__ __forceinline__ uint32_t reserve_landing(const BatchDev& b, uint32_t n)
 {
-    int32_t keys[kItems];
-    const uint32_t base = threadIdx.x * kItems;
-#pragma unroll
-    for (int i = 0; i < kItems; ++i) keys[i] = base + i < n ? seg[base + i] : INT32_MAX;
-    Sorter(tmp).Sort(keys, 0, (int)id_bits);
-#pragma unroll
-    for (int i = 0; i < kItems; ++i)
-        if (base + i < n) seg[base + i] = keys[i];
+    const unsigned long long s = atomicAdd(&b.land_ctr[0], (unsigned long long)n);
+    return s + n <= b.land_cap ? (uint32_t)s : kNotLanded;
 }
 
 // kW waves per query.  With 8-wave blocks at eight waves per SIMD (<= 64
 // VGPRs), a batch of 1024 queries is resident at once on 256 CUs; 4-wave
 // blocks may use twice the registers.  The autotuner picks the shape per
-// workload.
+// workload.  A result of up to kResLds ids never touches HBM: it is
+// collected in LDS, sorted there and written once, coalesced, to the
+// landing area in host memory.
 template <uint32_t kW>
 __global__ __launch_bounds__(kW * 64) __attribute__((amdgpu_waves_per_eu(kW, 8)))
 void k_batch_query(DsDev ds, BatchDev b)
 {
     constexpr uint32_t kThreads = kW * 64;
-    constexpr uint32_t kSortMax = kThreads * 8; // segment ids sorted in the block
+    constexpr int kResItems = (int)(kResLds / kThreads); // result ids per thread in the in-block sort
+    static_assert(kResItems * kThreads == kResLds, "result buffer per thread");
     extern __shared__ __align__(16) unsigned char smem[];
     QueryCtl& ctl = *reinterpret_cast<QueryCtl*>(smem);
     uint32_t* ctl_words = reinterpret_cast<uint32_t*>(smem);
     uint32_t* cover = reinterpret_cast<uint32_t*>(smem + kCtlBytes);
     unsigned char* cand = smem + kCtlBytes + kCoverBytes;
-    SharedRecs<kRecLds>& S = *reinterpret_cast<SharedRecs<kRecLds>*>(cand);
+    SharedRecsIdx<kRecLds>& S = *reinterpret_cast<SharedRecsIdx<kRecLds>*>(cand);
     WaveLds* wl = reinterpret_cast<WaveLds*>(cand);
-    uint32_t* front = reinterpret_cast<uint32_t*>(cand + kCandBytes);
+    uint32_t* res = reinterpret_cast<uint32_t*>(cand + kCandBytes);
+    uint32_t* front = reinterpret_cast<uint32_t*>(cand + kCandBytes + kResBytes);
     const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
 
     for (uint32_t i = tid; i < kPhaseItems * 64; i += kThreads) cover[i] = 0;
@@ -1060,31 +1078,32 @@ void k_batch_query(DsDev ds, BatchDev b)
         uint32_t* stat = b.qstat + (uint64_t)qi * kQueryStats;
         if (Q.trivial) { // k > |C|: every user qualifies; the host fills the result
             if (tid < kQueryStats) stat[tid] = 0;
-            if (tid == 0) b.qcount[qi] = 0;
+            if (tid == 0) {
+                b.qcount[qi] = 0;
+                b.qland[qi] = 0;
+            }
             continue;
         }
         if (tid < kCtlWords) ctl_words[tid] = 0;
         __syncthreads();
         // candidate records (binary64 -> fp32): into LDS sorted by sign
-        // quadrant when they fit, else into the batch's global record array
+        // quadrant when they fit, each with its facility index in the spare
+        // word, else into the batch's global record array
         const bool shared = Q.nc <= kRecLds;
         const double2 w = b.wbound[qi];
         if (shared) {
             constexpr int kPer = (kRecLds + kThreads - 1) / kThreads; // records per thread
             CandRec r[kPer];
-            float2 o[kPer];
             uint32_t qd[kPer];
 #pragma unroll
             for (int h = 0; h < kPer; ++h) {
                 const uint32_t j = tid + h * kThreads;
                 r[h] = {0.f, 0.f, 0.f, 0.f};
-                o[h] = make_float2(0.f, 0.f);
                 qd[h] = 4;
                 if (j < Q.nc) {
                     const uint32_t f = b.cand[Q.off + j];
-                    const float x = b.fx[f], y = b.fy[f];
-                    r[h] = make_record(Q.qx, Q.qy, x, y, w.x, w.y);
-                    o[h] = make_float2(x, y);
+                    r[h] = make_record(Q.qx, Q.qy, b.fx[f], b.fy[f], w.x, w.y);
+                    r[h].pad = __uint_as_float(f);
                     qd[h] = quadrant(r[h]);
                 }
             }
@@ -1107,11 +1126,7 @@ void k_batch_query(DsDev ds, BatchDev b)
             for (int g = 0; g < 4; ++g) start[g + 1] = start[g] + ctl.qcnt[g];
 #pragma unroll
             for (int h = 0; h < kPer; ++h)
-                if (qd[h] < 4) {
-                    const uint32_t slot = start[qd[h]] + rank[h];
-                    S.rec[slot] = r[h];
-                    S.orig[slot] = o[h];
-                }
+                if (qd[h] < 4) S.rec[start[qd[h]] + rank[h]] = r[h];
             if (tid < 5) ctl.quad[tid] = start[tid];
         } else {
             CandRec* grec = b.rec + Q.off;
@@ -1125,9 +1140,9 @@ void k_batch_query(DsDev ds, BatchDev b)
         }
         __syncthreads();
         if (shared)
-            descend<kW, true>(ds, b, Q, qi, ctl, front, cover, S.rec, S.orig, &wl[wave]);
+            descend<kW, true>(ds, b, Q, qi, ctl, front, cover, S.rec, nullptr, &wl[wave], res);
         else
-            descend<kW, false>(ds, b, Q, qi, ctl, front, cover, b.rec + Q.off, b.orig + Q.off, &wl[wave]);
+            descend<kW, false>(ds, b, Q, qi, ctl, front, cover, b.rec + Q.off, b.orig + Q.off, &wl[wave], res);
         __syncthreads();
         const uint32_t nres = ctl.nres;
         if (tid == 0) {
@@ -1135,88 +1150,66 @@ void k_batch_query(DsDev ds, BatchDev b)
             stat[0] = ctl.fp64;
         }
         if (tid < kMaxLevels) stat[1 + tid] = ctl.items[tid];
-        const uint32_t ns = min(nres, b.seg_cap);
-        bool sorted = ns <= 1;
-        if (ns > 1 && ns <= kSortMax) {
-            uint32_t* keys = reinterpret_cast<uint32_t*>(cand);
-            sorted = bucket_sort<kThreads, 8, kQuerySortBuckets>(b.seg + (uint64_t)qi * b.seg_cap, ns, keys,
-                                                                  keys + kQuerySortMax, ctl.red);
+        int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
+        uint32_t later = 0; // the landing kernel sorts and lands this query
+        uint32_t landed = kNotLanded;
+        if (nres == 0) {
+            landed = 0;
+        } else if (nres <= b.seg_cap && nres <= kResLds) {
+            uint32_t key[kResItems];
+#pragma unroll
+            for (int i = 0; i < kResItems; ++i) {
+                const uint32_t j = tid + i * kThreads;
+                key[i] = j < nres ? (res[j] ^ 0x80000000u) : 0u;
+            }
+            uint32_t* s_out = reinterpret_cast<uint32_t*>(cand); // the records are done with
+            if (bucket_sort_regs<kThreads, kResItems, kResBuckets>(key, nres, s_out, s_out + kResLds, ctl.red)) {
+                if (tid == 0) ctl.land = reserve_landing(b, nres);
+                __syncthreads();
+                landed = ctl.land;
+                // a full landing area: the sorted ids stay in the segment (the run is repeated)
+                int32_t* dst = landed != kNotLanded ? b.dland + landed : seg;
+                for (uint32_t i = tid; i < nres; i += kThreads) dst[i] = (int32_t)(s_out[i] ^ 0x80000000u);
+            } else {
+                for (uint32_t i = tid; i < nres; i += kThreads) seg[i] = (int32_t)res[i];
+                later = 1;
+            }
+        } else if (nres <= b.seg_cap) { // the LDS part joins the spill in the segment
+            for (uint32_t i = tid; i < kResLds; i += kThreads) seg[i] = (int32_t)res[i];
+            later = 1;
         }
-        if (tid == 0) stat[kSortSlot] = sorted ? 0u : 1u;
+        if (tid == 0) {
+            stat[kSortSlot] = later;
+            b.qland[qi] = later ? kNotLanded : landed;
+        }
         __syncthreads(); // before the next query resets the state
     }
 }
 
-// Segments the traversal kernel left unsorted (larger than kQuerySortMax,
-// or bunched ids): bucket sort with up to kDeviceSortMax keys, else the
-// radix sort.  Larger segments are sorted when fetched (k_batch_compact).
+// The step's last launch: a few blocks, each taking every kLandGrid-th
+// query.  A query the traversal kernel left to this launch (results beyond
+// the block's LDS buffer, or ids its bucket sort could not place) is sorted
+// here first: up to kDeviceSortMax ids bucket-sorted in LDS (rocPRIM's block
+// radix sort via hipCUB when they bunch), larger segments by LSD radix passes
+// between the segment and the landing area.  Then a wave per query writes
+// its sorted ids, count and landing offset to pinned host memory.  Kernel
+// stores to pinned memory measured 53-56 GB/s, above the runtime's copy (49,
+// a full-grid copy kernel that takes CU slots from the next step's query
+// blocks); a few blocks keep PCIe busy while holding few slots.
 constexpr int kSortBlock = 512;
+constexpr int kLandGrid = 32;
 constexpr uint32_t kSortBuckets = 4096;
 static_assert(kDeviceSortMax == kSortBlock * 16, "device sort capacity");
 using SortLarge = hipcub::BlockRadixSort<int32_t, kSortBlock, 16>;
 
-__global__ __launch_bounds__(kSortBlock) void k_batch_sort(BatchDev b)
-{
-    __shared__ union {
-        struct {
-            uint32_t keys[kDeviceSortMax];
-            uint32_t cnt[kSortBuckets];
-            uint32_t red[2 * kSortBlock / 64 + 1];
-        } bk;
-        typename SortLarge::TempStorage radix;
-    } lds;
-    for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
-        if (b.qstat[(uint64_t)qi * kQueryStats + kSortSlot] == 0) continue;
-        const uint32_t n = min(b.qcount[qi], b.seg_cap);
-        if (n > kDeviceSortMax) continue;
-        int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
-        if (!bucket_sort<kSortBlock, 16, kSortBuckets>(seg, n, lds.bk.keys, lds.bk.cnt, lds.bk.red))
-            sort_segment<SortLarge, 16>(seg, n, b.id_bits, lds.radix);
-        __syncthreads();
-    }
-}
-
-// Fetch-time: exclusive scan of the per-query result counts (one block) for
-// the compacted layout.
-__global__ __launch_bounds__(1024) void k_batch_scan(BatchDev b)
-{
-    __shared__ uint64_t s_part[1024];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (b.nq + 1023) / 1024;
-    const uint32_t lo = min(b.nq, t * per), hi = min(b.nq, lo + per);
-    uint64_t sum = 0;
-    for (uint32_t i = lo; i < hi; ++i) sum += min(b.qcount[i], b.seg_cap);
-    s_part[t] = sum;
-    __syncthreads();
-    for (uint32_t o = 1; o < 1024; o <<= 1) {
-        const uint64_t v = t >= o ? s_part[t - o] : 0;
-        __syncthreads();
-        s_part[t] += v;
-        __syncthreads();
-    }
-    uint64_t run = s_part[t] - sum;
-    for (uint32_t i = lo; i < hi; ++i) {
-        b.qoff[i] = run;
-        run += min(b.qcount[i], b.seg_cap);
-    }
-    if (t == 1023) b.qoff[b.nq] = s_part[1023];
-}
-
-// Fetch-time: each query's segment to its place in the compacted array.
-// Segments larger than kDeviceSortMax (left unsorted by the traversal and
-// k_batch_sort) are sorted on the way by a stable LSD radix sort: 8-bit
-// digits over the bits below the largest id, passes alternating between the
-// segment and its destination so that the last one lands in the
-// destination.  Each wave owns a contiguous eighth of the keys: it counts
-// its digits, and the block turns the counts into a start per (wave, digit)
-// (digit order first, wave order within a digit).  Then each wave scatters
-// its keys 64 at a time in order, a key's place among the equal digits of
-// its chunk coming from eight ballots.  The cost does not depend on how the
-// ids are spread (a bucket sort's does).  The segment itself is scratch for
-// half the passes; its contents stay a permutation of the result set, so a
-// second fetch gives the same output.
-constexpr int kCompactBlock = 512;
-constexpr uint32_t kCompactWaves = kCompactBlock / 64;
+// Stable LSD radix sort passes for segments beyond kDeviceSortMax: 8-bit
+// digits over the bits below the largest id.  Each of the block's waves owns
+// a contiguous eighth of the keys: it counts its digits, and the block turns
+// the counts into a start per (wave, digit) (digit order first, wave order
+// within a digit).  Then each wave scatters its keys 64 at a time in order,
+// a key's place among the equal digits of its chunk coming from eight
+// ballots.  The cost does not depend on how the ids are spread.
+constexpr uint32_t kLandWaves = kSortBlock / 64;
 constexpr uint32_t kRadixDigits = 256;
 
 __device__ void radix_pass(const int32_t* __restrict__ src, int32_t* __restrict__ dst, uint32_t n, uint32_t per,
@@ -1231,7 +1224,7 @@ __device__ void radix_pass(const int32_t* __restrict__ src, int32_t* __restrict_
     // digits) and each wave's start within the digit (a scan over waves)
     uint32_t tot = 0, inc = 0;
     if (tid < kRadixDigits) {
-        for (uint32_t w = 0; w < kCompactWaves; ++w) {
+        for (uint32_t w = 0; w < kLandWaves; ++w) {
             const uint32_t c = hist[w][tid];
             hist[w][tid] = tot;
             tot += c;
@@ -1247,7 +1240,7 @@ __device__ void radix_pass(const int32_t* __restrict__ src, int32_t* __restrict_
     if (tid < kRadixDigits) {
         uint32_t base = inc - tot;
         for (uint32_t w = 0; w < wave; ++w) base += red[w];
-        for (uint32_t w = 0; w < kCompactWaves; ++w) hist[w][tid] += base;
+        for (uint32_t w = 0; w < kLandWaves; ++w) hist[w][tid] += base;
     }
     __syncthreads();
     for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
@@ -1268,40 +1261,136 @@ __device__ void radix_pass(const int32_t* __restrict__ src, int32_t* __restrict_
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kCompactBlock) void k_batch_compact(BatchDev b)
+__global__ __launch_bounds__(kSortBlock) void k_batch_land(BatchDev b)
 {
-    __shared__ uint32_t hist[kCompactWaves][kRadixDigits];
-    __shared__ uint32_t red[kCompactWaves];
+    __shared__ union {
+        struct {
+            uint32_t keys[kDeviceSortMax];
+            uint32_t cnt[kSortBuckets];
+            uint32_t red[2 * kSortBlock / 64 + 1];
+        } bk;
+        typename SortLarge::TempStorage radix;
+        struct {
+            uint32_t hist[kLandWaves][kRadixDigits];
+            uint32_t red[kLandWaves];
+        } rx;
+    } lds;
+    __shared__ uint32_t s_start;
+    const uint32_t tid = threadIdx.x;
     for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
-        const uint32_t n = min(b.qcount[qi], b.seg_cap);
-        int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
-        int32_t* dst = b.out + b.qoff[qi];
-        if (n <= kDeviceSortMax) { // sorted already
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = seg[i];
-            continue;
-        }
-        // ids below 2^id_bits (non-negative), or any int32 (id_bits = 32:
-        // the sign bit flipped so that unsigned digit order is id order)
-        const uint32_t bits = b.id_bits < 32 ? b.id_bits : 32;
-        const uint32_t flip = bits == 32 ? 0x80000000u : 0u;
-        const uint32_t passes = (bits + 7) / 8;
-        const uint32_t per = (n + kCompactWaves - 1) / kCompactWaves;
-        int32_t* from = seg;
-        int32_t* to = dst;
-        if ((passes & 1u) == 0) { // an even pass count ends where it starts: start in the destination
-            for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = seg[i];
+        const uint32_t n = b.qcount[qi];
+        uint32_t start = b.qland[qi];
+        if (b.qstat[(uint64_t)qi * kQueryStats + kSortSlot]) { // n <= seg_cap: the traversal leaves no other here
+            int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
+            if (tid == 0) s_start = reserve_landing(b, n);
             __syncthreads();
-            from = dst;
-            to = seg;
+            start = s_start;
+            if (start != kNotLanded) {
+                int32_t* dst = b.dland + start;
+                if (n <= kDeviceSortMax) {
+                    uint32_t key[16];
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) {
+                        const uint32_t j = tid + i * kSortBlock;
+                        key[i] = j < n ? ((uint32_t)seg[j] ^ 0x80000000u) : 0u;
+                    }
+                    if (bucket_sort_regs<kSortBlock, 16, kSortBuckets>(key, n, lds.bk.keys, lds.bk.cnt,
+                                                                       lds.bk.red)) {
+                        for (uint32_t i = tid; i < n; i += kSortBlock)
+                            dst[i] = (int32_t)(lds.bk.keys[i] ^ 0x80000000u);
+                    } else {
+                        __syncthreads(); // every thread has read the bucket sort's flag: the union is free
+                        int32_t k2[16];
+                        const uint32_t base = tid * 16;
+#pragma unroll
+                        for (int i = 0; i < 16; ++i) k2[i] = base + i < n ? seg[base + i] : INT32_MAX;
+                        // only the bits below the largest id; padding sorts last (stable, placed last)
+                        SortLarge(lds.radix).SortBlockedToStriped(k2, 0, (int)b.id_bits);
+#pragma unroll
+                        for (int i = 0; i < 16; ++i) {
+                            const uint32_t j = tid + i * kSortBlock;
+                            if (j < n) dst[j] = k2[i];
+                        }
+                    }
+                } else {
+                    // ids below 2^id_bits (non-negative), or any int32 (id_bits =
+                    // 32: the sign bit flipped so that unsigned digit order is id
+                    // order); passes alternate between the segment and the landing
+                    // area so that the last one ends in the landing area
+                    const uint32_t bits = b.id_bits < 32 ? b.id_bits : 32;
+                    const uint32_t flip = bits == 32 ? 0x80000000u : 0u;
+                    const uint32_t passes = (bits + 7) / 8;
+                    const uint32_t per = (n + kLandWaves - 1) / kLandWaves;
+                    int32_t* from = seg;
+                    int32_t* to = dst;
+                    if ((passes & 1u) == 0) { // an even pass count ends where it starts: start in the landing area
+                        for (uint32_t i = tid; i < n; i += kSortBlock) dst[i] = seg[i];
+                        __syncthreads();
+                        from = dst;
+                        to = seg;
+                    }
+                    for (uint32_t p = 0; p < passes; ++p) {
+                        radix_pass(from, to, n, per, 8 * p, flip, lds.rx.hist, lds.rx.red);
+                        int32_t* t = from;
+                        from = to;
+                        to = t;
+                    }
+                }
+            }
+            __syncthreads(); // the sorted ids are all in place
         }
-        for (uint32_t p = 0; p < passes; ++p) {
-            radix_pass(from, to, n, per, 8 * p, flip, hist, red);
-            int32_t* t = from;
-            from = to;
-            to = t;
+        if (tid == 0) b.qland[qi] = start;
+        __syncthreads(); // before the next query reuses the block's LDS
+    }
+    // to host memory, a wave per query: the ids (landed, of a result that fit
+    // its segment), the count and the landing offset
+    const uint32_t wave = tid >> 6, lane = lane_id();
+    uint32_t k = 0;
+    for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x, ++k) {
+        if (k % kLandWaves != wave) continue;
+        const uint32_t n = b.qcount[qi];
+        const uint32_t start = b.qland[qi];
+        if (start != kNotLanded && n <= b.seg_cap && n) {
+            const int32_t* src = b.dland + start;
+            int32_t* dst = b.hland + start;
+            // 16-byte lanes from the first 16-byte boundary (both areas are aligned alike)
+            const uint32_t head = min(n, (4u - (start & 3u)) & 3u);
+            if (lane < head) dst[lane] = src[lane];
+            const uint32_t nv = (n - head) / 4;
+            const int4* s4 = reinterpret_cast<const int4*>(src + head);
+            int4* d4 = reinterpret_cast<int4*>(dst + head);
+            uint32_t i = lane;
+            for (; i + 192 < nv; i += 256) { // four loads in flight before the stores
+                const int4 a0 = s4[i], a1 = s4[i + 64], a2 = s4[i + 128], a3 = s4[i + 192];
+                d4[i] = a0;
+                d4[i + 64] = a1;
+                d4[i + 128] = a2;
+                d4[i + 192] = a3;
+            }
+            for (; i < nv; i += 64) d4[i] = s4[i];
+            for (uint32_t t = head + nv * 4 + lane; t < n; t += 64) dst[t] = src[t];
+        }
+        if (lane == 0) {
+            b.hqres[qi] = n;
+            b.hqres[b.nq + qi] = start;
+        }
+        // the wave's stores complete before its next query: a bounded number
+        // of writes in flight to PCIe (measured: 5-10% faster steps than
+        // letting them queue)
+        __threadfence_system();
+    }
+    __syncthreads();
+    // the last block to finish clears the counters for the next run (every
+    // reservation of this run is done by then)
+    if (tid == 0) {
+        __threadfence_system();
+        if (atomicAdd(&b.land_ctr[1], 1ull) == (unsigned long long)gridDim.x - 1) {
+            b.land_ctr[0] = 0;
+            b.land_ctr[1] = 0;
         }
     }
 }
+
 int grid_for(uint64_t n, int per_block, int max_blocks)
 {
     uint64_t g = (n + per_block - 1) / per_block;
@@ -1401,7 +1490,8 @@ void launch_scatter_counts(const void* src_sorted, bool narrow, const uint32_t* 
 
 size_t batch_query_lds_bytes(uint32_t n_levels)
 {
-    return kCtlBytes + kCoverBytes + kCandBytes + ((size_t)(n_levels - 1) * kFront + 64) * sizeof(uint32_t);
+    return kCtlBytes + kCoverBytes + kCandBytes + kResBytes +
+           ((size_t)(n_levels - 1) * kFront + 64) * sizeof(uint32_t);
 }
 
 void launch_batch_query(const DsDev& ds, const BatchDev& b, int shape, hipStream_t st)
@@ -1411,17 +1501,7 @@ void launch_batch_query(const DsDev& ds, const BatchDev& b, int shape, hipStream
         k_batch_query<4><<<grid, 256, batch_query_lds_bytes(ds.n_levels), st>>>(ds, b);
     else
         k_batch_query<8><<<grid, 512, batch_query_lds_bytes(ds.n_levels), st>>>(ds, b);
-    k_batch_sort<<<grid < 1024 ? grid : 1024, kSortBlock, 0, st>>>(b);
-}
-
-void launch_batch_scan(const BatchDev& b, hipStream_t st)
-{
-    k_batch_scan<<<1, 1024, 0, st>>>(b);
-}
-
-void launch_batch_compact(const BatchDev& b, hipStream_t st)
-{
-    k_batch_compact<<<b.nq < 65535 ? (b.nq ? b.nq : 1) : 65535, kCompactBlock, 0, st>>>(b);
+    k_batch_land<<<grid < (uint32_t)kLandGrid ? grid : (uint32_t)kLandGrid, kSortBlock, 0, st>>>(b);
 }
 
 } // namespace rk

@@ -13,15 +13,25 @@ namespace rk {
 constexpr int kBuckets = 64;
 
 // Batched form: one block of up to kQueryBlock threads per query; tree frontiers
-// of kFront nodes per level in LDS; result segments of up to kDeviceSortMax
-// ids are sorted inside the step, larger ones by the fetch-time compaction.
+// of kFront nodes per level in LDS.  A query's result ids stay in LDS (up to
+// kResLds of them), are sorted there and written once to the batch's landing
+// area in host memory; larger results spill to the query's segment in HBM
+// and are sorted and landed by a second kernel (bucket sort up to
+// kDeviceSortMax ids, LSD radix sort beyond).
 constexpr int kQueryBlock = 512;
 constexpr uint32_t kFront = 512;
+constexpr uint32_t kResLds = 2048;
 constexpr uint32_t kDeviceSortMax = 8192;
 // per-query statistics: [0] binary64 tests, [1 + l] nodes entering level l,
-// [kSortSlot] nonzero when the traversal kernel left the segment unsorted
+// [kSortSlot] nonzero when the traversal kernel left the segment to the
+// landing kernel
 constexpr uint32_t kSortSlot = 1 + kMaxLevels;
 constexpr uint32_t kQueryStats = 2 + kMaxLevels;
+// landing offset of a query whose ids did not reach the landing area (the
+// area was full, or the result outgrew its segment)
+constexpr uint32_t kNotLanded = 0xffffffffu;
+// the landing area holds fewer ids than kNotLanded
+constexpr uint64_t kMaxLandIds = 0xfffffff0ull;
 
 struct BatchDev {
     const QDev* q;          // nq queries
@@ -31,13 +41,17 @@ struct BatchDev {
     const float* fx;        // facility coordinates of the dataset
     const float* fy;
     const double2* wbound;  // per-query |u - q| axis bounds of the record scaling
-    CandRec* rec;           // candidate records of queries with > 64 candidates
+    CandRec* rec;           // candidate records of queries with > 1024 candidates
     float2* orig;           // their facility coordinates (for the binary64 re-check)
-    int32_t* seg;           // nq segments of seg_cap ids: query i's results at seg[i * seg_cap]
+    int32_t* seg;           // nq segments of seg_cap ids: query i's spill at seg[i * seg_cap]
     uint32_t* qcount;       // results per query (may exceed seg_cap: overflow)
+    uint32_t* qland;        // per query: offset of its sorted ids in land[], or kNotLanded
     uint32_t* qstat;        // nq x kQueryStats statistics
-    uint64_t* qoff;         // nq+1 exclusive offsets of the compacted layout
-    int32_t* out;           // compacted results (written when fetched)
+    int32_t* dland;         // landing area on the device, land_cap ids
+    int32_t* hland;         // its host twin (pinned, written by the landing kernel), land_cap ids
+    uint32_t* hqres;        // pinned host copy of (qcount, qland): 2 nq words, written by the landing kernel
+    uint64_t land_cap;
+    unsigned long long* land_ctr; // [0] ids reserved by this run, [1] landing-kernel blocks done
     uint32_t id_bits;       // user ids are non-negative and below 2^id_bits (32: any int32)
 };
 
@@ -80,17 +94,18 @@ void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2*
 void launch_scatter_counts(const void* src_sorted, bool narrow, const uint32_t* perm, uint32_t n,
                            uint32_t* dst_orig, hipStream_t st);
 
-// batched form: the whole step is the traversal launch (which also sorts
-// segments of up to 4096 ids) and the large-segment sort; capture them in a
-// graph
+// batched form: the whole step is the traversal launch (which sorts and
+// lands results of up to kResLds ids in the landing area on the device) and
+// the landing launch (sorts the larger ones, then writes every query's ids,
+// count and landing offset to pinned host memory); capture them in a graph.
+// After the step, hqres says where each query's sorted ids are in hland;
+// the landing kernel's last block clears land_ctr for the next run.
 size_t batch_query_lds_bytes(uint32_t n_levels);
 // block shapes of the traversal kernel (waves per query), chosen by the autotuner
 constexpr int kShape8Waves = 0;
 constexpr int kShape4Waves = 1;
 constexpr int kNumShapes = 2;
 void launch_batch_query(const DsDev& ds, const BatchDev& b, int shape, hipStream_t st);
-void launch_batch_scan(const BatchDev& b, hipStream_t st);    // offsets of the compacted layout
-void launch_batch_compact(const BatchDev& b, hipStream_t st); // segments -> compacted layout
 
 int device_grid(int n_cu, int waves_per_cu);
 

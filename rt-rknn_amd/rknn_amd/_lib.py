@@ -91,6 +91,7 @@ SIGNATURES = {
     "rk_batch_enqueue": (ST, [P]),
     "rk_batch_wait": (ST, [P, C.POINTER(C.c_double)]),
     "rk_batch_fetch": (ST, [P, I32P, C.c_size_t, U64P]),
+    "rk_batch_results": (ST, [P, C.POINTER(C.c_void_p), U64P]),
     "rk_batch_destroy": (ST, [P]),
     "rk_batch_stats": (ST, [P, U64P, C.c_size_t, SZP, U64P, U64P]),
     "rk_batch_config": (ST, [P, U32P]),

@@ -255,18 +255,48 @@ class Batch:
         self.ds.ctx.check(self._lib.rk_batch_wait(self.h, C.byref(s)))
         return s.value
 
-    def fetch(self):
-        offs = np.zeros(self.nq + 1, dtype=np.uint64)
-        cap = max(1, self.ds.nu)
+    def fetch_flat(self):
+        """rk_batch_fetch into a buffer the batch keeps (grown when short):
+        (ids, offsets) -- the ids of query i are ids[offsets[i]:offsets[i+1]];
+        overwritten by the next fetch."""
+        if not hasattr(self, "_fo"):
+            self._fo = np.zeros(self.nq + 1, dtype=np.uint64)
+            self._fbuf = np.empty(1 << 16, dtype=np.int32)
         while True:
-            out = np.empty(cap, dtype=np.int32)
-            st = self._lib.rk_batch_fetch(self.h, _ptr(out, C.c_int32), cap, _ptr(offs, C.c_uint64))
-            if st == L.RK_ERR_CAPACITY and int(offs[self.nq]) > cap:
-                cap = int(offs[self.nq])
+            out = self._fbuf
+            st = self._lib.rk_batch_fetch(self.h, _ptr(out, C.c_int32), len(out), _ptr(self._fo, C.c_uint64))
+            if st == L.RK_ERR_CAPACITY and int(self._fo[self.nq]) > len(out):
+                self._fbuf = np.empty(int(self._fo[self.nq]), dtype=np.int32)
                 continue
             self.ds.ctx.check(st)
-            break
+            return out, self._fo
+
+    def fetch(self):
+        out, offs = self.fetch_flat()
         return [out[int(offs[i]): int(offs[i + 1])].copy() for i in range(self.nq)]
+
+    def results_raw(self):
+        """rk_batch_results: per query the address and length of its sorted
+        ids in the batch's pinned landing area (no copy), as two uint64
+        arrays; valid until the batch runs again."""
+        if not hasattr(self, "_rptr"):
+            self._rptr = np.zeros(max(1, self.nq), dtype=np.uint64)
+            self._rn = np.zeros(max(1, self.nq), dtype=np.uint64)
+        self.ds.ctx.check(self._lib.rk_batch_results(self.h, self._rptr.ctypes.data_as(C.POINTER(C.c_void_p)),
+                                                     _ptr(self._rn, C.c_uint64)))
+        return self._rptr[: self.nq], self._rn[: self.nq]
+
+    def results(self):
+        """The last run's results as numpy views of the landing area (no
+        copy); valid until the batch runs again."""
+        ptrs, ns = self.results_raw()
+        out = []
+        for p, n in zip(ptrs.tolist(), ns.tolist()):
+            if n == 0:
+                out.append(np.empty(0, np.int32))
+            else:
+                out.append(np.ctypeslib.as_array(C.cast(C.c_void_p(p), C.POINTER(C.c_int32)), shape=(n,)))
+        return out
 
     def stats(self):
         """(items per tree level, leaf first; results; binary64 re-checks) of the last run."""

@@ -118,6 +118,14 @@ def test_large_results_sorted_on_the_device(rk, ctx, filt, negative_ids, monkeyp
         qs = [int(q) for q in rk.generate_queries(400, 64, seed)]
         ks = [8 + (i % 16) * 4 for i in range(64)]
         got = ds.query_batch(list(zip(qs, ks)))
+        if filt == "host":  # a prepared batch: fetch and the in-place results agree with the one call
+            b = ds.prepare_batch([(q, k, ds.filter(q, k)) for q, k in zip(qs, ks)])
+            try:
+                b.run()
+                for g, f, v in zip(got, b.fetch(), b.results()):
+                    assert np.array_equal(g, f) and np.array_equal(g, v)
+            finally:
+                b.close()
         sizes = [len(g) for g in got]
         print(f"{filt} negative={negative_ids}: sizes {min(sizes)}..{max(sizes)}, total {sum(sizes)}, "
               f"{sum(s > 8192 for s in sizes)} beyond 8192", flush=True)

@@ -80,6 +80,8 @@ def test_batch_enqueue_wait_golden(rk, g1, gties):
         for b, exp in bs:
             for g, e in zip(b.fetch(), exp):
                 assert np.array_equal(g, e), rnd
+            for g, e in zip(b.results(), exp):  # in place, in the pinned landing area
+                assert np.array_equal(g, e), rnd
     b = bs[0][0]
     with pytest.raises(rk.RkError) as e:
         b.wait()  # nothing enqueued
