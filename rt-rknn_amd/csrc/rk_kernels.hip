@@ -686,8 +686,13 @@ constexpr size_t kCandBytes = sizeof(SharedRecsIdx<kRecLds>) > kQWaves * sizeof(
                                   ? sizeof(SharedRecsIdx<kRecLds>)
                                   : kQWaves * sizeof(WaveLds);
 constexpr size_t kResBytes = kResLds * sizeof(uint32_t); // the query's result ids
-constexpr uint32_t kResBuckets = 1024;                    // buckets of the in-block sort
-static_assert((kResLds + kResBuckets) * sizeof(uint32_t) <= kCandBytes, "in-block sort storage");
+// After the descent the block sorts up to 8 result ids per thread itself (in
+// the LDS from the cover counts up to the frontiers, free by then); larger
+// results go to the landing kernel.
+constexpr int kInItems = 8;
+constexpr uint32_t kResBuckets = 1024; // buckets of the in-block sort
+static_assert((kQueryBlock * kInItems + kResBuckets) * sizeof(uint32_t) <= kCoverBytes + kCandBytes + kResBytes,
+              "in-block sort storage");
 
 // Sign quadrant of a record: bit 0 gx < 0, bit 1 gy < 0 (the complement of
 // the corner choice gx >= 0 ? low : high of the node test).
@@ -1059,8 +1064,8 @@ __global__ __launch_bounds__(kW * 64) __attribute__((amdgpu_waves_per_eu(kW, 8))
 void k_batch_query(DsDev ds, BatchDev b)
 {
     constexpr uint32_t kThreads = kW * 64;
-    constexpr int kResItems = (int)(kResLds / kThreads); // result ids per thread in the in-block sort
-    static_assert(kResItems * kThreads == kResLds, "result buffer per thread");
+    constexpr uint32_t kInSortMax = kThreads * kInItems; // result ids the block sorts itself
+    static_assert(kInSortMax >= kResLds, "the in-block sort takes the whole LDS result buffer");
     extern __shared__ __align__(16) unsigned char smem[];
     QueryCtl& ctl = *reinterpret_cast<QueryCtl*>(smem);
     uint32_t* ctl_words = reinterpret_cast<uint32_t*>(smem);
@@ -1155,25 +1160,34 @@ void k_batch_query(DsDev ds, BatchDev b)
         uint32_t landed = kNotLanded;
         if (nres == 0) {
             landed = 0;
-        } else if (nres <= b.seg_cap && nres <= kResLds) {
-            uint32_t key[kResItems];
+        } else if (nres <= b.seg_cap && nres <= kInSortMax) {
+            // the keys: the LDS buffer's, then the spill in the segment
+            uint32_t key[kInItems];
 #pragma unroll
-            for (int i = 0; i < kResItems; ++i) {
+            for (int i = 0; i < kInItems; ++i) {
                 const uint32_t j = tid + i * kThreads;
-                key[i] = j < nres ? (res[j] ^ 0x80000000u) : 0u;
+                key[i] = j < nres ? ((j < kResLds ? res[j] : (uint32_t)seg[j]) ^ 0x80000000u) : 0u;
             }
-            uint32_t* s_out = reinterpret_cast<uint32_t*>(cand); // the records are done with
-            if (bucket_sort_regs<kThreads, kResItems, kResBuckets>(key, nres, s_out, s_out + kResLds, ctl.red)) {
+            __syncthreads(); // every key is in registers: the LDS from the cover counts on is free
+            uint32_t* s_out = cover;
+            if (bucket_sort_regs<kThreads, kInItems, kResBuckets>(key, nres, s_out, s_out + kInSortMax, ctl.red)) {
                 if (tid == 0) ctl.land = reserve_landing(b, nres);
                 __syncthreads();
                 landed = ctl.land;
                 // a full landing area: the sorted ids stay in the segment (the run is repeated)
                 int32_t* dst = landed != kNotLanded ? b.dland + landed : seg;
                 for (uint32_t i = tid; i < nres; i += kThreads) dst[i] = (int32_t)(s_out[i] ^ 0x80000000u);
-            } else {
-                for (uint32_t i = tid; i < nres; i += kThreads) seg[i] = (int32_t)res[i];
+            } else { // bunched ids: the landing kernel's sort, from the segment
+#pragma unroll
+                for (int i = 0; i < kInItems; ++i) {
+                    const uint32_t j = tid + i * kThreads;
+                    if (j < nres) seg[j] = (int32_t)(key[i] ^ 0x80000000u);
+                }
                 later = 1;
             }
+            // the cover counts start at zero for the next query
+            __syncthreads();
+            for (uint32_t i = tid; i < kPhaseItems * 64; i += kThreads) cover[i] = 0;
         } else if (nres <= b.seg_cap) { // the LDS part joins the spill in the segment
             for (uint32_t i = tid; i < kResLds; i += kThreads) seg[i] = (int32_t)res[i];
             later = 1;
