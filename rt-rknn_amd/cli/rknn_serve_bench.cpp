@@ -138,10 +138,13 @@ int main(int argc, char** argv)
     }
 
     std::vector<int32_t> out(max_nu + 64);
-    for (double rate : rates) {
-        rk_serve_config cfg{max_batch, 0, wait};
-        rk_server* srv = nullptr;
-        if ((st = rk_serve_start(ctx, &cfg, &srv)) != RK_OK) return die("rk_serve_start", ctx, st);
+    // one server for the whole sweep, as a serving process runs: started and
+    // warmed once, then every rate in turn (each rate's requests all complete
+    // before the next rate starts)
+    rk_serve_config cfg{max_batch, 0, wait};
+    rk_server* srv = nullptr;
+    if ((st = rk_serve_start(ctx, &cfg, &srv)) != RK_OK) return die("rk_serve_start", ctx, st);
+    {
         // untimed warm-up: four rounds of a full step's worth per dataset, all
         // in flight together, grow the server's buffers to their working size
         for (int round = 0; round < 4; ++round) {
@@ -159,6 +162,8 @@ int main(int argc, char** argv)
                     std::this_thread::yield();
             }
         }
+    }
+    for (double rate : rates) {
         std::mt19937_64 rng(seed ^ (uint64_t)rate);
         std::exponential_distribution<double> gap(rate);
         std::uniform_int_distribution<int> kd(1, kmax);
@@ -207,7 +212,6 @@ int main(int argc, char** argv)
             }
         }
         submitter.join();
-        rk_serve_stop(srv);
         std::sort(lat.begin(), lat.end());
         auto pct = [&](double p) { return lat.empty() ? 0.0 : lat[std::min(lat.size() - 1, (size_t)(p * lat.size()))]; };
         const double span = last_done - t0;
@@ -217,6 +221,7 @@ int main(int argc, char** argv)
                     lat.empty() ? 0.0 : lat.back() * 1e3, res.size(), failed ? "true" : "false");
         std::fflush(stdout);
     }
+    rk_serve_stop(srv); // before the datasets it served go
     for (Resident& r : res) rk_dataset_destroy(r.ds);
     rk_shutdown(ctx);
     return 0;

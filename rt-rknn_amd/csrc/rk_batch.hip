@@ -80,7 +80,7 @@ rk_status recapture(rk_batch* b)
     return build_graph(b);
 }
 
-// The landing area holds at least `ids` ids (host and device twins, capped
+// The landing area holds at least `ids` ids (host and device sides, capped
 // at kMaxLandIds); its reservation counters start at zero.  Caller holds the
 // context lock; the batch has no run in flight.
 rk_status ensure_landing(rk_batch* b, uint64_t ids)
@@ -92,33 +92,47 @@ rk_status ensure_landing(rk_batch* b, uint64_t ids)
         RK_HIP(c, hipMemsetAsync(b->land_ctr.p, 0, 2 * sizeof(unsigned long long), c->stream));
         RK_HIP(c, hipStreamSynchronize(c->stream));
     }
-    if (ids <= b->land_cap) return RK_OK;
-    b->land_cap = 0;
-    b->hland_dev = nullptr;
-    // coherent: the kernel's stores reach host memory as they retire, and the
-    // host reads them after the run's event without any cache maintenance
-    RK_HIP(c, b->h_land.reserve(ids, hipHostMallocCoherent));
+    if (!b->land_host || b->land_host->n < ids) {
+        // a new host side (an old one may still be referenced by served
+        // results): from the batch's source of areas when it has one
+        const uint64_t want = std::min<uint64_t>(
+            b->land_host ? std::max<uint64_t>(ids, b->land_host->n + b->land_host->n / 2) : ids, kMaxLandIds);
+        if (b->area_source) {
+            b->land_host = b->area_source(want);
+            if (!b->land_host) return fail(c, RK_ERR_OUT_OF_MEMORY, "pinned landing area");
+        } else {
+            auto a = std::make_shared<PinnedIds>();
+            RK_HIP(c, a->alloc(want));
+            b->land_host = std::move(a);
+        }
+    }
     RK_HIP(c, b->dland.reserve(ids));
-    void* dp = nullptr;
-    RK_HIP(c, hipHostGetDevicePointer(&dp, b->h_land.p, 0));
-    b->hland_dev = static_cast<int32_t*>(dp);
-    b->land_cap = std::min<uint64_t>(std::min<uint64_t>(b->h_land.n, b->dland.n), kMaxLandIds);
+    b->land_cap = std::min<uint64_t>(std::min<uint64_t>(b->land_host->n, b->dland.n), kMaxLandIds);
     return RK_OK;
 }
 
-// Room for the counts and landing offsets of nq queries in pinned host
-// memory (written by the step's last kernel).  Caller holds the context lock.
+// Room for kQresWords words per query in pinned, coherent host memory
+// (written by the step's last kernel).  Caller holds the context lock.
 rk_status ensure_qres(rk_batch* b, uint64_t nq)
 {
     rk_ctx* c = b->ds->ctx;
-    const uint64_t want = std::max<uint64_t>(2 * nq, 2);
-    if (want <= b->h_qres.n && b->hqres_dev) return RK_OK;
-    b->hqres_dev = nullptr;
-    RK_HIP(c, b->h_qres.reserve(want, hipHostMallocCoherent));
-    void* dp = nullptr;
-    RK_HIP(c, hipHostGetDevicePointer(&dp, b->h_qres.p, 0));
-    b->hqres_dev = static_cast<uint32_t*>(dp);
+    RK_HIP(c, b->h_qres.reserve(std::max<uint64_t>(kQresWords * nq, kQresWords), hipHostMallocCoherent));
     return RK_OK;
+}
+
+// Blocks of the landing kernel: kLandGridMin, or one per query whose
+// expected result (k N_u / N_f ids: every user is in the reverse k-NN set
+// of exactly k facilities) is over half of what a traversal block sorts
+// itself -- those are sorted by the landing kernel.
+uint32_t landing_grid(const rk_dataset* ds, const uint32_t* k, size_t nq, const QDev* hq)
+{
+    const double per_k = ds->nf ? (double)ds->nu / (double)ds->nf : 0.0;
+    uint32_t big = 0;
+    for (size_t i = 0; i < nq; ++i) {
+        const uint32_t ki = k ? k[i] : hq[i].k;
+        if ((!hq || !hq[i].trivial) && ki * per_k > 0.5 * kQueryBlock * 8) ++big;
+    }
+    return std::max(kLandGridMin, std::min<uint32_t>(big, 1024));
 }
 
 // The landed extent of a finished run: every query that fit its segment has
@@ -144,18 +158,21 @@ uint64_t landed_extent(const rk_batch* b, bool* short_area, uint64_t* need)
     return ext;
 }
 
-// The kernels' view of a filled batch.
+// The kernels' view of a filled batch.  Inputs: a device-filtered batch's
+// queries and candidate lists are made on the device; a host-filtered one's
+// are in HBM (resident) or read in place from the pinned staging.
 void set_view(rk_batch* b, uint64_t seg_cap)
 {
     rk_dataset* ds = b->ds;
     BatchDev& v = b->view;
-    v.q = b->q.p;
+    const bool dev_inputs = b->dfilter || b->resident;
+    v.q = dev_inputs ? b->q.p : b->s_q.dev;
     v.nq = b->nq;
     v.seg_cap = (uint32_t)seg_cap;
-    v.cand = b->cand.p;
+    v.cand = dev_inputs ? b->cand.p : b->s_cand.dev;
     v.fx = ds->fx.p;
     v.fy = ds->fy.p;
-    v.wbound = b->wb.p;
+    v.wbound = dev_inputs ? b->wb.p : b->s_wb.dev;
     v.rec = b->rec.p;
     v.orig = b->orig.p;
     v.seg = b->seg.p;
@@ -163,12 +180,18 @@ void set_view(rk_batch* b, uint64_t seg_cap)
     v.qland = b->qres.p + b->nq;
     v.qstat = b->qstat.p;
     v.dland = b->dland.p;
-    v.hland = b->hland_dev;
-    v.hqres = b->hqres_dev;
+    v.hland = b->land_host ? b->land_host->dev : nullptr;
+    v.hqres = b->h_qres.dev;
+    v.dn = b->dfilter ? b->dn.p : nullptr;
+    v.land_grid = b->land_grid;
     v.land_cap = b->land_cap;
     v.land_ctr = b->land_ctr.p;
     v.id_bits = ds->id_bits;
 }
+
+// The device filter's inputs: in HBM (resident) or the pinned staging.
+const uint32_t* dfilter_q(const rk_batch* b) { return b->resident ? b->dq.p : b->s_dq.dev; }
+const uint32_t* dfilter_k(const rk_batch* b) { return b->resident ? b->dk.p : b->s_dk.dev; }
 
 // First landing area of a batch: room for 2,048 ids per query (a run that
 // needs more grows it and is repeated, rk_batch_wait).
@@ -297,9 +320,6 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
     hipStream_t st = c->stream;
-    RK_HIP(c, b->q.reserve(nq));
-    RK_HIP(c, b->wb.reserve(nq));
-    RK_HIP(c, b->cand.reserve(all.size()));
     RK_HIP(c, b->rec.reserve(all.size()));
     RK_HIP(c, b->orig.reserve(all.size()));
     RK_HIP(c, b->qres.reserve(2 * (uint64_t)nq));
@@ -314,11 +334,28 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
         if (ls == RK_OK) ls = ensure_qres(b, nq);
         if (ls != RK_OK) return ls;
     }
-    if (nq) RK_HIP(c, hipMemcpyAsync(b->q.p, b->hq.data(), nq * sizeof(QDev), hipMemcpyHostToDevice, st));
-    if (nq) RK_HIP(c, hipMemcpyAsync(b->wb.p, wb.data(), nq * sizeof(double2), hipMemcpyHostToDevice, st));
-    if (!all.empty())
-        RK_HIP(c, hipMemcpyAsync(b->cand.p, all.data(), all.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    RK_HIP(c, hipStreamSynchronize(st));
+    // the inputs into pinned staging (every previous run of this batch is
+    // done: fills refuse a pending batch)
+    RK_HIP(c, b->s_q.reserve(std::max<size_t>(nq, 1), hipHostMallocCoherent));
+    RK_HIP(c, b->s_wb.reserve(std::max<size_t>(nq, 1), hipHostMallocCoherent));
+    RK_HIP(c, b->s_cand.reserve(std::max<size_t>(all.size(), 1), hipHostMallocCoherent));
+    if (nq) {
+        std::memcpy(b->s_q.p, b->hq.data(), nq * sizeof(QDev));
+        std::memcpy(b->s_wb.p, wb.data(), nq * sizeof(double2));
+    }
+    if (!all.empty()) std::memcpy(b->s_cand.p, all.data(), all.size() * sizeof(uint32_t));
+    if (b->resident) { // to HBM, once
+        RK_HIP(c, b->q.reserve(nq));
+        RK_HIP(c, b->wb.reserve(nq));
+        RK_HIP(c, b->cand.reserve(all.size()));
+        if (nq) RK_HIP(c, hipMemcpyAsync(b->q.p, b->s_q.p, nq * sizeof(QDev), hipMemcpyHostToDevice, st));
+        if (nq) RK_HIP(c, hipMemcpyAsync(b->wb.p, b->s_wb.p, nq * sizeof(double2), hipMemcpyHostToDevice, st));
+        if (!all.empty())
+            RK_HIP(c, hipMemcpyAsync(b->cand.p, b->s_cand.p, all.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                     st));
+        RK_HIP(c, hipStreamSynchronize(st));
+    }
+    b->land_grid = landing_grid(ds, nullptr, nq, b->hq.data());
     set_view(b, seg_cap);
     return RK_OK;
 }
@@ -327,9 +364,9 @@ void batch_launch(rk_batch* b, hipStream_t st)
 {
     rk_dataset* ds = b->ds;
     if (b->dfilter) {
-        launch_dfilter(ds->fgrid, b->dq.p, b->dk.p, b->nq, b->cand.p, b->cand_cap, b->dn.p, st);
-        launch_dfilter_setup(ds->fgrid, b->dq.p, b->dk.p, b->nq, b->dn.p, b->cand_cap, (uint32_t)ds->nu, b->q.p,
-                             b->wb.p, st);
+        launch_dfilter(ds->fgrid, dfilter_q(b), dfilter_k(b), b->nq, b->cand.p, b->cand_cap, b->dn.p, st);
+        launch_dfilter_setup(ds->fgrid, dfilter_q(b), dfilter_k(b), b->nq, b->dn.p, b->cand_cap, (uint32_t)ds->nu,
+                             b->q.p, b->wb.p, st);
     }
     launch_batch_query(ds->view, b->view, b->shape, st);
 }
@@ -351,12 +388,12 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
     b->filter_s = 0.0;
     b->total_cand = 0;
     b->hq.assign(nq, QDev{});
+    for (size_t i = 0; i < nq; ++i) b->hq[i].k = k[i]; // the rest comes from each run
     b->h_dn.assign(nq, 0);
 
     // candidate segment per query: every facility at most, 4096 slots at most
     b->cand_cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ds->nf, dfilter_slots()));
     const uint64_t slots = std::max<uint64_t>(1, (uint64_t)nq * b->cand_cap);
-
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
     hipStream_t st = c->stream;
@@ -379,13 +416,70 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
         if (ls == RK_OK) ls = ensure_qres(b, nq);
         if (ls != RK_OK) return ls;
     }
+    RK_HIP(c, b->s_dq.reserve(std::max<size_t>(nq, 1), hipHostMallocCoherent));
+    RK_HIP(c, b->s_dk.reserve(std::max<size_t>(nq, 1), hipHostMallocCoherent));
     if (nq) {
-        RK_HIP(c, hipMemcpyAsync(b->dq.p, q, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-        RK_HIP(c, hipMemcpyAsync(b->dk.p, k, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        std::memcpy(b->s_dq.p, q, nq * sizeof(uint32_t));
+        std::memcpy(b->s_dk.p, k, nq * sizeof(uint32_t));
     }
-    RK_HIP(c, hipStreamSynchronize(st));
+    if (b->resident && nq) { // to HBM, once
+        RK_HIP(c, hipMemcpyAsync(b->dq.p, b->s_dq.p, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        RK_HIP(c, hipMemcpyAsync(b->dk.p, b->s_dk.p, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+        RK_HIP(c, hipStreamSynchronize(st));
+    }
+    b->land_grid = landing_grid(ds, k, nq, nullptr);
     set_view(b, seg_cap);
     return RK_OK;
+}
+
+rk_status batch_presize(rk_batch* b, size_t nq, bool dfilter)
+{
+    rk_dataset* ds = b->ds;
+    rk_ctx* c = ds->ctx;
+    if (b->pending) return fail(c, RK_ERR_INVALID_ARG, "the batch's last run has not been waited for");
+    if (!nq || nq > (1u << 24)) return RK_OK;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    RK_HIP(c, hipSetDevice(c->device));
+    RK_HIP(c, b->q.reserve(nq));
+    RK_HIP(c, b->wb.reserve(nq));
+    RK_HIP(c, b->qres.reserve(2 * (uint64_t)nq));
+    RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
+    RK_HIP(c, b->s_q.reserve(nq, hipHostMallocCoherent));
+    RK_HIP(c, b->s_wb.reserve(nq, hipHostMallocCoherent));
+    if (dfilter) {
+        const uint64_t slots = (uint64_t)nq * std::max<uint64_t>(1, std::min<uint64_t>(ds->nf, dfilter_slots()));
+        RK_HIP(c, b->dq.reserve(nq));
+        RK_HIP(c, b->dk.reserve(nq));
+        RK_HIP(c, b->dn.reserve(nq));
+        RK_HIP(c, b->cand.reserve(slots));
+        RK_HIP(c, b->rec.reserve(slots));
+        RK_HIP(c, b->orig.reserve(slots));
+        RK_HIP(c, b->s_dq.reserve(nq, hipHostMallocCoherent));
+        RK_HIP(c, b->s_dk.reserve(nq, hipHostMallocCoherent));
+    }
+    uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, pool_budget() / nq));
+    RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
+    rk_status s = ensure_landing(b, initial_landing(nq));
+    if (s == RK_OK) s = ensure_qres(b, nq);
+    return s;
+}
+
+// After a device-filtered run: the per-query views (trivial flag, candidate
+// count) from the words the landing kernel wrote to host memory; a
+// candidate list beyond its segment is an overflow.
+void views_from_host(rk_batch* b)
+{
+    const uint32_t* nc = b->h_qncand();
+    const uint32_t* fl = b->h_qflags();
+    b->cand_overflow = false;
+    b->total_cand = 0;
+    for (uint32_t i = 0; i < b->nq; ++i) {
+        b->h_dn[i] = nc[i];
+        b->hq[i].nc = std::min(nc[i], b->cand_cap);
+        b->hq[i].trivial = (fl[i] & kQflagTrivial) ? 1u : 0u;
+        b->cand_overflow |= nc[i] > b->cand_cap;
+        b->total_cand += b->hq[i].nc;
+    }
 }
 
 // After a device-filtered run: the per-query views (for fetch and the
@@ -450,11 +544,6 @@ rk_status run_on_stream(rk_batch* b)
 
 // Copies of (dst, src, n) id ranges, split over a few host threads when
 // large (one thread copies ~10 GB/s; a step's results are megabytes).
-struct IdRange {
-    int32_t* dst;
-    const int32_t* src;
-    uint64_t n;
-};
 
 void copy_ranges(const std::vector<IdRange>& r, uint64_t total)
 {
@@ -481,7 +570,6 @@ void copy_ranges(const std::vector<IdRange>& r, uint64_t total)
     for (std::thread& th : pool) th.join();
 }
 
-rk_status result_ranges(rk_batch* b, std::vector<IdRange>& ranges, std::vector<uint32_t>* skip);
 
 } // namespace rk
 
@@ -493,6 +581,7 @@ rk_status rk_batch_prepare_device(rk_dataset* ds, const uint32_t* q, const uint3
     *out = nullptr;
     std::unique_ptr<rk_batch> b(new rk_batch());
     b->ds = ds;
+    b->resident = true;
     rk_status s = batch_fill_device(b.get(), q, k, nq);
     if (s != RK_OK) return s;
     if (ds->nu && nq) {
@@ -518,6 +607,7 @@ rk_status rk_batch_prepare(rk_dataset* ds, const rk_query_desc* qs, size_t nq, r
     *out = nullptr;
     std::unique_ptr<rk_batch> b(new rk_batch());
     b->ds = ds;
+    b->resident = true;
     rk_status s = batch_fill(b.get(), qs, nq);
     if (s != RK_OK) return s;
     if (ds->nu && nq) {
@@ -580,10 +670,7 @@ rk_status rk_batch_wait(rk_batch* b, double* device_seconds)
     RK_HIP(c, hipEventElapsedTime(&ms, b->ev_start, b->ev_end));
     double dev_s = ms * 1e-3;
     for (int attempt = 0;; ++attempt) {
-        if (b->dfilter) {
-            const rk_status s = batch_download_views(b);
-            if (s != RK_OK) return s;
-        }
+        if (b->dfilter) views_from_host(b);
         // every query that fit its segment has its sorted ids in the landing
         // area, unless the area was too small: then it grows to 1.5x what the
         // run needed and the run is repeated (once; a repeat on an area of
@@ -684,7 +771,7 @@ rk_status result_ranges(rk_batch* b, std::vector<IdRange>& ranges, std::vector<u
         if (!cnt[i]) continue;
         if (land[i] == kNotLanded || (uint64_t)land[i] + cnt[i] > b->land_cap)
             return fail(c, RK_ERR_INTERNAL, "a result is missing from the landing area");
-        ranges[i] = {nullptr, b->h_land.p + land[i], cnt[i]};
+        ranges[i] = {nullptr, b->land_host->p + land[i], cnt[i]};
     }
     return RK_OK;
 }

@@ -6,6 +6,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <functional>
 #include <algorithm>
 #include <memory>
 #include <mutex>
@@ -59,10 +60,12 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
-// Owning pinned host allocation that only grows.
+// Owning pinned host allocation that only grows; dev is the address the
+// kernels reach it by (pinned memory is mapped into the device's space).
 template <class T>
 struct HostBuf {
     T* p = nullptr;
+    T* dev = nullptr;
     size_t n = 0;
     HostBuf() = default;
     HostBuf(const HostBuf&) = delete;
@@ -77,7 +80,7 @@ struct HostBuf {
         if (want <= n && p) return hipSuccess;
         if (p) want = std::max(want, n + n / 2);
         if (p) (void)hipHostFree(p);
-        p = nullptr;
+        p = dev = nullptr;
         n = 0;
         if (want == 0) want = 1;
         hipError_t e = hipHostMalloc((void**)&p, want * sizeof(T), flags);
@@ -85,7 +88,42 @@ struct HostBuf {
             p = nullptr;
             return e;
         }
+        e = hipHostGetDevicePointer((void**)&dev, p, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(p);
+            p = dev = nullptr;
+            return e;
+        }
         n = want;
+        return hipSuccess;
+    }
+};
+
+// Pinned, coherent host ids that a batch's landing kernel writes (the host
+// side of its landing area).  Shared: the serving loop hands a step's
+// results out by reference and gives the batch another area for its next
+// step.
+struct PinnedIds {
+    int32_t* p = nullptr;   // host address
+    int32_t* dev = nullptr; // the address the kernels write through
+    size_t n = 0;
+    PinnedIds() = default;
+    PinnedIds(const PinnedIds&) = delete;
+    PinnedIds& operator=(const PinnedIds&) = delete;
+    ~PinnedIds()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t alloc(size_t ids)
+    {
+        hipError_t e = hipHostMalloc((void**)&p, std::max<size_t>(ids, 1) * sizeof(int32_t), hipHostMallocCoherent);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        e = hipHostGetDevicePointer((void**)&dev, p, 0);
+        if (e != hipSuccess) return e;
+        n = std::max<size_t>(ids, 1);
         return hipSuccess;
     }
 };
@@ -161,18 +199,30 @@ struct rk_batch {
     rk::DevBuf<uint32_t> qstat;
     // landing area: the kernels sort each query's ids into dland (HBM) at
     // the offset qres[nq + i], and the run's last kernel writes them to
-    // h_land (pinned, coherent host memory), with the counts and offsets to
-    // h_qres; land_ctr: the run's reservation counters
+    // land_host (pinned, coherent host memory), with kQresWords words per
+    // query to h_qres (pinned, coherent); land_ctr: the run's reservation
+    // counters
     rk::HostBuf<uint32_t> h_qres;
-    rk::HostBuf<int32_t> h_land;
-    uint32_t* hqres_dev = nullptr; // h_qres and h_land as the device addresses them
-    int32_t* hland_dev = nullptr;
+    std::shared_ptr<rk::PinnedIds> land_host;
+    // where a new host side comes from (at least the ids asked for); empty:
+    // a plain allocation
+    std::function<std::shared_ptr<rk::PinnedIds>(size_t)> area_source;
     rk::DevBuf<int32_t> dland;
     rk::DevBuf<unsigned long long> land_ctr;
     uint64_t land_cap = 0;
+    uint32_t land_grid = 0;
+    // inputs staged in pinned, coherent host memory by every fill: a batch
+    // refilled step after step (direct) has its kernels read them in place,
+    // with no copy; a prepared batch (resident) copies them to HBM once
+    bool resident = false;
+    rk::HostBuf<rk::QDev> s_q;
+    rk::HostBuf<double2> s_wb;
+    rk::HostBuf<uint32_t> s_cand, s_dq, s_dk;
     rk::BatchDev view{};
     uint32_t* h_qcount() const { return h_qres.p; }
     uint32_t* h_qland() const { return h_qres.p + nq; }
+    uint32_t* h_qncand() const { return h_qres.p + 2 * (size_t)nq; }
+    uint32_t* h_qflags() const { return h_qres.p + 3 * (size_t)nq; }
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool ran = false;
@@ -239,6 +289,22 @@ rk_status batch_collect(rk_batch* b, rk_batch* redo, const rk_query_desc* qd, si
                         std::vector<uint64_t>& off);
 // the device work of one step of a batch (filter stages when device-filtered)
 void batch_launch(rk_batch* b, hipStream_t st);
+// the per-query buffers of a batch that will take up to nq queries (of the
+// device-filtered kind when dfilter), allocated ahead (a fill's growth
+// allocates, and device allocations wait for the device); candidate lists
+// of host-filtered queries still grow with their sizes
+rk_status batch_presize(rk_batch* b, size_t nq, bool dfilter);
+// a range of result ids: n ids at src (to be copied to dst)
+struct IdRange {
+    int32_t* dst;
+    const int32_t* src;
+    uint64_t n;
+};
+// after a run: where each query's sorted ids are (its landing-area range, or
+// the dataset's list of every user); with skip, the queries that outgrew a
+// segment get empty ranges and are listed there.  Caller holds the context
+// lock.
+rk_status result_ranges(rk_batch* b, std::vector<IdRange>& ranges, std::vector<uint32_t>* skip);
 // picks the traversal block shape for a filled batch (autotuner); caller
 // holds the context lock
 rk_status batch_tune(rk_batch* b);

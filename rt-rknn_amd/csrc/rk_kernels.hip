@@ -1200,8 +1200,8 @@ void k_batch_query(DsDev ds, BatchDev b)
     }
 }
 
-// The step's last launch: a few blocks, each taking every kLandGrid-th
-// query.  A query the traversal kernel left to this launch (results beyond
+// The step's last launch: a few blocks (BatchDev::land_grid), each taking
+// every land_grid-th query.  A query the traversal kernel left to this launch (results beyond
 // the block's LDS buffer, or ids its bucket sort could not place) is sorted
 // here first: up to kDeviceSortMax ids bucket-sorted in LDS (rocPRIM's block
 // radix sort via hipCUB when they bunch), larger segments by LSD radix passes
@@ -1211,7 +1211,6 @@ void k_batch_query(DsDev ds, BatchDev b)
 // a full-grid copy kernel that takes CU slots from the next step's query
 // blocks); a few blocks keep PCIe busy while holding few slots.
 constexpr int kSortBlock = 512;
-constexpr int kLandGrid = 32;
 constexpr uint32_t kSortBuckets = 4096;
 static_assert(kDeviceSortMax == kSortBlock * 16, "device sort capacity");
 using SortLarge = hipcub::BlockRadixSort<int32_t, kSortBlock, 16>;
@@ -1385,8 +1384,11 @@ __global__ __launch_bounds__(kSortBlock) void k_batch_land(BatchDev b)
             for (uint32_t t = head + nv * 4 + lane; t < n; t += 64) dst[t] = src[t];
         }
         if (lane == 0) {
+            const QDev& Q = b.q[qi];
             b.hqres[qi] = n;
             b.hqres[b.nq + qi] = start;
+            b.hqres[2 * b.nq + qi] = b.dn ? b.dn[qi] : Q.nc;
+            b.hqres[3 * b.nq + qi] = Q.trivial ? kQflagTrivial : 0u;
         }
         // the wave's stores complete before its next query: a bounded number
         // of writes in flight to PCIe (measured: 5-10% faster steps than
@@ -1515,7 +1517,8 @@ void launch_batch_query(const DsDev& ds, const BatchDev& b, int shape, hipStream
         k_batch_query<4><<<grid, 256, batch_query_lds_bytes(ds.n_levels), st>>>(ds, b);
     else
         k_batch_query<8><<<grid, 512, batch_query_lds_bytes(ds.n_levels), st>>>(ds, b);
-    k_batch_land<<<grid < (uint32_t)kLandGrid ? grid : (uint32_t)kLandGrid, kSortBlock, 0, st>>>(b);
+    const uint32_t lg = b.land_grid ? b.land_grid : kLandGridMin;
+    k_batch_land<<<grid < lg ? grid : lg, kSortBlock, 0, st>>>(b);
 }
 
 } // namespace rk

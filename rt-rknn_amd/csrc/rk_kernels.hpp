@@ -32,6 +32,12 @@ constexpr uint32_t kQueryStats = 2 + kMaxLevels;
 constexpr uint32_t kNotLanded = 0xffffffffu;
 // the landing area holds fewer ids than kNotLanded
 constexpr uint64_t kMaxLandIds = 0xfffffff0ull;
+// words per query of BatchDev::hqres, and its flag bits
+constexpr uint32_t kQresWords = 4;
+constexpr uint32_t kQflagTrivial = 1u; // k > |C|: every user qualifies (no traversal ran)
+// landing-kernel blocks: this many at least, more when many results are
+// larger than a traversal block sorts itself
+constexpr uint32_t kLandGridMin = 32;
 
 struct BatchDev {
     const QDev* q;          // nq queries
@@ -49,7 +55,11 @@ struct BatchDev {
     uint32_t* qstat;        // nq x kQueryStats statistics
     int32_t* dland;         // landing area on the device, land_cap ids
     int32_t* hland;         // its host twin (pinned, written by the landing kernel), land_cap ids
-    uint32_t* hqres;        // pinned host copy of (qcount, qland): 2 nq words, written by the landing kernel
+    uint32_t* hqres;        // pinned host words per query, written by the landing kernel: [0, nq) result
+                            // count, [nq, 2 nq) landing offset, [2 nq, 3 nq) candidates (the device
+                            // filter's count, beyond its segment when it overflowed), [3 nq, 4 nq) flags
+    const uint32_t* dn;     // device filter's candidate counts (device-filtered batches), else null
+    uint32_t land_grid;     // blocks of the landing kernel
     uint64_t land_cap;
     unsigned long long* land_ctr; // [0] ids reserved by this run, [1] landing-kernel blocks done
     uint32_t id_bits;       // user ids are non-negative and below 2^id_bits (32: any int32)

@@ -44,11 +44,13 @@ struct Request {
     std::vector<uint32_t> cand; // host-filtered requests: filled by the host stage
 };
 
-// A finished request: its ids are [off, off + n) of its step's result
-// array, which the step's requests share (copied out once, by the poller).
+// A finished request: its n ids at ids, inside memory that keep holds (the
+// step's landing area, shared by the step's requests, or a step's merged
+// array when some of them ran again), copied out once, by the poller.
 struct Done {
-    std::shared_ptr<const std::vector<int32_t>> buf;
-    uint64_t off = 0, n = 0;
+    std::shared_ptr<const void> keep;
+    const int32_t* ids = nullptr;
+    uint64_t n = 0;
     double latency = 0.0;
     rk_status status = RK_OK;
 };
@@ -69,9 +71,52 @@ struct IdPool {
     }
 };
 
+// Host landing areas of finished steps, returned by the last poller of a
+// step instead of freed (pinned allocations are slow and freeing one can
+// wait for the device): a batch whose area is still referenced by served
+// results takes one of these for its next step.
+// A keeper thread allocates areas ahead of need (a pinned allocation of a
+// few megabytes takes milliseconds), so that the device stage finds kSpare
+// free ones when results of earlier steps are still being polled.
+constexpr size_t kAreasKeep = 32; // areas kept
+constexpr size_t kSpare = 3;      // free areas the keeper maintains
+struct AreaPool {
+    std::mutex mu;
+    std::condition_variable cv; // wakes the keeper
+    bool stop = false;
+    std::vector<rk::PinnedIds*> free;
+    size_t largest = 0; // every new area is at least as large as the largest so far
+    ~AreaPool()
+    {
+        for (auto* a : free) delete a;
+    }
+};
+
+// The keeper: tops the free list up to kSpare areas of the largest size.
+static void area_keeper(std::shared_ptr<AreaPool> pool)
+{
+    std::unique_lock<std::mutex> lk(pool->mu);
+    while (true) {
+        pool->cv.wait(lk, [&] { return pool->stop || (pool->largest && pool->free.size() < kSpare); });
+        if (pool->stop) return;
+        const size_t n = pool->largest;
+        lk.unlock();
+        rk::PinnedIds* a = new rk::PinnedIds();
+        const bool ok = a->alloc(n) == hipSuccess;
+        lk.lock();
+        if (!ok) { // no more pinned memory: the device stage allocates (or fails) itself
+            delete a;
+            pool->cv.wait(lk, [&] { return pool->stop || pool->largest > n; });
+            continue;
+        }
+        pool->free.push_back(a);
+    }
+}
+
 struct rk_server {
-    // first member: destroyed last, after the finished results that return to it
+    // first members: destroyed last, after the finished results that return to them
     std::shared_ptr<IdPool> pool = std::make_shared<IdPool>();
+    std::shared_ptr<AreaPool> areas = std::make_shared<AreaPool>();
     rk_ctx* ctx = nullptr;
     rk_serve_config cfg{};
     std::mutex mu;
@@ -88,6 +133,7 @@ struct rk_server {
     bool stop = false;
     std::vector<std::thread> host_threads; // filter workers
     std::thread dev_thread;
+    std::thread keeper;                    // allocates landing areas ahead of need
     std::condition_variable cv_dev;
     // device stage only, two of each kind per dataset: a step is filled and
     // enqueued on one while the previous step's run, on the other, finishes
@@ -148,13 +194,22 @@ void host_stage(rk_server* s)
     }
 }
 
-rk_batch* batch_for(std::map<rk_dataset*, std::unique_ptr<rk_batch>>& m, rk_dataset* ds)
+std::shared_ptr<rk::PinnedIds> pooled_area(const std::shared_ptr<AreaPool>& pool, size_t n);
+
+// The batch object of a dataset, made on first use; a step object's
+// per-query buffers are sized for a full step (max_batch queries of its
+// kind) at once, so that no later step waits on a device allocation.
+rk_batch* batch_for(rk_server* s, std::map<rk_dataset*, std::unique_ptr<rk_batch>>& m, rk_dataset* ds,
+                    int kind)
 {
     std::unique_ptr<rk_batch>& b = m[ds];
     if (!b) {
         b.reset(new rk_batch());
         b->ds = ds;
         b->direct = true;
+        const std::shared_ptr<AreaPool> pool = s->areas;
+        b->area_source = [pool](size_t n) { return pooled_area(pool, n); };
+        if (kind >= 0) (void)rk::batch_presize(b.get(), s->cfg.max_batch, kind == 1); // a failure shows at the fill
     }
     return b.get();
 }
@@ -168,7 +223,7 @@ struct Step {
     rk_batch* b = nullptr;
     bool enqueued = false;
     rk_status st = RK_OK;
-    double t_start = 0, t_filled = 0, t_enqueued = 0; // phase clocks for RK_SERVE_TRACE
+    double t_start = 0, t_area = 0, t_filled = 0, t_enqueued = 0; // phase clocks for RK_SERVE_TRACE
 };
 
 // RK_SERVE_TRACE=1: a line on stderr for every step with a phase longer than
@@ -188,11 +243,58 @@ bool serve_trace()
 // steps time the traversal shapes for a new workload class first;
 // device-filtered steps take the default shape (8 waves per query), since
 // timing shapes needs filtered lists, which only the step itself makes.
+// A landing area of at least n ids for a batch whose own is still held by
+// served results: one from the pool, else a new one; it returns to the
+// pool when its last reference goes.
+std::shared_ptr<rk::PinnedIds> pooled_area(const std::shared_ptr<AreaPool>& pool, size_t n)
+{
+    rk::PinnedIds* a = nullptr;
+    size_t want = n;
+    {
+        std::lock_guard<std::mutex> lk(pool->mu);
+        for (size_t i = 0; i < pool->free.size(); ++i)
+            if (pool->free[i]->n >= n) {
+                a = pool->free[i];
+                pool->free.erase(pool->free.begin() + (ptrdiff_t)i);
+                break;
+            }
+        // areas of one size: any free one fits any batch
+        want = std::max(n, pool->largest);
+        if (!a) pool->largest = want;
+        if (pool->free.size() < kSpare) pool->cv.notify_one();
+    }
+    if (!a) {
+        a = new rk::PinnedIds();
+        if (a->alloc(want) != hipSuccess) {
+            delete a;
+            return nullptr;
+        }
+    }
+    return std::shared_ptr<rk::PinnedIds>(a, [pool](rk::PinnedIds* x) {
+        std::lock_guard<std::mutex> lk(pool->mu);
+        if (pool->free.size() < kAreasKeep)
+            pool->free.push_back(x);
+        else
+            delete x;
+    });
+}
+
+// The batch gets a landing area no served result refers to.
+void fresh_area(rk_server* s, rk_batch* b)
+{
+    if (!b->land_host || b->land_host.use_count() == 1) return;
+    std::shared_ptr<rk::PinnedIds> a = pooled_area(s->areas, b->land_host->n);
+    if (a) b->land_host = std::move(a); // else the fill allocates one
+    else b->land_host.reset();
+}
+
 void start_step(rk_server* s, Step& p, int slot)
 {
     p.t_start = rk::now_s();
     if (p.dev) {
-        p.b = batch_for(s->dbatches[slot], p.ds);
+        p.b = batch_for(s, s->dbatches[slot], p.ds, 1);
+        fresh_area(s, p.b);
+        p.t_area = rk::now_s();
         std::vector<uint32_t> q(p.qd.size()), k(p.qd.size());
         for (size_t i = 0; i < p.qd.size(); ++i) {
             q[i] = p.qd[i].q;
@@ -200,7 +302,9 @@ void start_step(rk_server* s, Step& p, int slot)
         }
         p.st = rk::batch_fill_device(p.b, q.data(), k.data(), q.size());
     } else {
-        p.b = batch_for(s->batches[slot], p.ds);
+        p.b = batch_for(s, s->batches[slot], p.ds, 0);
+        fresh_area(s, p.b);
+        p.t_area = rk::now_s();
         p.st = rk::batch_fill(p.b, p.qd.data(), p.qd.size());
         if (p.st == RK_OK) {
             std::lock_guard<std::recursive_mutex> lk(p.ds->ctx->mu);
@@ -213,29 +317,66 @@ void start_step(rk_server* s, Step& p, int slot)
     p.t_enqueued = rk::now_s();
 }
 
-// Waits for the step's run, collects its ids (queries that outgrew a
-// segment run again alone) and publishes them.
+// Waits for the step's run and publishes its results: by reference into the
+// batch's landing area when every query fit its segments, else merged with
+// the re-runs of the queries that outgrew one (batch_collect) into an array
+// of the step's own.
 void finish_step(rk_server* s, Step& p)
 {
-    std::vector<uint64_t> off(p.req.size() + 1);
-    std::vector<int32_t>* ids_buf = nullptr;
-    {
-        std::lock_guard<std::mutex> pl(s->pool->mu);
-        if (!s->pool->free.empty()) {
-            ids_buf = s->pool->free.back();
-            s->pool->free.pop_back();
-        }
-    }
-    if (!ids_buf) ids_buf = new std::vector<int32_t>();
-    std::vector<int32_t>& ids = *ids_buf; // keeps its size: a step refills it without zeroing
+    const size_t nr = p.req.size();
+    std::vector<Done> fin(nr);
     rk_status st = p.st;
     const double t_finish = rk::now_s();
     double t_waited = t_finish;
+    uint64_t n_ids = 0;
     if (p.enqueued) {
         st = rk_batch_wait(p.b, nullptr);
         t_waited = rk::now_s();
-        if (st == RK_OK || st == RK_ERR_CAPACITY)
-            st = rk::batch_collect(p.b, batch_for(s->rbatches, p.ds), p.qd.data(), p.qd.size(), ids, off);
+        std::vector<rk::IdRange> ranges;
+        std::vector<uint32_t> redo;
+        if (st == RK_OK || st == RK_ERR_CAPACITY) {
+            std::lock_guard<std::recursive_mutex> lk(p.ds->ctx->mu);
+            st = rk::result_ranges(p.b, ranges, &redo);
+        }
+        if (st == RK_OK && redo.empty()) {
+            const std::shared_ptr<const void> area = p.b->land_host;
+            for (size_t i = 0; i < nr; ++i) {
+                fin[i].ids = ranges[i].src;
+                fin[i].n = ranges[i].n;
+                // a landed range lives in the area; every-user ranges in the
+                // dataset, which outlives the server's use of it
+                if (ranges[i].src && p.b->land_host && ranges[i].src >= p.b->land_host->p &&
+                    ranges[i].src < p.b->land_host->p + p.b->land_host->n)
+                    fin[i].keep = area;
+                n_ids += ranges[i].n;
+            }
+        } else if (st == RK_OK) {
+            std::vector<int32_t>* ids_buf = nullptr;
+            {
+                std::lock_guard<std::mutex> pl(s->pool->mu);
+                if (!s->pool->free.empty()) {
+                    ids_buf = s->pool->free.back();
+                    s->pool->free.pop_back();
+                }
+            }
+            if (!ids_buf) ids_buf = new std::vector<int32_t>();
+            std::vector<uint64_t> off(nr + 1);
+            st = rk::batch_collect(p.b, batch_for(s, s->rbatches, p.ds, -1), p.qd.data(), nr, *ids_buf, off);
+            const std::shared_ptr<IdPool> pool = s->pool;
+            const std::shared_ptr<const std::vector<int32_t>> shared(ids_buf, [pool](const std::vector<int32_t>* v) {
+                auto* w = const_cast<std::vector<int32_t>*>(v);
+                std::lock_guard<std::mutex> pl(pool->mu);
+                if (pool->free.size() < kPoolKeep) pool->free.push_back(w);
+                else delete w;
+            });
+            if (st == RK_OK)
+                for (size_t i = 0; i < nr; ++i) {
+                    fin[i].keep = shared;
+                    fin[i].ids = shared->data() + off[i];
+                    fin[i].n = off[i + 1] - off[i];
+                }
+            n_ids = off[nr];
+        }
     }
     const double t_end = rk::now_s();
     if (serve_trace()) {
@@ -243,33 +384,23 @@ void finish_step(rk_server* s, Step& p)
                               1e3 * (t_waited - t_finish), 1e3 * (t_end - t_waited)};
         if (ms[0] > 1.0 || ms[1] > 1.0 || ms[2] > 1.0 || ms[3] > 1.0)
             std::fprintf(stderr,
-                         "rk_serve: slow step: %zu requests, %zu ids; fill %.3f ms, enqueue %.3f ms, "
+                         "rk_serve: slow step: %zu requests, %zu ids; fill %.3f ms (area %.3f), enqueue %.3f ms, "
                          "wait %.3f ms, collect %.3f ms; %.3f ms from enqueue to finish\n",
-                         p.req.size(), (size_t)off[p.req.size()], ms[0], ms[1], ms[2], ms[3],
+                         nr, (size_t)n_ids, ms[0], 1e3 * (p.t_area - p.t_start), ms[1], ms[2], ms[3],
                          1e3 * (t_finish - p.t_enqueued));
     }
-    // the step's ids become one shared array, each request a range of it
-    // (set up before the lock is taken: submitters and pollers share it)
-    const std::shared_ptr<IdPool> pool = s->pool;
-    const std::shared_ptr<const std::vector<int32_t>> shared(ids_buf, [pool](const std::vector<int32_t>* v) {
-        auto* w = const_cast<std::vector<int32_t>*>(v);
-        std::lock_guard<std::mutex> pl(pool->mu);
-        if (pool->free.size() < kPoolKeep) pool->free.push_back(w);
-        else delete w;
-    });
-    std::vector<Done> fin(p.req.size());
-    for (size_t i = 0; i < p.req.size(); ++i) {
+    for (size_t i = 0; i < nr; ++i) {
         fin[i].status = st;
         fin[i].latency = t_end - p.req[i].t_submit;
-        if (st == RK_OK) {
-            fin[i].buf = shared;
-            fin[i].off = off[i];
-            fin[i].n = off[i + 1] - off[i];
+        if (st != RK_OK) {
+            fin[i].keep.reset();
+            fin[i].ids = nullptr;
+            fin[i].n = 0;
         }
     }
     std::lock_guard<std::mutex> lk(s->mu);
     bool freed = false;
-    for (size_t i = 0; i < p.req.size(); ++i) {
+    for (size_t i = 0; i < nr; ++i) {
         s->done[p.req[i].ticket] = std::move(fin[i]);
         s->reserved -= p.req[i].reserve;
         freed |= p.req[i].reserve > 0;
@@ -353,6 +484,7 @@ rk_status rk_serve_start(rk_ctx* ctx, const rk_serve_config* cfg, rk_server** ou
     const unsigned nf = rk::filter_threads(1u << 20);
     for (unsigned i = 0; i < nf; ++i) s->host_threads.emplace_back(host_stage, s);
     s->dev_thread = std::thread(dev_stage, s);
+    s->keeper = std::thread(area_keeper, s->areas);
     *out = s;
     return RK_OK;
 }
@@ -405,7 +537,7 @@ rk_status rk_serve_poll(rk_server* s, uint64_t ticket, int32_t* out_ids, size_t 
     if (d.n > cap) return RK_ERR_CAPACITY; // keep the result for a larger buffer
     s->done.erase(it);
     lk.unlock(); // the copy runs outside the server's lock
-    if (d.n) std::memcpy(out_ids, d.buf->data() + d.off, d.n * sizeof(int32_t));
+    if (d.n) std::memcpy(out_ids, d.ids, d.n * sizeof(int32_t));
     return RK_OK;
 }
 
@@ -421,6 +553,12 @@ rk_status rk_serve_stop(rk_server* s)
     for (auto& t : s->host_threads)
         if (t.joinable()) t.join();
     if (s->dev_thread.joinable()) s->dev_thread.join();
+    {
+        std::lock_guard<std::mutex> lk(s->areas->mu);
+        s->areas->stop = true;
+    }
+    s->areas->cv.notify_all();
+    if (s->keeper.joinable()) s->keeper.join();
     delete s;
     return RK_OK;
 }
