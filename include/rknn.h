@@ -230,7 +230,11 @@ rk_status rk_batch_config(const rk_batch* b, uint32_t* waves_per_query);
 /* ---- serving: continuous batching over resident datasets. ---- */
 typedef struct rk_serve_config {
     uint32_t max_batch;          /* queries per device step (default 256)          */
-    uint64_t memory_budget;      /* device bytes for in-flight work (default 4 GiB) */
+    uint64_t memory_budget;      /* device bytes the server may allocate (default  *
+                                  * 4 GiB): its step and re-run buffers, laid out  *
+                                  * once from the budget; steps take only what     *
+                                  * that layout holds.  Resident datasets and      *
+                                  * pinned host memory are not counted.            */
     double max_wait_s;           /* longest a request waits for batch-mates while   *
                                   * no step is in flight                            */
 } rk_serve_config;
@@ -240,13 +244,19 @@ typedef struct rk_serve_config {
  * before.  A step in flight is finished at once when nothing else is ready. */
 
 rk_status rk_serve_start(rk_ctx* ctx, const rk_serve_config* cfg, rk_server** out);
-/* Thread-safe.  Returns a ticket; RK_ERR_ADMISSION if the request can never
- * fit the budget. */
+/* Thread-safe.  Returns a ticket; RK_ERR_ADMISSION if the budget holds no
+ * step at all.  A request is admitted into a step when the step's layout
+ * holds it; one that no step could hold (a host candidate list longer than a
+ * step's candidate slots, a result beyond the re-run share) completes with
+ * RK_ERR_ADMISSION from rk_serve_poll. */
 rk_status rk_serve_submit(rk_server* s, rk_dataset* ds, uint32_t q, uint32_t k, uint64_t* ticket);
 /* RK_ERR_NOT_READY while in flight; on RK_OK the ticket is retired.
  * latency_s = completion - submission. */
 rk_status rk_serve_poll(rk_server* s, uint64_t ticket, int32_t* out_ids, size_t cap,
                         size_t* n_out, double* latency_s);
+/* Device bytes the server's buffers hold (at most memory_budget) and the
+ * queries a step takes at most.  No reference counterpart. */
+rk_status rk_serve_info(rk_server* s, uint64_t* device_bytes, uint32_t* step_queries);
 rk_status rk_serve_stop(rk_server* s);
 
 #ifdef __cplusplus

@@ -45,6 +45,11 @@ uint64_t env_count(const char* name, uint64_t dflt)
 // memory per batch, and the queries that outgrow them run again
 // (batch_collect).
 uint64_t pool_budget() { return env_count("RK_BATCH_POOL_IDS", kPoolBudget); }
+} // namespace
+namespace rk {
+uint64_t batch_pool_ids() { return pool_budget(); }
+} // namespace rk
+namespace {
 uint64_t dfilter_slots() { return std::min<uint64_t>(env_count("RK_DFILTER_SLOTS", kDfBatchCand), kDfBatchCand); }
 
 rk_status build_graph(rk_batch* b)
@@ -320,18 +325,26 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
     hipStream_t st = c->stream;
-    RK_HIP(c, b->rec.reserve(all.size()));
-    RK_HIP(c, b->orig.reserve(all.size()));
-    RK_HIP(c, b->qres.reserve(2 * (uint64_t)nq));
-    RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
-    // result segment per query: the whole budget split evenly, never more than the users
-    uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? pool_budget() / nq : 1));
-    seg_cap = std::max<uint64_t>(seg_cap, std::min<uint64_t>(ds->nu, b->seg_min));
-    seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
-    RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
+    uint64_t seg_cap = 0;
+    if (b->cap_nq) { // a fixed layout: nothing grows
+        if (nq > b->cap_nq || all.size() > b->cap_nq * b->cap_cand)
+            return fail(c, RK_ERR_ADMISSION, "the step exceeds the batch's fixed layout (its memory budget)");
+        seg_cap = b->cap_seg;
+    } else {
+        RK_HIP(c, b->rec.reserve(all.size()));
+        RK_HIP(c, b->orig.reserve(all.size()));
+        RK_HIP(c, b->qres.reserve(2 * (uint64_t)nq));
+        RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
+        // result segment per query: the whole budget split evenly, never more than the users
+        seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? pool_budget() / nq : 1));
+        seg_cap = std::max<uint64_t>(seg_cap, std::min<uint64_t>(ds->nu, b->seg_min));
+        seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
+        RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
+        const rk_status ls = ensure_landing(b, initial_landing(nq));
+        if (ls != RK_OK) return ls;
+    }
     {
-        rk_status ls = ensure_landing(b, initial_landing(nq));
-        if (ls == RK_OK) ls = ensure_qres(b, nq);
+        const rk_status ls = ensure_qres(b, nq);
         if (ls != RK_OK) return ls;
     }
     // the inputs into pinned staging (every previous run of this batch is
@@ -344,7 +357,7 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
         std::memcpy(b->s_wb.p, wb.data(), nq * sizeof(double2));
     }
     if (!all.empty()) std::memcpy(b->s_cand.p, all.data(), all.size() * sizeof(uint32_t));
-    if (b->resident) { // to HBM, once
+    if (b->resident) { // to HBM, once (a fixed layout is never resident)
         RK_HIP(c, b->q.reserve(nq));
         RK_HIP(c, b->wb.reserve(nq));
         RK_HIP(c, b->cand.reserve(all.size()));
@@ -392,28 +405,38 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
     b->h_dn.assign(nq, 0);
 
     // candidate segment per query: every facility at most, 4096 slots at most
+    // (the fixed layout's slots at most)
     b->cand_cap = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ds->nf, dfilter_slots()));
+    if (b->cap_nq) b->cand_cap = std::min(b->cand_cap, b->cap_cand);
     const uint64_t slots = std::max<uint64_t>(1, (uint64_t)nq * b->cand_cap);
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
     hipStream_t st = c->stream;
-    RK_HIP(c, b->dq.reserve(nq));
-    RK_HIP(c, b->dk.reserve(nq));
-    RK_HIP(c, b->dn.reserve(nq));
-    RK_HIP(c, b->q.reserve(nq));
-    RK_HIP(c, b->wb.reserve(nq));
-    RK_HIP(c, b->cand.reserve(slots));
-    RK_HIP(c, b->rec.reserve(slots));
-    RK_HIP(c, b->orig.reserve(slots));
-    RK_HIP(c, b->qres.reserve(2 * (uint64_t)nq));
-    RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
-    uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? pool_budget() / nq : 1));
-    seg_cap = std::max<uint64_t>(seg_cap, std::min<uint64_t>(ds->nu, b->seg_min));
-    seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
-    RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
+    uint64_t seg_cap = 0;
+    if (b->cap_nq) { // a fixed layout: nothing grows
+        if (nq > b->cap_nq)
+            return fail(c, RK_ERR_ADMISSION, "the step exceeds the batch's fixed layout (its memory budget)");
+        seg_cap = b->cap_seg;
+    } else {
+        RK_HIP(c, b->dq.reserve(nq));
+        RK_HIP(c, b->dk.reserve(nq));
+        RK_HIP(c, b->dn.reserve(nq));
+        RK_HIP(c, b->q.reserve(nq));
+        RK_HIP(c, b->wb.reserve(nq));
+        RK_HIP(c, b->cand.reserve(slots));
+        RK_HIP(c, b->rec.reserve(slots));
+        RK_HIP(c, b->orig.reserve(slots));
+        RK_HIP(c, b->qres.reserve(2 * (uint64_t)nq));
+        RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
+        seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, nq ? pool_budget() / nq : 1));
+        seg_cap = std::max<uint64_t>(seg_cap, std::min<uint64_t>(ds->nu, b->seg_min));
+        seg_cap = std::min<uint64_t>(seg_cap, 0x7fffffffull);
+        RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
+        const rk_status ls = ensure_landing(b, initial_landing(nq));
+        if (ls != RK_OK) return ls;
+    }
     {
-        rk_status ls = ensure_landing(b, initial_landing(nq));
-        if (ls == RK_OK) ls = ensure_qres(b, nq);
+        const rk_status ls = ensure_qres(b, nq);
         if (ls != RK_OK) return ls;
     }
     RK_HIP(c, b->s_dq.reserve(std::max<size_t>(nq, 1), hipHostMallocCoherent));
@@ -432,36 +455,53 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
     return RK_OK;
 }
 
-rk_status batch_presize(rk_batch* b, size_t nq, bool dfilter)
+uint64_t batch_fixed_bytes(uint64_t nq, uint64_t seg, uint64_t cand)
 {
-    rk_dataset* ds = b->ds;
-    rk_ctx* c = ds->ctx;
-    if (b->pending) return fail(c, RK_ERR_INVALID_ARG, "the batch's last run has not been waited for");
-    if (!nq || nq > (1u << 24)) return RK_OK;
+    const uint64_t per_query = sizeof(QDev) + sizeof(double2) + 2 * sizeof(uint32_t) + kQueryStats * sizeof(uint32_t) +
+                               3 * sizeof(uint32_t);
+    const uint64_t per_slot = sizeof(uint32_t) + sizeof(CandRec) + sizeof(float2);
+    return nq * per_query + nq * cand * per_slot + 2 * nq * seg * sizeof(int32_t) + 2 * sizeof(unsigned long long);
+}
+
+rk_status batch_fix(rk_batch* b, uint64_t nq, uint64_t seg, uint32_t cand)
+{
+    rk_ctx* c = b->ds->ctx;
+    if (!nq || !seg || !cand || nq * seg > kMaxLandIds) return fail(c, RK_ERR_INVALID_ARG, "an empty or oversized layout");
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     RK_HIP(c, hipSetDevice(c->device));
     RK_HIP(c, b->q.reserve(nq));
     RK_HIP(c, b->wb.reserve(nq));
-    RK_HIP(c, b->qres.reserve(2 * (uint64_t)nq));
-    RK_HIP(c, b->qstat.reserve((uint64_t)nq * kQueryStats));
-    RK_HIP(c, b->s_q.reserve(nq, hipHostMallocCoherent));
-    RK_HIP(c, b->s_wb.reserve(nq, hipHostMallocCoherent));
-    if (dfilter) {
-        const uint64_t slots = (uint64_t)nq * std::max<uint64_t>(1, std::min<uint64_t>(ds->nf, dfilter_slots()));
-        RK_HIP(c, b->dq.reserve(nq));
-        RK_HIP(c, b->dk.reserve(nq));
-        RK_HIP(c, b->dn.reserve(nq));
-        RK_HIP(c, b->cand.reserve(slots));
-        RK_HIP(c, b->rec.reserve(slots));
-        RK_HIP(c, b->orig.reserve(slots));
-        RK_HIP(c, b->s_dq.reserve(nq, hipHostMallocCoherent));
-        RK_HIP(c, b->s_dk.reserve(nq, hipHostMallocCoherent));
-    }
-    uint64_t seg_cap = std::max<uint64_t>(1, std::min<uint64_t>(ds->nu, pool_budget() / nq));
-    RK_HIP(c, b->seg.reserve(std::max<uint64_t>(1, nq * seg_cap)));
+    RK_HIP(c, b->qres.reserve(2 * nq));
+    RK_HIP(c, b->qstat.reserve(nq * kQueryStats));
+    RK_HIP(c, b->dq.reserve(nq));
+    RK_HIP(c, b->dk.reserve(nq));
+    RK_HIP(c, b->dn.reserve(nq));
+    RK_HIP(c, b->cand.reserve(nq * cand));
+    RK_HIP(c, b->rec.reserve(nq * cand));
+    RK_HIP(c, b->orig.reserve(nq * cand));
+    RK_HIP(c, b->seg.reserve(nq * seg));
+    // the device side holds every result that fits its segment; the host
+    // side (pinned host memory, not device memory) grows with the results
+    RK_HIP(c, b->dland.reserve(nq * seg));
     rk_status s = ensure_landing(b, initial_landing(nq));
     if (s == RK_OK) s = ensure_qres(b, nq);
-    return s;
+    if (s != RK_OK) return s;
+    RK_HIP(c, b->s_q.reserve(nq, hipHostMallocCoherent));
+    RK_HIP(c, b->s_wb.reserve(nq, hipHostMallocCoherent));
+    RK_HIP(c, b->s_dq.reserve(nq, hipHostMallocCoherent));
+    RK_HIP(c, b->s_dk.reserve(nq, hipHostMallocCoherent));
+    RK_HIP(c, b->s_cand.reserve(nq * cand, hipHostMallocCoherent));
+    b->cap_nq = nq;
+    b->cap_seg = seg;
+    b->cap_cand = cand;
+    return RK_OK;
+}
+
+uint64_t batch_device_bytes(const rk_batch* b)
+{
+    return b->q.bytes() + b->wb.bytes() + b->qres.bytes() + b->qstat.bytes() + b->dq.bytes() + b->dk.bytes() +
+           b->dn.bytes() + b->cand.bytes() + b->rec.bytes() + b->orig.bytes() + b->seg.bytes() + b->dland.bytes() +
+           b->land_ctr.bytes();
 }
 
 // After a device-filtered run: the per-query views (trivial flag, candidate
@@ -680,7 +720,9 @@ rk_status rk_batch_wait(rk_batch* b, double* device_seconds)
         (void)landed_extent(b, &short_area, &need);
         if (!short_area) break;
         if (attempt > 0) return fail(c, RK_ERR_INTERNAL, "a repeated run fell short of its landing area");
-        rk_status s = ensure_landing(b, need + need / 2);
+        // (a fixed layout's device side holds every result already: only
+        // its host side grows)
+        rk_status s = ensure_landing(b, b->cap_nq ? std::min<uint64_t>(need + need / 2, b->dland.n) : need + need / 2);
         if (s != RK_OK) return s;
         set_view(b, b->view.seg_cap);
         if (b->exec) { // the graph holds the old area's addresses: capture it again
@@ -837,26 +879,46 @@ rk_status batch_collect(rk_batch* b, rk_batch* r, const rk_query_desc* qd, size_
             }
     std::vector<int32_t> rids;
     std::vector<uint64_t> roff;
+    r->ds = ds;
     r->direct = true;
-    r->seg_min = need;
-    if (host) {
-        st = batch_fill(r, rq.data(), rq.size());
-    } else {
-        std::vector<uint32_t> q(rq.size()), k(rq.size());
-        for (size_t j = 0; j < rq.size(); ++j) {
-            q[j] = rq[j].q;
-            k[j] = rq[j].k;
+    // the re-run queries [j0, j1) in one run; segments hold the counts the
+    // step measured and host lists never overflow, so nothing is left over
+    auto rerun = [&](size_t j0, size_t j1, uint64_t seg_need) -> rk_status {
+        if (r->cap_nq && seg_need > r->cap_seg)
+            return fail(ds->ctx, RK_ERR_ADMISSION, "a result larger than the memory budget's re-run share");
+        r->seg_min = seg_need;
+        rk_status s;
+        if (host) {
+            s = batch_fill(r, rq.data() + j0, j1 - j0);
+        } else {
+            std::vector<uint32_t> q(j1 - j0), k(j1 - j0);
+            for (size_t j = j0; j < j1; ++j) {
+                q[j - j0] = rq[j].q;
+                k[j - j0] = rq[j].k;
+            }
+            s = batch_fill_device(r, q.data(), k.data(), q.size());
         }
-        st = batch_fill_device(r, q.data(), k.data(), q.size());
-    }
-    if (st == RK_OK) st = rk_batch_run(r, nullptr);
-    if (st == RK_OK || st == RK_ERR_CAPACITY) {
+        if (s == RK_OK) s = rk_batch_run(r, nullptr);
+        if (s != RK_OK && s != RK_ERR_CAPACITY) return s;
+        std::vector<int32_t> ids1;
+        std::vector<uint64_t> off1;
         std::vector<uint32_t> again;
-        st = batch_fetch_partial(r, rids, roff, again);
-        // the segments hold the measured counts and host lists never
-        // overflow, so nothing is left over
-        if (st == RK_OK && !again.empty())
+        s = batch_fetch_partial(r, ids1, off1, again);
+        if (s == RK_OK && !again.empty())
             return fail(ds->ctx, RK_ERR_INTERNAL, "a re-run query outgrew segments sized to its count");
+        if (s != RK_OK) return s;
+        if (roff.empty()) roff.push_back(0);
+        for (size_t j = 0; j < j1 - j0; ++j) {
+            rids.insert(rids.end(), ids1.begin() + off1[j], ids1.begin() + off1[j + 1]);
+            roff.push_back(rids.size());
+        }
+        return RK_OK;
+    };
+    if (r->cap_nq) { // a fixed layout: one query at a time, within its segment
+        for (size_t j = 0; j < rq.size() && st == RK_OK; ++j)
+            st = rerun(j, j + 1, b->h_qcount()[redo[j]]);
+    } else {
+        st = rerun(0, rq.size(), need);
     }
     if (st != RK_OK) return st;
     std::vector<int32_t> all;

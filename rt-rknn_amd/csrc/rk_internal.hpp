@@ -218,6 +218,15 @@ struct rk_batch {
     rk::HostBuf<rk::QDev> s_q;
     rk::HostBuf<double2> s_wb;
     rk::HostBuf<uint32_t> s_cand, s_dq, s_dk;
+    // a fixed layout (the serving loop's objects, sized from its memory
+    // budget by batch_fix): at most cap_nq queries per fill, result segments
+    // of cap_seg ids, cap_cand candidate slots per query (device lists, and
+    // the records of host lists: a fill's lists hold cap_nq * cap_cand
+    // candidates at most), a landing area of cap_nq * cap_seg ids; device
+    // buffers allocated once, never grown
+    uint64_t cap_nq = 0;
+    uint64_t cap_seg = 0;
+    uint32_t cap_cand = 0;
     rk::BatchDev view{};
     uint32_t* h_qcount() const { return h_qres.p; }
     uint32_t* h_qland() const { return h_qres.p + nq; }
@@ -289,11 +298,15 @@ rk_status batch_collect(rk_batch* b, rk_batch* redo, const rk_query_desc* qd, si
                         std::vector<uint64_t>& off);
 // the device work of one step of a batch (filter stages when device-filtered)
 void batch_launch(rk_batch* b, hipStream_t st);
-// the per-query buffers of a batch that will take up to nq queries (of the
-// device-filtered kind when dfilter), allocated ahead (a fill's growth
-// allocates, and device allocations wait for the device); candidate lists
-// of host-filtered queries still grow with their sizes
-rk_status batch_presize(rk_batch* b, size_t nq, bool dfilter);
+// device bytes of a fixed layout (nq queries, seg ids per segment, cand
+// candidate slots per query), and the layout given to a batch: every device
+// buffer allocated at once, exactly
+uint64_t batch_fixed_bytes(uint64_t nq, uint64_t seg, uint64_t cand);
+// result ids a batch's segments hold in all (2^26, or RK_BATCH_POOL_IDS)
+uint64_t batch_pool_ids();
+rk_status batch_fix(rk_batch* b, uint64_t nq, uint64_t seg, uint32_t cand);
+// device bytes a batch holds
+uint64_t batch_device_bytes(const rk_batch* b);
 // a range of result ids: n ids at src (to be copied to dst)
 struct IdRange {
     int32_t* dst;

@@ -12,9 +12,12 @@
 //     a dataset (device-filtered and host-filtered requests in separate
 //     steps), refilling that dataset's batch object in place.
 // RK_FILTER=host sends every request through the host filter.
-// Admission is by memory: each request reserves an estimate of the device
-// bytes it needs while in flight; a request waits until the reservation fits
-// the budget, and one that could never fit is rejected at submission.
+// Admission is by memory: the server's device memory is three batch objects
+// (two step slots that alternate, one for re-runs), shared by every dataset
+// and laid out once from memory_budget (45% per step slot, 10% for re-runs);
+// a step takes only the requests its slot's layout holds (queries, candidate
+// slots, result segments), so the server never allocates beyond the budget,
+// and a request is rejected at submission when no step could hold it.
 // Results do not depend on batch-mates: each query's set is computed only
 // from its own candidates.
 #include <algorithm>
@@ -38,7 +41,6 @@ struct Request {
     uint64_t ticket;
     rk_dataset* ds;
     uint32_t q, k;
-    uint64_t reserve;
     double t_submit;
     bool dev;                   // filtered on the device, inside the step
     std::vector<uint32_t> cand; // host-filtered requests: filled by the host stage
@@ -113,6 +115,16 @@ static void area_keeper(std::shared_ptr<AreaPool> pool)
     }
 }
 
+// The device-memory layout of a server's batch objects (see the header):
+// queries per step, ids per result segment and candidate slots per query of
+// the step slots, and the re-run object's segment and candidate slots (one
+// query at a time).
+struct ServeLayout {
+    bool ok = false;
+    uint64_t nq = 0, seg = 0, re_seg = 0;
+    uint32_t cand = 0, re_cand = 0;
+};
+
 struct rk_server {
     // first members: destroyed last, after the finished results that return to them
     std::shared_ptr<IdPool> pool = std::make_shared<IdPool>();
@@ -129,33 +141,55 @@ struct rk_server {
     size_t n_ready = 0;
     std::unordered_map<uint64_t, Done> done;
     uint64_t next_ticket = 1;
-    uint64_t reserved = 0;
     bool stop = false;
+    ServeLayout layout;
     std::vector<std::thread> host_threads; // filter workers
     std::thread dev_thread;
     std::thread keeper;                    // allocates landing areas ahead of need
     std::condition_variable cv_dev;
-    // device stage only, two of each kind per dataset: a step is filled and
-    // enqueued on one while the previous step's run, on the other, finishes
-    std::map<rk_dataset*, std::unique_ptr<rk_batch>> batches[2];  // host-filtered steps
-    std::map<rk_dataset*, std::unique_ptr<rk_batch>> dbatches[2]; // device-filtered steps
-    std::map<rk_dataset*, std::unique_ptr<rk_batch>> rbatches; // re-runs of the queries that outgrew a segment
-    bool host_filter_only = false;                             // RK_FILTER=host
+    // device stage only: two step slots (a step is filled and enqueued on
+    // one while the previous step's run, on the other, finishes) and the
+    // re-run object, for every dataset and both filter sides
+    std::unique_ptr<rk_batch> slots[2];
+    std::unique_ptr<rk_batch> rerun;
+    bool host_filter_only = false; // RK_FILTER=host
 };
 
 namespace {
 
-uint64_t estimate_bytes(const rk_dataset* ds, uint32_t k, bool dev)
+// Candidate slots per query of a step slot: a device list of k <= 64 is
+// about 5k + 25 facilities (20% more than the host filter's), so 16 k + 512
+// at the largest k leaves room to spare; a longer list runs again through
+// the host filter.
+constexpr uint32_t kServeCand = 16 * rk::kDfMaxK + 512;
+constexpr uint64_t kServeSegMin = 4096, kServeSegMax = 1u << 16; // result segment ids
+
+ServeLayout serve_layout(uint64_t budget, uint32_t max_batch)
 {
-    // candidate records + indices (|C| grows roughly linearly in k; a
-    // device-filtered request holds a whole candidate segment), result
-    // pool entries (12 B each) for the expected result size k*N_u/N_f with
-    // slack, and one tile list entry per tile of the zone's neighbourhood
-    const uint64_t nf = ds->nf ? ds->nf : 1;
-    const uint64_t cand = dev ? std::min<uint64_t>(nf, rk::kDfBatchCand) : 64ull * k + 256;
-    uint64_t results = 8ull * k * ds->nu / nf + 4096;
-    results = std::min<uint64_t>(results, ds->nu + 64);
-    return cand * 28 + results * 16 + 8 * 4096 + 4096;
+    ServeLayout L;
+    const uint64_t step_share = budget / 100 * 45, re_share = budget / 100 * 10;
+    for (uint64_t nq = max_batch; nq >= 1; nq = nq > 3 ? nq * 3 / 4 : nq - 1) {
+        const uint64_t fixed = rk::batch_fixed_bytes(nq, 0, kServeCand);
+        if (fixed >= step_share) continue;
+        // segments: the pool split over the step's queries, at most what the
+        // share leaves (kServeSegMin at least, unless the pool is smaller)
+        const uint64_t want = std::max<uint64_t>(1, std::min<uint64_t>(kServeSegMax, rk::batch_pool_ids() / nq));
+        const uint64_t seg = std::min<uint64_t>(want, (step_share - fixed) / (2 * sizeof(int32_t) * nq));
+        if (seg >= std::min(kServeSegMin, want)) {
+            L.nq = nq;
+            L.seg = seg;
+            L.cand = kServeCand;
+            break;
+        }
+    }
+    // the re-run object: one query, half its share for candidates, half for
+    // the segment and its landing twin
+    const uint64_t per_slot = sizeof(uint32_t) + sizeof(rk::CandRec) + sizeof(float2);
+    const uint64_t half = re_share > 4096 ? (re_share - 4096) / 2 : 0;
+    L.re_cand = (uint32_t)std::min<uint64_t>(half / per_slot, 1u << 30);
+    L.re_seg = std::min<uint64_t>(half / (2 * sizeof(int32_t)), 1ull << 30);
+    L.ok = L.nq > 0 && L.re_cand > 0 && L.re_seg > 0;
+    return L;
 }
 
 // caller holds s->mu
@@ -179,11 +213,10 @@ void host_stage(rk_server* s)
 {
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
-        s->cv.wait(lk, [&] { return s->stop || (!s->waiting.empty() && s->reserved + s->waiting.front().reserve <= s->cfg.memory_budget); });
+        s->cv.wait(lk, [&] { return s->stop || !s->waiting.empty(); });
         if (s->stop) return;
         Request r = std::move(s->waiting.front());
         s->waiting.pop_front();
-        s->reserved += r.reserve;
         if (!s->waiting.empty()) s->cv.notify_one();
         if (!r.dev) {
             lk.unlock();
@@ -196,21 +229,24 @@ void host_stage(rk_server* s)
 
 std::shared_ptr<rk::PinnedIds> pooled_area(const std::shared_ptr<AreaPool>& pool, size_t n);
 
-// The batch object of a dataset, made on first use; a step object's
-// per-query buffers are sized for a full step (max_batch queries of its
-// kind) at once, so that no later step waits on a device allocation.
-rk_batch* batch_for(rk_server* s, std::map<rk_dataset*, std::unique_ptr<rk_batch>>& m, rk_dataset* ds,
-                    int kind)
+// A step slot (0, 1) or the re-run object (2), made on first use with its
+// fixed layout, then pointed at the dataset of each step.
+rk_batch* object_for(rk_server* s, int which, rk_dataset* ds, rk_status* st)
 {
-    std::unique_ptr<rk_batch>& b = m[ds];
+    std::unique_ptr<rk_batch>& b = which < 2 ? s->slots[which] : s->rerun;
+    *st = RK_OK;
     if (!b) {
-        b.reset(new rk_batch());
-        b->ds = ds;
-        b->direct = true;
+        std::unique_ptr<rk_batch> n(new rk_batch());
+        n->ds = ds;
+        n->direct = true;
         const std::shared_ptr<AreaPool> pool = s->areas;
-        b->area_source = [pool](size_t n) { return pooled_area(pool, n); };
-        if (kind >= 0) (void)rk::batch_presize(b.get(), s->cfg.max_batch, kind == 1); // a failure shows at the fill
+        n->area_source = [pool](size_t k) { return pooled_area(pool, k); };
+        const ServeLayout& L = s->layout;
+        *st = which < 2 ? rk::batch_fix(n.get(), L.nq, L.seg, L.cand) : rk::batch_fix(n.get(), 1, L.re_seg, L.re_cand);
+        if (*st != RK_OK) return nullptr;
+        b = std::move(n);
     }
+    b->ds = ds;
     return b.get();
 }
 
@@ -291,10 +327,14 @@ void fresh_area(rk_server* s, rk_batch* b)
 void start_step(rk_server* s, Step& p, int slot)
 {
     p.t_start = rk::now_s();
+    p.b = object_for(s, slot, p.ds, &p.st);
+    if (!p.b) {
+        p.t_area = p.t_filled = p.t_enqueued = rk::now_s();
+        return;
+    }
+    fresh_area(s, p.b);
+    p.t_area = rk::now_s();
     if (p.dev) {
-        p.b = batch_for(s, s->dbatches[slot], p.ds, 1);
-        fresh_area(s, p.b);
-        p.t_area = rk::now_s();
         std::vector<uint32_t> q(p.qd.size()), k(p.qd.size());
         for (size_t i = 0; i < p.qd.size(); ++i) {
             q[i] = p.qd[i].q;
@@ -302,9 +342,6 @@ void start_step(rk_server* s, Step& p, int slot)
         }
         p.st = rk::batch_fill_device(p.b, q.data(), k.data(), q.size());
     } else {
-        p.b = batch_for(s, s->batches[slot], p.ds, 0);
-        fresh_area(s, p.b);
-        p.t_area = rk::now_s();
         p.st = rk::batch_fill(p.b, p.qd.data(), p.qd.size());
         if (p.st == RK_OK) {
             std::lock_guard<std::recursive_mutex> lk(p.ds->ctx->mu);
@@ -361,7 +398,8 @@ void finish_step(rk_server* s, Step& p)
             }
             if (!ids_buf) ids_buf = new std::vector<int32_t>();
             std::vector<uint64_t> off(nr + 1);
-            st = rk::batch_collect(p.b, batch_for(s, s->rbatches, p.ds, -1), p.qd.data(), nr, *ids_buf, off);
+            rk_batch* r = object_for(s, 2, p.ds, &st);
+            if (r) st = rk::batch_collect(p.b, r, p.qd.data(), nr, *ids_buf, off);
             const std::shared_ptr<IdPool> pool = s->pool;
             const std::shared_ptr<const std::vector<int32_t>> shared(ids_buf, [pool](const std::vector<int32_t>* v) {
                 auto* w = const_cast<std::vector<int32_t>*>(v);
@@ -399,14 +437,19 @@ void finish_step(rk_server* s, Step& p)
         }
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    bool freed = false;
-    for (size_t i = 0; i < nr; ++i) {
-        s->done[p.req[i].ticket] = std::move(fin[i]);
-        s->reserved -= p.req[i].reserve;
-        freed |= p.req[i].reserve > 0;
-    }
-    if (freed && !s->waiting.empty()) s->cv.notify_all();
+    for (size_t i = 0; i < nr; ++i) s->done[p.req[i].ticket] = std::move(fin[i]);
     s->cv_done.notify_all();
+}
+
+// A request no step can hold (its host candidate list alone outgrows a
+// step's layout) is finished at once with RK_ERR_ADMISSION.  Caller holds
+// s->mu.
+void refuse(rk_server* s, Request&& r)
+{
+    Done d;
+    d.status = RK_ERR_ADMISSION;
+    d.latency = rk::now_s() - r.t_submit;
+    s->done[r.ticket] = std::move(d);
 }
 
 // The device stage keeps up to two steps alive: it fills and enqueues the
@@ -436,12 +479,27 @@ void dev_stage(rk_server* s)
                 s->cv_dev.wait_for(lk, std::chrono::microseconds(200));
             }
         }
-        // one step: the queue (dataset, filter side) whose head waited longest
+        // one step: the queue (dataset, filter side) whose head waited longest,
+        // as many requests as a step slot's layout holds (queries; for host
+        // lists also candidates)
         std::deque<Request>& qu = oldest_ready(s);
         std::unique_ptr<Step> cur(new Step());
         cur->ds = qu.front().ds;
         cur->dev = qu.front().dev;
-        const size_t take = std::min<size_t>(qu.size(), s->cfg.max_batch);
+        size_t take = std::min<size_t>(qu.size(), std::min<uint64_t>(s->cfg.max_batch, s->layout.nq));
+        if (!cur->dev) {
+            const uint64_t room = s->layout.nq * s->layout.cand;
+            uint64_t sum = 0;
+            size_t t = 0;
+            while (t < take && sum + qu[t].cand.size() <= room) sum += qu[t++].cand.size();
+            if (t == 0) { // the head's list alone outgrows a step
+                refuse(s, std::move(qu.front()));
+                qu.pop_front();
+                --s->n_ready;
+                continue;
+            }
+            take = t;
+        }
         cur->req.reserve(take);
         for (size_t i = 0; i < take; ++i) {
             cur->req.push_back(std::move(qu.front()));
@@ -481,6 +539,7 @@ rk_status rk_serve_start(rk_ctx* ctx, const rk_serve_config* cfg, rk_server** ou
     }
     const char* env = std::getenv("RK_FILTER");
     s->host_filter_only = env && std::strcmp(env, "host") == 0;
+    s->layout = serve_layout(s->cfg.memory_budget, s->cfg.max_batch);
     const unsigned nf = rk::filter_threads(1u << 20);
     for (unsigned i = 0; i < nf; ++i) s->host_threads.emplace_back(host_stage, s);
     s->dev_thread = std::thread(dev_stage, s);
@@ -494,22 +553,19 @@ rk_status rk_serve_submit(rk_server* s, rk_dataset* ds, uint32_t q, uint32_t k, 
     if (!s || !ds || !ticket) return RK_ERR_INVALID_ARG;
     if (ds->ctx != s->ctx) return rk::fail(s->ctx, RK_ERR_INVALID_ARG, "dataset belongs to another context");
     if (q >= ds->nf || k == 0) return rk::fail(s->ctx, RK_ERR_INVALID_ARG, "q must be < N_f and k >= 1");
+    if (!s->layout.ok)
+        return rk::fail(s->ctx, RK_ERR_ADMISSION, "the serving memory budget holds no step; no request can run");
     Request r;
     r.ds = ds;
     r.q = q;
     r.k = k;
     r.dev = !s->host_filter_only && k <= rk::kDfMaxK;
-    r.reserve = estimate_bytes(ds, k, r.dev);
-    if (r.reserve > s->cfg.memory_budget)
-        return rk::fail(s->ctx, RK_ERR_ADMISSION, "request can never fit the serving memory budget");
     r.t_submit = rk::now_s();
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stop) return RK_ERR_INVALID_ARG;
     r.ticket = s->next_ticket++;
     *ticket = r.ticket;
-    if (r.dev && s->waiting.empty() && s->reserved + r.reserve <= s->cfg.memory_budget) {
-        // nothing to filter on the host: admitted here, straight to the device stage
-        s->reserved += r.reserve;
+    if (r.dev) { // nothing to filter on the host: straight to the device stage
         push_ready(s, std::move(r));
         return RK_OK;
     }
@@ -538,6 +594,23 @@ rk_status rk_serve_poll(rk_server* s, uint64_t ticket, int32_t* out_ids, size_t 
     s->done.erase(it);
     lk.unlock(); // the copy runs outside the server's lock
     if (d.n) std::memcpy(out_ids, d.ids, d.n * sizeof(int32_t));
+    return RK_OK;
+}
+
+rk_status rk_serve_info(rk_server* s, uint64_t* device_bytes, uint32_t* step_queries)
+{
+    if (!s) return RK_ERR_INVALID_ARG;
+    if (step_queries) *step_queries = (uint32_t)std::min<uint64_t>(s->layout.nq, s->cfg.max_batch);
+    if (device_bytes) {
+        // the batch objects belong to the device stage; read them under the
+        // context lock, which their fills and runs take
+        std::lock_guard<std::recursive_mutex> lk(s->ctx->mu);
+        uint64_t b = 0;
+        for (const auto& o : s->slots)
+            if (o) b += rk::batch_device_bytes(o.get());
+        if (s->rerun) b += rk::batch_device_bytes(s->rerun.get());
+        *device_bytes = b;
+    }
     return RK_OK;
 }
 

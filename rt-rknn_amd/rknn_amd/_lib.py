@@ -98,6 +98,7 @@ SIGNATURES = {
     "rk_serve_start": (ST, [P, C.POINTER(rk_serve_config), C.POINTER(P)]),
     "rk_serve_submit": (ST, [P, P, C.c_uint32, C.c_uint32, U64P]),
     "rk_serve_poll": (ST, [P, C.c_uint64, I32P, C.c_size_t, SZP, C.POINTER(C.c_double)]),
+    "rk_serve_info": (ST, [P, U64P, U32P]),
     "rk_serve_stop": (ST, [P]),
     # rknn_io.h
     "rk_load_text": (ST, [C.c_char_p, C.POINTER(F32P), C.POINTER(F32P), C.POINTER(I32P), U64P,

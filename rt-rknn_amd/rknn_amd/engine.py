@@ -348,6 +348,12 @@ class Server:
         self.ctx.check(st)
         return out[: n.value].copy(), lat.value
 
+    def info(self):
+        """(device bytes the server's buffers hold, queries a step takes at most)."""
+        b, n = C.c_uint64(), C.c_uint32()
+        self.ctx.check(self._lib.rk_serve_info(self.h, C.byref(b), C.byref(n)))
+        return b.value, n.value
+
     def stop(self):
         if self.h:
             self._lib.rk_serve_stop(self.h)

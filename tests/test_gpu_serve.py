@@ -103,3 +103,54 @@ def test_admission_throttles_but_completes(rk, ctx, datasets):
         assert done == 40
     finally:
         srv.stop()
+
+
+def device_free_bytes():
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_budget_bounds_device_memory(rk, ctx, datasets):
+    """A tight memory budget: the server lays its buffers out within it (steps
+    of fewer queries), every request -- both datasets, both filter sides --
+    still completes with its result alone, and the device's free memory
+    drops by no more than the budget plus what the HIP runtime takes for the
+    server's streams and events (allowed 16 MiB)."""
+    budget = 16 << 20
+    r = np.random.default_rng(7)
+    reqs = [(int(r.integers(0, 2)), int(r.integers(0, 32)), int(r.integers(1, 81))) for _ in range(200)]
+    reqs = [(d, int(datasets[d][2][i]), k) for d, i, k in reqs]
+    alone = [datasets[d][1].query(q, k) for d, q, k in reqs]  # before the measurement: this allocates
+    free0 = device_free_bytes()
+    srv = rk.Server(ctx, max_batch=256, memory_budget=budget, max_wait_s=0.001)
+    try:
+        tickets = []
+        for i, (d, q, k) in enumerate(reqs):
+            tickets.append(srv.submit(datasets[d][1], q, k))
+            if i % 50 == 49:
+                time.sleep(0.002)
+        res = {}
+        deadline = time.time() + 90
+        while len(res) < len(tickets) and time.time() < deadline:
+            for t in tickets:
+                if t not in res:
+                    got = srv.poll(t, datasets[0][1].nu + datasets[1][1].nu)
+                    if got is not None:
+                        res[t] = got[0]
+            time.sleep(0.001)
+        assert len(res) == len(tickets)
+        held, step = srv.info()
+        grew = free0 - device_free_bytes()
+        print(f"budget {budget >> 20} MiB: buffers {held / 2**20:.1f} MiB, steps of {step} queries, "
+              f"device free memory down {grew / 2**20:.1f} MiB", flush=True)
+        assert held <= budget
+        assert step < 256  # the budget, not max_batch, sized the steps
+        assert grew <= budget + (16 << 20)
+    finally:
+        srv.stop()
+    for t, exp, req in zip(tickets, alone, reqs):
+        assert np.array_equal(res[t], exp), req
