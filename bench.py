@@ -344,6 +344,24 @@ def main():
                 kts[kern].append(t.kernel_s)
                 flops[kern] += 4.0 * args.users * t.n_candidates
     del os.environ["RK_COUNT_KERNEL"]
+    # the whole count-all call, counts in caller order (a scatter pass after
+    # the kernel) and in the dataset's order (the kernel's output as it is),
+    # each including its device-to-host copy of nu counts
+    call_ms = {}
+    counts_out = np.zeros(args.users, dtype=np.uint32)
+    for name, so in (("caller_order", False), ("dataset_order", True)):
+        tt, kk, dd = [], [], []
+        for i in range(min(args.roofline_queries, args.queries)):
+            t0 = time.perf_counter()
+            _, t = ds.count_all(qs[i], cands[i], timing=True, sorted_order=so, out=counts_out)
+            tt.append(time.perf_counter() - t0)
+            kk.append(t.kernel_s)
+            dd.append(t.download_s)
+        call_ms[name] = {"call": round(float(np.median(tt)) * 1e3, 3),
+                         "kernel": round(float(np.median(kk)) * 1e3, 4),
+                         "order_and_copy": round(float(np.median(dd)) * 1e3, 3)}
+        log(f"count-all call, {name}: {call_ms[name]['call']} ms (kernel {call_ms[name]['kernel']} ms, "
+            f"order + copy of {args.users} counts {call_ms[name]['order_and_copy']} ms)")
     for kern, kt in kts.items():
         variants[kern] = (flops[kern] / sum(kt) / 1e12, float(np.mean(kt)), flops[kern] / len(kt))
         log(f"count-all kernel ({kern}): {variants[kern][0]:.2f} TFLOP/s over {len(kt)} launches, "
@@ -473,6 +491,7 @@ def main():
         "batched_sync_ms_per_step": round(sync_ms, 4),
         "filter_qps": round(args.queries / filter_s, 1),
         "roofline": roofline,
+        "count_all_call_ms": call_ms,
         "cpu_baseline": cpu,
         "cpu_baseline_mt": cpu_mt,
         "cpu_bruteforce": cpu_bf,

@@ -152,6 +152,20 @@ rk_status rk_query(rk_dataset* ds, uint32_t q, uint32_t k,
 rk_status rk_count_all(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc,
                        uint32_t* counts, rk_timing* t);
 
+/* ---- count-all in the dataset's own user order: counts[j] belongs to user
+ *      order[j] of rk_dataset_create's arrays, order from rk_dataset_order.
+ *      The counts leave the count kernel as they are, with no scatter pass
+ *      to caller order (a random 4-byte write per user).  Same counts as
+ *      rk_count_all (the rknn-rt counter rt.cu:95-111 with k = inf). ---- */
+rk_status rk_count_all_sorted(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc,
+                              uint32_t* counts, rk_timing* t);
+
+/* ---- the dataset's user order: order[j] = index in rk_dataset_create's
+ *      arrays of the j-th user in the engine's layout (nu entries).  Fixed
+ *      for the dataset's life: fetch once, apply to any number of
+ *      rk_count_all_sorted results.  No reference counterpart. ---- */
+rk_status rk_dataset_order(const rk_dataset* ds, uint32_t* order);
+
 /* ---- batched form: many queries, uneven candidate counts and mixed k in
  *      one graph-captured launch sequence.  No reference counterpart beyond
  *      looping gpu_get_rknn_candidates per query. ---- */

@@ -1487,9 +1487,9 @@ const char* count_variant_name(int variant)
 bool count_narrow(uint32_t nc) { return nc < 65536u; }
 
 void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, void* counts,
-                      uint32_t* rechecks, int variant, hipStream_t st)
+                      uint32_t* rechecks, int variant, hipStream_t st, bool wide)
 {
-    if (count_narrow(q.nc))
+    if (!wide && count_narrow(q.nc))
         launch_count_typed(ds, q, rec, orig, (uint16_t*)counts, rechecks, variant, st);
     else
         launch_count_typed(ds, q, rec, orig, (uint32_t*)counts, rechecks, variant, st);

@@ -99,8 +99,9 @@ constexpr int kCountValu6 = 2;
 constexpr int kCountVariants = 3;
 const char* count_variant_name(int variant);
 bool count_narrow(uint32_t nc);
+// wide: 32-bit counts whatever the candidate count
 void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, void* counts,
-                      uint32_t* rechecks, int variant, hipStream_t st);
+                      uint32_t* rechecks, int variant, hipStream_t st, bool wide = false);
 void launch_scatter_counts(const void* src_sorted, bool narrow, const uint32_t* perm, uint32_t n,
                            uint32_t* dst_orig, hipStream_t st);
 

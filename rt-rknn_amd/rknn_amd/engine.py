@@ -156,14 +156,29 @@ class Dataset:
         ids, t = self._ids_call(self._lib.rk_query, q, k)
         return (ids, t) if timing else ids
 
-    def count_all(self, q, cand, timing=False):
+    def count_all(self, q, cand, timing=False, sorted_order=False, out=None):
+        """Per-user counts in caller order, or with sorted_order=True in the
+        dataset's own order (counts[j] belongs to user order()[j]); out: a
+        uint32 array of at least nu entries to write them into."""
         cand = _u32(cand)
-        counts = np.zeros(max(1, self.nu), dtype=np.uint32)
+        if out is not None:
+            if out.dtype != np.uint32 or not out.flags.c_contiguous or len(out) < max(1, self.nu):
+                raise ValueError("out must be a contiguous uint32 array of at least nu entries")
+            counts = out
+        else:
+            counts = np.zeros(max(1, self.nu), dtype=np.uint32)
         t = L.rk_timing()
-        self.ctx.check(self._lib.rk_count_all(self.h, q, _ptr(cand, C.c_uint32), len(cand),
-                                              _ptr(counts, C.c_uint32), C.byref(t)))
+        fn = self._lib.rk_count_all_sorted if sorted_order else self._lib.rk_count_all
+        self.ctx.check(fn(self.h, q, _ptr(cand, C.c_uint32), len(cand), _ptr(counts, C.c_uint32), C.byref(t)))
         counts = counts[: self.nu]
         return (counts, t) if timing else counts
+
+    def order(self):
+        """order[j]: index in the constructor's user arrays of the dataset's
+        j-th user."""
+        out = np.zeros(max(1, self.nu), dtype=np.uint32)
+        self.ctx.check(self._lib.rk_dataset_order(self.h, _ptr(out, C.c_uint32)))
+        return out[: self.nu]
 
     @staticmethod
     def _descs(queries):

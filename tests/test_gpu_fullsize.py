@@ -129,3 +129,21 @@ def test_deep_served(rk, ctx, deep):
     arrs, ds, qs, ks, cands, exp = deep
     for i, g in enumerate(served(rk, ctx, ds, qs, ks)):
         assert np.array_equal(g, exp[i]), (qs[i], ks[i])
+
+
+def test_deep_count_all(deep):
+    """Count-all at 17M users: the counts cross PCIe through the pinned chunk
+    ring (a pageable destination of 68 MB), in caller order and in the
+    dataset's order, and the users with fewer than k closer candidates are
+    exactly the result set."""
+    arrs, ds, qs, ks, cands, exp = deep
+    ux, uy, uid, fx, fy, fid = arrs
+    order = ds.order()
+    out = np.full(ds.nu, 0xFFFFFFFF, dtype=np.uint32)
+    for i in (0, 1, 5):
+        want = O.counts(ux, uy, fx, fy, qs[i], cand=cands[i])
+        got = ds.count_all(qs[i], cands[i])
+        assert np.array_equal(got, want), qs[i]
+        assert np.array_equal(np.sort(uid[got < ks[i]]), exp[i]), qs[i]
+        ds.count_all(qs[i], cands[i], sorted_order=True, out=out)
+        assert np.array_equal(out, want[order]), qs[i]

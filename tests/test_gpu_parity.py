@@ -129,6 +129,15 @@ def test_count_all_matches_oracle(g1, gties, kernel, monkeypatch):
             allf = np.arange(ds.nf, dtype=np.uint32)
             exp = O.counts(d["ux"], d["uy"], d["fx"], d["fy"], q, cand=allf)
             assert np.array_equal(ds.count_all(q, allf), exp)
+            # the dataset-order form: the same counts, permuted by order()
+            assert np.array_equal(ds.count_all(q, allf, sorted_order=True), exp[ds.order()])
+
+
+def test_dataset_order_is_a_permutation(g1):
+    d, ds, _, _ = g1
+    order = ds.order()
+    assert len(order) == ds.nu
+    assert np.array_equal(np.sort(order), np.arange(ds.nu, dtype=np.uint32))
 
 
 def test_hand_cases_on_gpu(rk, ctx):
@@ -309,9 +318,12 @@ def test_count_all_large_vs_oracle(rk, ctx, kernel, monkeypatch):
     try:
         for q in rk.generate_queries(nf, 4, seed)[:3]:
             q = int(q)
+            order = ds.order()
             for cand in (ds.filter(q, 25), np.arange(nf, dtype=np.uint32)):
                 exp = O.counts(ux, uy, fx, fy, q, cand=cand)
                 got, t = ds.count_all(q, cand, timing=True)
                 assert np.array_equal(got, exp), (kernel, q, len(cand))
+                got = ds.count_all(q, cand, sorted_order=True)
+                assert np.array_equal(got, exp[order]), (kernel, q, len(cand))
     finally:
         ds.close()
