@@ -198,7 +198,12 @@ rk_status rk_load_text(const char* path, float** fx, float** fy, int32_t** fid, 
     float* ys[2] = {nullptr, nullptr};
     int32_t* ids[2] = {nullptr, nullptr};
     for (int part = 0; part < 2; ++part) {
-        if (!parse_u64(p, end, &counts[part]) || counts[part] > (1ull << 31)) goto fail;
+        // a record takes at least 5 bytes ("i x y" and a separator but the
+        // last), so a count the rest of the file cannot hold fails before
+        // anything is allocated for it
+        if (!parse_u64(p, end, &counts[part]) || counts[part] > (1ull << 31) ||
+            counts[part] > (uint64_t)(end - p) / 5)
+            goto fail;
         xs[part] = (float*)std::malloc(sizeof(float) * (counts[part] ? counts[part] : 1));
         ys[part] = (float*)std::malloc(sizeof(float) * (counts[part] ? counts[part] : 1));
         ids[part] = (int32_t*)std::malloc(sizeof(int32_t) * (counts[part] ? counts[part] : 1));
