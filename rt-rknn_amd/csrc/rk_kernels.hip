@@ -432,16 +432,14 @@ __device__ __forceinline__ void load_group(const DsDev& ds, uint32_t g, float* p
     }
 }
 
-template <int U, typename OutT>
-__device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint32_t nc, bool lds_recs,
+template <int U, bool kLdsRecs, typename OutT>
+__device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint32_t nc,
                                             const CandRec* __restrict__ srec, const CandRec* __restrict__ rec,
                                             const float2* __restrict__ orig, OutT* __restrict__ counts,
-                                            CandRec* wl, uint32_t g, const float* px, const float* py,
-                                            float4 m, uint32_t& rech)
+                                            CandRec* wl, uint32_t t0, uint32_t nt, const float* px,
+                                            const float* py, float4 m, uint32_t& rech)
 {
     const uint32_t lane = lane_id();
-    const uint32_t t0 = g * U;
-    const uint32_t nt = min((uint32_t)U, ds.n_tiles - t0);
     float wx[U], wy[U], acc[U]; // acc: the count so far, a whole number
 #pragma unroll
     for (int j = 0; j < U; ++j) {
@@ -455,12 +453,12 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
     for (uint32_t c0 = 0; c0 < nc; c0 += 64) {
         const uint32_t jc = c0 + lane;
         const bool have = jc < nc;
+        // the records' place is a template parameter: a loop holding both
+        // paths was compiled with a wait for every outstanding load
+        // (vmcnt(0) lgkmcnt(0)) at each chunk, the global path's loads
+        // sharing the LDS path's registers
         CandRec r = {0.f, 0.f, 0.f, 0.f};
-        if (lds_recs) { // two explicit paths: a selected pointer would become a flat load
-            if (have) r = srec[jc];
-        } else {
-            if (have) r = rec[jc];
-        }
+        if (have) r = kLdsRecs ? srec[jc] : rec[jc];
         const bool clean = have && clean_for(r, lx, ly, hx, hy);
         const bool slow = have && !clean;
         const uint64_t mc = ballot(clean), ms = ballot(slow);
@@ -517,7 +515,11 @@ void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec, const float2
     }
     uint32_t rech = 0;
     while (g < n_groups) {
-        count_group<U, OutT>(ds, q, nc, lds_recs, s_recs, rec, orig, counts, s_w[wave], g, px, py, m, rech);
+        const uint32_t t0 = g * U, nt = min((uint32_t)U, ds.n_tiles - t0);
+        if (lds_recs)
+            count_group<U, true, OutT>(ds, q, nc, s_recs, rec, orig, counts, s_w[wave], t0, nt, px, py, m, rech);
+        else
+            count_group<U, false, OutT>(ds, q, nc, s_recs, rec, orig, counts, s_w[wave], t0, nt, px, py, m, rech);
         g += stride;
         if (g < n_groups) load_group<U>(ds, g, px, py, m);
     }
@@ -1481,7 +1483,8 @@ void launch_count_typed(const DsDev& ds, QDev q, const CandRec* rec, const float
 
 const char* count_variant_name(int variant)
 {
-    return variant == kCountValu8 ? "valu8" : variant == kCountValu6 ? "valu6" : "valu4";
+    static const char* const names[kCountVariants] = {"valu4", "valu8", "valu6"};
+    return variant >= 0 && variant < kCountVariants ? names[variant] : "valu4";
 }
 
 bool count_narrow(uint32_t nc) { return nc < 65536u; }

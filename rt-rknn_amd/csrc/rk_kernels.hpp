@@ -91,8 +91,8 @@ void launch_build_records(const uint32_t* cand, uint32_t nc, uint32_t q, float q
 // count-all form: exact per-user count (sorted order), no early exit.
 // rechecks has kBuckets slots.  Counts are 16-bit when count_narrow(nc)
 // (fewer than 2^16 candidates), else 32-bit; the buffer holds n_pad of them.
-// variants: 4 or 8 users per lane, or 4 held to 8 waves per SIMD (3
-// instructions per clean test in each)
+// variants: 4, 6 or 8 users per lane (3 instructions per clean test in
+// each), one group of tiles per wave
 constexpr int kCountValu4 = 0;
 constexpr int kCountValu8 = 1;
 constexpr int kCountValu6 = 2;
