@@ -376,6 +376,18 @@ def main():
                 "kernel": kernel_names[best],
                 "avg_launch_ms": round(avg_s * 1e3, 4), "flops_per_launch": flops_launch,
                 "variants_tflops": {k: round(v[0], 3) for k, v in variants.items()}}
+    # the same kernel over the batched workload's own k mix (k ~ U{1..50}: the
+    # mixed-k step's candidate lists), flop-weighted over its queries
+    os.environ["RK_COUNT_KERNEL"] = best
+    kmix_s, kmix_f = 0.0, 0.0
+    for i in range(min(2 * args.roofline_queries, args.queries)):
+        _, t = ds.count_all(qs[i], mcands[i], timing=True)
+        kmix_s += t.kernel_s
+        kmix_f += 4.0 * args.users * t.n_candidates
+    del os.environ["RK_COUNT_KERNEL"]
+    roofline["kmix_tflops"] = round(kmix_f / kmix_s / 1e12, 3)
+    roofline["kmix_frac"] = round(kmix_f / kmix_s / 1e12 / PEAK_FP32_TFLOPS, 4)
+    log(f"count-all kernel ({best}) over the k ~ U{{1..50}} lists: {roofline['kmix_tflops']:.2f} TFLOP/s")
 
     # ---- CPU baseline: the reference's tree-culled verification restated in C
     # (oracle/oracle_tree.c, rknn-inf/inf.h:194-249) on the same queries and

@@ -22,8 +22,8 @@
 //    ballot into LDS) and then update the sectors (lane = sector, each lane
 //    walking the batch's relevant facilities on its own): every sector keeps
 //    the k smallest exit distances (tightened by the oracle's error, stored
-//    as fp32 rounded up) in an LDS max-heap, and once full its bound is the
-//    heap's largest.  The largest bound over the sectors is refreshed once
+//    as fp32 rounded up) in a sorted register list, and once it holds k its
+//    bound is the k-th.  The largest bound over the sectors is refreshed once
 //    per batch (a stale, larger value only stops the walk later).
 //
 // The visiting order differs from the host's nearest-first stream, and the
@@ -36,6 +36,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "rk_dfilter.hpp"
@@ -92,21 +93,25 @@ __device__ __forceinline__ double df_delta(double rbox2, double gg)
     return 8.02 * 0x1p-53 * (3.0 * rbox2 + 2.0 * gg) + 0x1p-100;
 }
 
-// Loosened smallest entry distance of half-plane g into sector s.
-__device__ __forceinline__ double df_loose(const double* ex, const double* ey, int s, double gx, double gy, double gg,
-                                           double glen, double itol, double dl)
+// The loosened smallest entry distance l of half-plane g into sector s, as
+// a pair (lhs, unit) with  l < B  <=>  lhs < unit * B  (no division):
+//  * g's direction inside the sector: l = |g|/2 * shrink, so (hl, 1);
+//  * else l = |g|^2 / (2 c) * shrink over the sector's ray with the larger
+//    c = e.g > 0, so (|g|^2 * shrink, 2 c); no ray with c > 0: never.
+// shrink = (1 - dl)(1 - 1e-12) loosens by the oracle's error; the few ulps
+// of the multiplied-out form lie far inside its 1e-12.
+struct Entry {
+    double lhs, unit;
+};
+
+__device__ __forceinline__ Entry df_entry(const double* ex, const double* ey, int s, double gx, double gy, double hl,
+                                          double ggs, double itol)
 {
-    const double ca = ex[s] * gx + ey[s] * gy, cb = ex[s + 1] * gx + ey[s + 1] * gy;
     const double xa = ex[s] * gy - ey[s] * gx, xb = ex[s + 1] * gy - ey[s + 1] * gx;
-    double rmin;
-    if (xa >= -itol && -xb >= -itol) {
-        rmin = 0.5 * glen;
-    } else {
-        const double ra = ca > 0.0 ? gg / (2.0 * ca) : INFINITY;
-        const double rb = cb > 0.0 ? gg / (2.0 * cb) : INFINITY;
-        rmin = fmin(ra, rb);
-    }
-    return rmin * (1.0 - dl) * (1.0 - 1e-12);
+    if (xa >= -itol && -xb >= -itol) return {hl, 1.0};
+    const double ca = ex[s] * gx + ey[s] * gy, cb = ex[s + 1] * gx + ey[s + 1] * gy;
+    const double m = fmax(ca, cb);
+    return m > 0.0 ? Entry{ggs, 2.0 * m} : Entry{1.0, 0.0};
 }
 
 __device__ __forceinline__ bool df_inside(const double* ex, const double* ey, int s, double gx, double gy, double itol)
@@ -124,7 +129,8 @@ __device__ bool df_relevant(const double* ex, const double* ey, const double* bo
     const double shrink = (1.0 - dl) * (1.0 - 1e-12);
     if (0.25 * gg * shrink * shrink >= R * R * (1.0 + 1e-9)) return false;
     const double glen = sqrt(gg);
-    if (0.5 * glen * shrink >= R) return false;
+    const double hl = 0.5 * glen * shrink, ggs = gg * shrink;
+    if (hl >= R) return false;
     const double itol = 1e-9 * glen;
     float ang = atan2f((float)gy, (float)gx);
     if (ang < 0.f) ang += (float)kTwoPi;
@@ -136,8 +142,10 @@ __device__ bool df_relevant(const double* ex, const double* ey, const double* bo
         } else if (df_inside(ex, ey, sr, gx, gy, itol)) {
             s0 = sr;
         } else { // no sector found near the estimate: test them all
-            for (int s = 0; s < kDfSectors; ++s)
-                if (df_loose(ex, ey, s, gx, gy, gg, glen, itol, dl) < bound[s]) return true;
+            for (int s = 0; s < kDfSectors; ++s) {
+                const Entry e = df_entry(ex, ey, s, gx, gy, hl, ggs, itol);
+                if (e.lhs < e.unit * bound[s]) return true;
+            }
             return false;
         }
     }
@@ -147,42 +155,45 @@ __device__ bool df_relevant(const double* ex, const double* ey, const double* bo
     for (int d = 0; d < kDfSectors / 2 && (right || left); ++d) {
         if (right) {
             const int s = (s0 + d) % kDfSectors;
-            const double l = df_loose(ex, ey, s, gx, gy, gg, glen, itol, dl);
-            if (l < bound[s]) return true;
-            right = l < stop;
+            const Entry e = df_entry(ex, ey, s, gx, gy, hl, ggs, itol);
+            if (e.lhs < e.unit * bound[s]) return true;
+            right = e.lhs < e.unit * stop;
         }
         if (left) {
             const int s = (s0 + kDfSectors - 1 - d) % kDfSectors;
-            const double l = df_loose(ex, ey, s, gx, gy, gg, glen, itol, dl);
-            if (l < bound[s]) return true;
-            left = l < stop;
+            const Entry e = df_entry(ex, ey, s, gx, gy, hl, ggs, itol);
+            if (e.lhs < e.unit * bound[s]) return true;
+            left = e.lhs < e.unit * stop;
         }
     }
     return false;
 }
 
-// Up to this k the sector lanes keep their k smallest exit distances in
-// registers, sorted (an insertion is kDfRegK independent compare/select
-// pairs); beyond it in the LDS heaps (a sift walks dependent LDS accesses).
-constexpr uint32_t kDfRegK = 16;
-
-// Inserts v into h[0 .. kDfRegK), sorted ascending (h[i] becomes the larger
-// of h[i-1] and min(h[i], v): every new value depends only on old ones).
+// Each sector lane keeps the K smallest exit distances it has seen, sorted,
+// in registers: K = 16, 32 or 64, the smallest that holds k.  One wave runs
+// one query, a wave to a SIMD for a batch of about a thousand queries, so
+// registers are not what limits the filter: the dependent latency of its
+// instructions is.  An insertion is K independent compare/select pairs.
+//
+// Inserts v into h[0 .. K), sorted ascending (h[i] becomes the larger of
+// h[i-1] and min(h[i], v): every new value depends only on old ones).
+template <int K>
 __device__ __forceinline__ void reg_insert(float* h, float v)
 {
 #pragma unroll
-    for (int i = kDfRegK - 1; i > 0; --i) h[i] = v < h[i - 1] ? h[i - 1] : fminf(h[i], v);
+    for (int i = K - 1; i > 0; --i) h[i] = v < h[i - 1] ? h[i - 1] : fminf(h[i], v);
     h[0] = fminf(h[0], v);
 }
 
 // h[k - 1] of the sorted (non-negative) list, as the largest of its first k:
 // a reduction, so the array is never indexed at run time (which would move
 // it to scratch memory)
+template <int K>
 __device__ __forceinline__ float reg_kth(const float* h, uint32_t k)
 {
     float t = 0.f;
 #pragma unroll
-    for (int i = 0; i < (int)kDfRegK; ++i) t = fmaxf(t, (uint32_t)i < k ? h[i] : 0.f);
+    for (int i = 0; i < K; ++i) t = fmaxf(t, (uint32_t)i < k ? h[i] : 0.f);
     return t;
 }
 
@@ -194,200 +205,201 @@ __device__ __forceinline__ float round_up_f(double v)
     return f;
 }
 
+// Exit distance of relevant facility e from the sector (ax, ay)..(bx, by),
+// tightened by the oracle's error and rounded up to fp32; infinite when the
+// half-plane does not cross both of the sector's rays.
+__device__ __forceinline__ float exit_dist(const double4& e, double ax, double ay, double bx, double by)
+{
+    const double ca = ax * e.x + ay * e.y, cb = bx * e.x + by * e.y;
+    const bool cut = ca > 0.0 && cb > 0.0;
+    const double den = cut ? 2.0 * fmin(ca, cb) : 1.0;
+    const double rmax = e.z * (1.0 + e.w) * (1.0 + 1e-12) / den;
+    return cut ? round_up_f(rmax) : INFINITY;
+}
+
+struct DfShared {
+    double bound[kDfSectors];
+    double ex[kDfSectors + 1], ey[kDfSectors + 1];
+    uint32_t st[64], pre[64];
+    double4 rel[64]; // the batch's relevant facilities: (gx, gy, |g|^2, oracle allowance)
+};
+
+// The candidate list of one query (q, k <= K) into out[0 .. cap), its length
+// returned (it may exceed cap: overflow).  One wave; lane = sector.
+template <int K>
+__device__ uint32_t filter_query(const FacGridDev& g, DfShared& sh, uint32_t q, uint32_t k, uint32_t* out,
+                                 uint32_t cap)
+{
+    const uint32_t lane = threadIdx.x;
+    const double ax = sh.ex[lane], ay = sh.ey[lane], bxs = sh.ex[lane + 1], bys = sh.ey[lane + 1];
+    const double bx0 = g.box[0], by0 = g.box[1], bx1 = g.box[2], by1 = g.box[3];
+    const int G = (int)g.G;
+    const double qx = g.ofx[q], qy = g.ofy[q];
+    uint32_t n = 0;
+    // farthest squared distance from q to the users' box
+    double rbox2 = 0.0;
+    const double cx4[4] = {bx0, bx1, bx0, bx1}, cy4[4] = {by0, by0, by1, by1};
+    for (int i = 0; i < 4; ++i) {
+        const double dx = cx4[i] - qx, dy = cy4[i] - qy;
+        rbox2 = fmax(rbox2, dx * dx + dy * dy);
+    }
+    // this lane's sector: the farthest point of the users' box inside it
+    double best = -1.0;
+    {
+        const double tol = 1e-12;
+        if (qx >= bx0 && qx <= bx1 && qy >= by0 && qy <= by1) best = 0.0;
+        for (int i = 0; i < 4; ++i) {
+            const double dx = cx4[i] - qx, dy = cy4[i] - qy;
+            const double sc = tol * (fabs(dx) + fabs(dy));
+            if (ax * dy - ay * dx >= -sc && dx * bys - dy * bxs >= -sc) best = fmax(best, dx * dx + dy * dy);
+        }
+        const double span = (bx1 - bx0) + (by1 - by0) + 1.0;
+        for (int e = 0; e < 2; ++e) {
+            const double ux = e ? bxs : ax, uy = e ? bys : ay;
+            if (ux != 0.0) {
+                for (int w = 0; w < 2; ++w) {
+                    const double X = w ? bx1 : bx0;
+                    const double t = (X - qx) / ux;
+                    const double y = qy + t * uy;
+                    if (t >= 0.0 && y >= by0 - tol * span && y <= by1 + tol * span) best = fmax(best, t * t);
+                }
+            }
+            if (uy != 0.0) {
+                for (int w = 0; w < 2; ++w) {
+                    const double Y = w ? by1 : by0;
+                    const double t = (Y - qy) / uy;
+                    const double x = qx + t * ux;
+                    if (t >= 0.0 && x >= bx0 - tol * span && x <= bx1 + tol * span) best = fmax(best, t * t);
+                }
+            }
+        }
+    }
+    const double capv = best < 0.0 ? 0.0 : sqrt(best) * (1.0 + 1e-9) + 1e-9;
+    double bound = capv;
+    sh.bound[lane] = bound;
+    double R = wave_max_d(bound);
+    float h[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) h[i] = INFINITY;
+    __syncthreads();
+    const bool have_users = bx0 <= bx1 && by0 <= by1;
+    const int cx = min(max((int)floor((qx - g.x0) / g.cs), 0), G - 1);
+    const int cy = min(max((int)floor((qy - g.y0) / g.cs), 0), G - 1);
+    for (int r = 0; have_users && k > 0; ++r) {
+        if (r >= 2) {
+            const double dmin = (double)(r - 1) * g.cs * (1.0 - 1e-9);
+            const double key = dmin * dmin;
+            const double dl = df_delta(rbox2, key) / key;
+            const double lower = dl >= 1.0 ? 0.0 : 0.5 * dmin * (1.0 - dl) * (1.0 - 1e-12);
+            if (lower >= R) break;
+        }
+        if (r > 0 && cx - r < 0 && cx + r >= G && cy - r < 0 && cy + r >= G) break; // past every cell
+        const uint32_t ncells = r == 0 ? 1u : 8u * (uint32_t)r;
+        for (uint32_t c0 = 0; c0 < ncells; c0 += 64) {
+            uint32_t st = 0, cnt = 0;
+            if (c0 + lane < ncells) {
+                int x, y;
+                ring_cell(cx, cy, r, c0 + lane, x, y);
+                if (x >= 0 && x < G && y >= 0 && y < G) {
+                    const uint32_t c = (uint32_t)y * (uint32_t)G + (uint32_t)x;
+                    st = g.start[c];
+                    cnt = g.start[c + 1] - st;
+                }
+            }
+            uint32_t total = 0;
+            const uint32_t pre = wave_scan_excl(cnt, &total);
+            sh.st[lane] = st;
+            sh.pre[lane] = pre;
+            __syncthreads();
+            for (uint32_t j0 = 0; j0 < total; j0 += 64) {
+                const uint32_t j = j0 + lane;
+                bool rel = false;
+                double gx = 0.0, gy = 0.0, gg = 0.0, dl = 0.0;
+                uint32_t f = 0;
+                if (j < total) {
+                    int lo = 0, hi = 63; // the last cell whose prefix is <= j (it holds slot j)
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (sh.pre[mid] <= j) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    const uint32_t slot = sh.st[lo] + (j - sh.pre[lo]);
+                    f = g.fid[slot];
+                    gx = (double)g.fx[slot] - qx;
+                    gy = (double)g.fy[slot] - qy;
+                    gg = gx * gx + gy * gy;
+                    if (f != q && gg != 0.0) { // co-located with q: never strictly closer
+                        dl = df_delta(rbox2, gg) / gg;
+                        rel = df_relevant(sh.ex, sh.ey, sh.bound, gx, gy, gg, dl, R);
+                    }
+                }
+                // candidates, in lane order, and their parameters for the sector lanes
+                const uint64_t rm = __builtin_amdgcn_ballot_w64(rel);
+                const uint32_t nrel = (uint32_t)__popcll(rm);
+                if (rel) {
+                    const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(rm >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)rm, 0u));
+                    sh.rel[rk] = make_double4(gx, gy, gg, dl);
+                    if (n + rk < cap) out[n + rk] = f;
+                }
+                n += nrel;
+                __syncthreads();
+                // lane = sector: the relevant facilities' exit distances from
+                // this sector, four at a time (independent chains), then
+                // inserted; an insertion no lane needs is skipped
+                for (uint32_t i0 = 0; i0 < nrel; i0 += 4) {
+                    float v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        v[u] = i0 + u < nrel ? exit_dist(sh.rel[i0 + u], ax, ay, bxs, bys) : INFINITY;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        if (__builtin_amdgcn_ballot_w64(v[u] < h[K - 1]) != 0) reg_insert<K>(h, v[u]);
+                }
+                if (nrel) {
+                    const float kth = reg_kth<K>(h, k);
+                    if (kth < INFINITY) { // the sector has k exits
+                        const double bnd = fmin(capv, (double)kth);
+                        if (bnd < bound) {
+                            bound = bnd;
+                            sh.bound[lane] = bnd;
+                        }
+                    }
+                    R = wave_max_d(bound);
+                }
+                __syncthreads(); // sh.bound is read by the next batch's relevance tests
+            }
+            __syncthreads(); // sh.st / sh.pre are rewritten for the next chunk
+        }
+    }
+    return n;
+}
+
 __global__ __launch_bounds__(64) void k_dfilter(FacGridDev g, const uint32_t* __restrict__ qs,
                                                 const uint32_t* __restrict__ ks, uint32_t nq,
                                                 uint32_t* __restrict__ cand, uint32_t cap,
                                                 uint32_t* __restrict__ ncand)
 {
-    __shared__ float s_heap[kDfMaxK][kDfSectors]; // [i][sector]: sector's max-heap of k exit distances
-    __shared__ double s_bound[kDfSectors];
-    __shared__ double s_ex[kDfSectors + 1], s_ey[kDfSectors + 1];
-    __shared__ uint32_t s_st[64], s_pre[64];
-    __shared__ double4 s_rel[64]; // the batch's relevant facilities: (gx, gy, |g|^2, oracle allowance)
+    __shared__ DfShared sh;
     const uint32_t lane = threadIdx.x;
-    s_ex[lane] = g.ex[lane];
-    s_ey[lane] = g.ey[lane];
+    sh.ex[lane] = g.ex[lane];
+    sh.ey[lane] = g.ey[lane];
     if (lane == 0) {
-        s_ex[kDfSectors] = g.ex[kDfSectors];
-        s_ey[kDfSectors] = g.ey[kDfSectors];
+        sh.ex[kDfSectors] = g.ex[kDfSectors];
+        sh.ey[kDfSectors] = g.ey[kDfSectors];
     }
     __syncthreads();
-    const double ax = s_ex[lane], ay = s_ey[lane], bxs = s_ex[lane + 1], bys = s_ey[lane + 1];
-    const double bx0 = g.box[0], by0 = g.box[1], bx1 = g.box[2], by1 = g.box[3];
-    const int G = (int)g.G;
     for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
         const uint32_t q = qs[qi];
         const uint32_t k = min(ks[qi], kDfMaxK);
-        const double qx = g.ofx[q], qy = g.ofy[q];
         uint32_t* out = cand + (uint64_t)qi * cap;
-        uint32_t n = 0;
-        // farthest squared distance from q to the users' box
-        double rbox2 = 0.0;
-        const double cx4[4] = {bx0, bx1, bx0, bx1}, cy4[4] = {by0, by0, by1, by1};
-        for (int i = 0; i < 4; ++i) {
-            const double dx = cx4[i] - qx, dy = cy4[i] - qy;
-            rbox2 = fmax(rbox2, dx * dx + dy * dy);
-        }
-        // this lane's sector: the farthest point of the users' box inside it
-        double best = -1.0;
-        {
-            const double tol = 1e-12;
-            if (qx >= bx0 && qx <= bx1 && qy >= by0 && qy <= by1) best = 0.0;
-            for (int i = 0; i < 4; ++i) {
-                const double dx = cx4[i] - qx, dy = cy4[i] - qy;
-                const double sc = tol * (fabs(dx) + fabs(dy));
-                if (ax * dy - ay * dx >= -sc && dx * bys - dy * bxs >= -sc) best = fmax(best, dx * dx + dy * dy);
-            }
-            const double span = (bx1 - bx0) + (by1 - by0) + 1.0;
-            for (int e = 0; e < 2; ++e) {
-                const double ux = e ? bxs : ax, uy = e ? bys : ay;
-                if (ux != 0.0) {
-                    for (int w = 0; w < 2; ++w) {
-                        const double X = w ? bx1 : bx0;
-                        const double t = (X - qx) / ux;
-                        const double y = qy + t * uy;
-                        if (t >= 0.0 && y >= by0 - tol * span && y <= by1 + tol * span) best = fmax(best, t * t);
-                    }
-                }
-                if (uy != 0.0) {
-                    for (int w = 0; w < 2; ++w) {
-                        const double Y = w ? by1 : by0;
-                        const double t = (Y - qy) / uy;
-                        const double x = qx + t * ux;
-                        if (t >= 0.0 && x >= bx0 - tol * span && x <= bx1 + tol * span) best = fmax(best, t * t);
-                    }
-                }
-            }
-        }
-        const double capv = best < 0.0 ? 0.0 : sqrt(best) * (1.0 + 1e-9) + 1e-9;
-        double bound = capv;
-        s_bound[lane] = bound;
-        double R = wave_max_d(bound);
-        uint32_t hs = 0; // heap size of this lane's sector
-        const bool in_regs = k <= kDfRegK; // this query's sector lists in registers
-        float h[kDfRegK];
-#pragma unroll
-        for (int i = 0; i < (int)kDfRegK; ++i) h[i] = INFINITY;
-        __syncthreads();
-        const bool have_users = bx0 <= bx1 && by0 <= by1;
-        const int cx = min(max((int)floor((qx - g.x0) / g.cs), 0), G - 1);
-        const int cy = min(max((int)floor((qy - g.y0) / g.cs), 0), G - 1);
-        for (int r = 0; have_users && k > 0; ++r) {
-            if (r >= 2) {
-                const double dmin = (double)(r - 1) * g.cs * (1.0 - 1e-9);
-                const double key = dmin * dmin;
-                const double dl = df_delta(rbox2, key) / key;
-                const double lower = dl >= 1.0 ? 0.0 : 0.5 * dmin * (1.0 - dl) * (1.0 - 1e-12);
-                if (lower >= R) break;
-            }
-            if (r > 0 && cx - r < 0 && cx + r >= G && cy - r < 0 && cy + r >= G) break; // past every cell
-            const uint32_t ncells = r == 0 ? 1u : 8u * (uint32_t)r;
-            for (uint32_t c0 = 0; c0 < ncells; c0 += 64) {
-                uint32_t st = 0, cnt = 0;
-                if (c0 + lane < ncells) {
-                    int x, y;
-                    ring_cell(cx, cy, r, c0 + lane, x, y);
-                    if (x >= 0 && x < G && y >= 0 && y < G) {
-                        const uint32_t c = (uint32_t)y * (uint32_t)G + (uint32_t)x;
-                        st = g.start[c];
-                        cnt = g.start[c + 1] - st;
-                    }
-                }
-                uint32_t total = 0;
-                const uint32_t pre = wave_scan_excl(cnt, &total);
-                s_st[lane] = st;
-                s_pre[lane] = pre;
-                __syncthreads();
-                for (uint32_t j0 = 0; j0 < total; j0 += 64) {
-                    const uint32_t j = j0 + lane;
-                    bool rel = false;
-                    double gx = 0.0, gy = 0.0, gg = 0.0, dl = 0.0;
-                    uint32_t f = 0;
-                    if (j < total) {
-                        int lo = 0, hi = 63; // the last cell whose prefix is <= j (it holds slot j)
-                        while (lo < hi) {
-                            const int mid = (lo + hi + 1) >> 1;
-                            if (s_pre[mid] <= j) lo = mid;
-                            else hi = mid - 1;
-                        }
-                        const uint32_t slot = s_st[lo] + (j - s_pre[lo]);
-                        f = g.fid[slot];
-                        gx = (double)g.fx[slot] - qx;
-                        gy = (double)g.fy[slot] - qy;
-                        gg = gx * gx + gy * gy;
-                        if (f != q && gg != 0.0) { // co-located with q: never strictly closer
-                            dl = df_delta(rbox2, gg) / gg;
-                            rel = df_relevant(s_ex, s_ey, s_bound, gx, gy, gg, dl, R);
-                        }
-                    }
-                    // candidates, in lane order, and their parameters for the sector lanes
-                    const uint64_t rm = __builtin_amdgcn_ballot_w64(rel);
-                    const uint32_t nrel = (uint32_t)__popcll(rm);
-                    if (rel) {
-                        const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(rm >> 32),
-                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)rm, 0u));
-                        s_rel[rk] = make_double4(gx, gy, gg, dl);
-                        if (n + rk < cap) out[n + rk] = f;
-                    }
-                    n += nrel;
-                    __syncthreads();
-                    // lane = sector: every relevant facility's exit distance from this sector
-                    bool upd = false;
-                    for (uint32_t i = 0; i < nrel; ++i) {
-                        const double4 e = s_rel[i];
-                        const double ca = ax * e.x + ay * e.y, cb = bxs * e.x + bys * e.y;
-                        if (!(ca > 0.0 && cb > 0.0)) continue;
-                        const double rmax = e.z * (1.0 + e.w) * (1.0 + 1e-12) / (2.0 * fmin(ca, cb));
-                        const float v = round_up_f(rmax);
-                        if (in_regs) { // the k smallest: h[k - 1] is the LDS heap's top
-                            if (v < h[kDfRegK - 1]) {
-                                reg_insert(h, v);
-                                hs += hs < k ? 1u : 0u;
-                                upd = true;
-                            }
-                        } else if (hs < k) { // push, sift up
-                            uint32_t j = hs++;
-                            while (j > 0) {
-                                const uint32_t p = (j - 1) >> 1;
-                                const float pv = s_heap[p][lane];
-                                if (pv >= v) break;
-                                s_heap[j][lane] = pv;
-                                j = p;
-                            }
-                            s_heap[j][lane] = v;
-                            upd = true;
-                        } else if (v < s_heap[0][lane]) { // replace the largest, sift down
-                            uint32_t j = 0;
-                            while (true) {
-                                const uint32_t c1 = 2 * j + 1, c2 = c1 + 1;
-                                if (c1 >= hs) break;
-                                uint32_t c = c1;
-                                float cv = s_heap[c1][lane];
-                                if (c2 < hs && s_heap[c2][lane] > cv) {
-                                    c = c2;
-                                    cv = s_heap[c2][lane];
-                                }
-                                if (cv <= v) break;
-                                s_heap[j][lane] = cv;
-                                j = c;
-                            }
-                            s_heap[j][lane] = v;
-                            upd = true;
-                        }
-                    }
-                    if (upd && hs == k) {
-                        const double bnd = fmin(capv, (double)(in_regs ? reg_kth(h, k) : s_heap[0][lane]));
-                        if (bnd < bound) {
-                            bound = bnd;
-                            s_bound[lane] = bnd;
-                        }
-                    }
-                    if (nrel) R = wave_max_d(bound);
-                    __syncthreads(); // s_bound is read by the next batch's relevance tests
-                }
-                __syncthreads(); // s_st / s_pre are rewritten for the next chunk
-            }
-        }
+        uint32_t n;
+        if (k <= 16)
+            n = filter_query<16>(g, sh, q, k, out, cap);
+        else if (k <= 32)
+            n = filter_query<32>(g, sh, q, k, out, cap);
+        else
+            n = filter_query<64>(g, sh, q, k, out, cap);
         if (lane == 0) ncand[qi] = n;
         __syncthreads();
     }
@@ -437,7 +449,8 @@ rk_status build_facility_grid(rk_dataset* ds)
             y1 = std::max(y1, (double)fy[i]);
         }
     }
-    // about two facilities per cell
+    // about two facilities per cell (4, 8 and 16 measured slower: fewer
+    // rings, but more facilities tested against looser bounds)
     const uint32_t G = (uint32_t)std::max<double>(1.0, std::ceil(std::sqrt((double)nf / 2.0)));
     const double ext = std::max(x1 - x0, y1 - y0);
     const double cs = ext > 0 ? ext / G : 1.0;
