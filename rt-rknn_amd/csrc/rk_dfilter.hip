@@ -206,22 +206,31 @@ __device__ __forceinline__ float round_up_f(double v)
 }
 
 // Exit distance of relevant facility e from the sector (ax, ay)..(bx, by),
-// tightened by the oracle's error and rounded up to fp32; infinite when the
-// half-plane does not cross both of the sector's rays.
+//   rmax = |g|^2 (1 + dl)(1 + 1e-12) / (2 min(ca, cb)),
+// tightened by the oracle's error, as an fp32 value >= rmax; infinite when
+// the half-plane does not cross both of the sector's rays.  e.z holds the
+// numerator, already raised to an fp32 value >= it (once per facility).
+// The quotient is taken in fp32 with directed margins: the denominator is
+// lowered by 2^-22 before its conversion (which rounds by <= 2^-24), the
+// reciprocal (<= 1 ulp) and the product (<= 1/2 ulp) are covered by the
+// final factor 1 + 2^-21 (its own rounding <= 1/2 ulp).  A denominator
+// below fp32's range gives an infinite value, and one of 2^100 or more (its
+// reciprocal near the flushed range) is not used: an infinite exit distance
+// only leaves the sector's bound looser, which is always safe.
 __device__ __forceinline__ float exit_dist(const double4& e, double ax, double ay, double bx, double by)
 {
     const double ca = ax * e.x + ay * e.y, cb = bx * e.x + by * e.y;
-    const bool cut = ca > 0.0 && cb > 0.0;
-    const double den = cut ? 2.0 * fmin(ca, cb) : 1.0;
-    const double rmax = e.z * (1.0 + e.w) * (1.0 + 1e-12) / den;
-    return cut ? round_up_f(rmax) : INFINITY;
+    const float den = (float)(2.0 * fmin(ca, cb) * (1.0 - 0x1p-22));
+    const bool use = ca > 0.0 && cb > 0.0 && den < 0x1p100f;
+    const float v = (float)e.z * __builtin_amdgcn_rcpf(den) * (1.0f + 0x1p-21f);
+    return use ? v : INFINITY;
 }
 
 struct DfShared {
     double bound[kDfSectors];
     double ex[kDfSectors + 1], ey[kDfSectors + 1];
     uint32_t st[64], pre[64];
-    double4 rel[64]; // the batch's relevant facilities: (gx, gy, |g|^2, oracle allowance)
+    double4 rel[64]; // the batch's relevant facilities (exit_dist's operands)
 };
 
 // The candidate list of one query (q, k <= K) into out[0 .. cap), its length
@@ -339,7 +348,8 @@ __device__ uint32_t filter_query(const FacGridDev& g, DfShared& sh, uint32_t q, 
                 if (rel) {
                     const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(rm >> 32),
                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)rm, 0u));
-                    sh.rel[rk] = make_double4(gx, gy, gg, dl);
+                    // (gx, gy, fp32 value >= |g|^2 (1 + dl)(1 + 1e-12), unused)
+                    sh.rel[rk] = make_double4(gx, gy, (double)round_up_f(gg * (1.0 + dl) * (1.0 + 1e-12)), 0.0);
                     if (n + rk < cap) out[n + rk] = f;
                 }
                 n += nrel;
