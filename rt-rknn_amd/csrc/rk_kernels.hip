@@ -1046,6 +1046,16 @@ __device__ bool bucket_sort_regs(const uint32_t (&key)[kItems], uint32_t n, uint
     return red[2 * kW] == 0;
 }
 
+// v itself, opaque to the compiler: addresses formed from it inside a loop
+// are not hoisted out of the loop (where, kept for the whole kernel, they
+// spilled to scratch: 56 bytes per lane, 29 MB written per 1024-query
+// dispatch)
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
 // Takes n ids of the landing area for one query: the offset, or kNotLanded
 // when the area cannot hold them (the run is then repeated with a larger
 // area).
@@ -1081,6 +1091,8 @@ void k_batch_query(DsDev ds, BatchDev b)
 
     for (uint32_t i = tid; i < kPhaseItems * 64; i += kThreads) cover[i] = 0;
     for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
+        // per query: the thread's addresses are formed afresh
+        const uint32_t tid = opaque_u32(threadIdx.x), wave = tid >> 6, lane = opaque_u32(lane_id());
         const QDev Q = uniform_q(b.q[qi]);
         uint32_t* stat = b.qstat + (uint64_t)qi * kQueryStats;
         if (Q.trivial) { // k > |C|: every user qualifies; the host fills the result
