@@ -501,7 +501,7 @@ def main():
                              "p99": round(float(np.percentile(lat_ms, 99)), 4)},
         "batched_device_ms": round(float(np.mean(dev_sync)) * 1e3, 4),
         "batched_sync_ms_per_step": round(sync_ms, 4),
-        "filter_qps": round(args.queries / filter_s, 1),
+        "filter_qps": round(nb * nq / filter_s, 1),  # the host filter, one query per call
         "roofline": roofline,
         "count_all_call_ms": call_ms,
         "cpu_baseline": cpu,
