@@ -12,7 +12,10 @@ flight, each step's results taken in place (rk_batch_results) after its wait;
 `value_fetch` is the same with every step's ids copied into a caller buffer
 (rk_batch_fetch).  The JSON line also carries:
   roofline      dense counting kernel (count-all: every user x every candidate,
-                4 fp32 flops per test) timed with HIP events on its stream
+                4 fp32 flops per test) timed with HIP events on its stream;
+                kmix_tflops: the same kernel over the k ~ U{1..50} lists
+  count_all_call_ms  the whole count-all call, counts in caller order and in
+                the dataset's order, with the copy of 10M counts to the host
   cpu_baseline  the reference's tree-culled verification (rknn-inf) restated
                 in C, exact in binary64, on the step's queries and candidate
                 lists, one host thread (cpu_baseline_mt: every host core;
