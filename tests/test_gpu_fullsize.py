@@ -5,6 +5,7 @@ enough for a 4-level tile tree (more than 64^4 = 16,777,216 users), as the
 device-filtered batch, one call, served -- against the FP64 oracle over the
 host filter's candidate lists (the lists are sufficient: the filter tests
 pin that against brute force)."""
+import ctypes
 import time
 
 import numpy as np
@@ -147,3 +148,18 @@ def test_deep_count_all(deep):
         assert np.array_equal(np.sort(uid[got < ks[i]]), exp[i]), qs[i]
         ds.count_all(qs[i], cands[i], sorted_order=True, out=out)
         assert np.array_equal(out, want[order]), qs[i]
+    # a pinned destination (hipHostMalloc) takes one DMA instead of the ring
+    hip = ctypes.CDLL("libamdhip64.so")
+    ptr = ctypes.c_void_p()
+    assert hip.hipHostMalloc(ctypes.byref(ptr), ctypes.c_size_t(4 * ds.nu), ctypes.c_uint(0)) == 0
+    try:
+        pinned = np.ctypeslib.as_array((ctypes.c_uint32 * ds.nu).from_address(ptr.value))
+        pinned[:] = 0xFFFFFFFF
+        want = O.counts(ux, uy, fx, fy, qs[2], cand=cands[2])
+        ds.count_all(qs[2], cands[2], out=pinned)
+        assert np.array_equal(pinned, want)
+        ds.count_all(qs[2], cands[2], sorted_order=True, out=pinned)
+        assert np.array_equal(pinned, want[order])
+        del pinned
+    finally:
+        hip.hipHostFree(ptr)
