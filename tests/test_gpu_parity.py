@@ -133,6 +133,32 @@ def test_count_all_matches_oracle(g1, gties, kernel, monkeypatch):
             assert np.array_equal(ds.count_all(q, allf, sorted_order=True), exp[ds.order()])
 
 
+@pytest.mark.parametrize("kernel", ["valu4", "valu6", "valu8"])
+def test_count_all_wide_counts(rk, ctx, kernel, monkeypatch):
+    """65,536 candidates or more: counts no longer fit 16 bits, so the kernel
+    writes 32-bit ones (and the dataset-order form always does)."""
+    monkeypatch.setenv("RK_COUNT_KERNEL", kernel)
+    r = np.random.default_rng(17)
+    nu, nf = 3000, 70_000
+    ux, uy = (r.random(nu) * 1000).astype(np.float32), (r.random(nu) * 1000).astype(np.float32)
+    fx, fy = (r.random(nf) * 1000).astype(np.float32), (r.random(nf) * 1000).astype(np.float32)
+    fx[0], fy[0] = 5000.0, 5000.0  # far outside the users' box: nearly every facility is closer
+    uid = np.arange(nf, nf + nu, dtype=np.int32)
+    fid = np.arange(nf, dtype=np.int32)
+    ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    O.set_threads(16)
+    try:
+        allf = np.arange(nf, dtype=np.uint32)
+        for q in (0, 12345):
+            exp = O.counts(ux, uy, fx, fy, q, cand=allf)
+            if q == 0:
+                assert exp.min() >= 65536  # every count needs more than 16 bits
+            assert np.array_equal(ds.count_all(q, allf), exp)
+            assert np.array_equal(ds.count_all(q, allf, sorted_order=True), exp[ds.order()])
+    finally:
+        ds.close()
+
+
 def test_dataset_order_is_a_permutation(g1):
     d, ds, _, _ = g1
     order = ds.order()
