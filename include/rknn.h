@@ -193,8 +193,8 @@ rk_status rk_query_batch(rk_dataset* ds, const rk_query_desc* qs, size_t nq,
  * own data; RK_AUTOTUNE=0 disables, =8 or =4 forces that many waves per
  * query, RK_TUNE_FILE names the JSON cache) and
  * captures the hipGraph; run replays it.  A run ends with every query's
- * sorted ids in the batch's pinned host memory (the step's last kernel
- * writes them there); fetch copies them into the caller's buffer and
+ * sorted ids in the batch's pinned host memory (the step's kernels write
+ * them there); fetch copies them into the caller's buffer and
  * rk_batch_results hands out pointers to them.  run returns RK_ERR_CAPACITY
  * when a query's result outgrows its share of the result pool (2^26 ids per
  * batch, RK_BATCH_POOL_IDS); fewer queries per batch give each a larger
@@ -215,8 +215,8 @@ rk_status rk_batch_fetch(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* of
 rk_status rk_batch_enqueue(rk_batch* b);
 rk_status rk_batch_wait(rk_batch* b, double* device_seconds);
 /* The last run's results in place, without a copy: ids[i] points at query
- * i's n[i] sorted ids in the batch's pinned landing area (the step's last
- * kernel wrote them there; or in the dataset's list of every user), valid
+ * i's n[i] sorted ids in the batch's pinned landing area (the step's
+ * kernels wrote them there; or in the dataset's list of every user), valid
  * until the batch is filled, enqueued or destroyed.  Arrays of nq entries;
  * the conditions of rk_batch_fetch apply.  No reference counterpart (the
  * reference copies into the caller's vector, inf_kernel.cu:115-120). */

@@ -60,7 +60,7 @@ rk_status build_graph(rk_batch* b)
     RK_HIP(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     batch_launch(b, st);
     // (the results, counts and landing offsets reach pinned host memory from
-    // the step's last kernel: the graph holds no copy node)
+    // the step's kernels: the graph holds no copy node)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         hipGraph_t g = nullptr;
@@ -117,7 +117,7 @@ rk_status ensure_landing(rk_batch* b, uint64_t ids)
 }
 
 // Room for kQresWords words per query in pinned, coherent host memory
-// (written by the step's last kernel).  Caller holds the context lock.
+// (written by the step's kernels).  Caller holds the context lock.
 rk_status ensure_qres(rk_batch* b, uint64_t nq)
 {
     rk_ctx* c = b->ds->ctx;
@@ -505,7 +505,7 @@ uint64_t batch_device_bytes(const rk_batch* b)
 }
 
 // After a device-filtered run: the per-query views (trivial flag, candidate
-// count) from the words the landing kernel wrote to host memory; a
+// count) from the words the step's kernels wrote to host memory; a
 // candidate list beyond its segment is an overflow.
 void views_from_host(rk_batch* b)
 {

@@ -100,8 +100,8 @@ struct HostBuf {
     }
 };
 
-// Pinned, coherent host ids that a batch's landing kernel writes (the host
-// side of its landing area).  Shared: the serving loop hands a step's
+// Pinned, coherent host ids that a batch's kernels write (the host side of
+// its landing area).  Shared: the serving loop hands a step's
 // results out by reference and gives the batch another area for its next
 // step.
 struct PinnedIds {
@@ -203,11 +203,12 @@ struct rk_batch {
     rk::DevBuf<int32_t> seg;     // per-query result segments (spill of results beyond the kernel's LDS)
     rk::DevBuf<uint32_t> qres;   // [0, nq): results per query, [nq, 2 nq): landing offsets
     rk::DevBuf<uint32_t> qstat;
-    // landing area: the kernels sort each query's ids into dland (HBM) at
-    // the offset qres[nq + i], and the run's last kernel writes them to
-    // land_host (pinned, coherent host memory), with kQresWords words per
-    // query to h_qres (pinned, coherent); land_ctr: the run's reservation
-    // counters
+    // landing area: each query's sorted ids go to land_host (pinned,
+    // coherent host memory) at the offset qres[nq + i], written by the
+    // traversal kernel, or by the run's last kernel for a query it sorts in
+    // dland (HBM) first; the kernel that finishes a query writes its
+    // kQresWords words to h_qres (pinned, coherent); land_ctr: the run's
+    // reservation counters
     rk::HostBuf<uint32_t> h_qres;
     std::shared_ptr<rk::PinnedIds> land_host;
     // where a new host side comes from (at least the ids asked for); empty:

@@ -1065,6 +1065,17 @@ __device__ __forceinline__ uint32_t reserve_landing(const BatchDev& b, uint32_t 
     return s + n <= b.land_cap ? (uint32_t)s : kNotLanded;
 }
 
+// Query qi's four words in host memory: result count, landing offset,
+// candidates (the device filter's count when it made the list: beyond the
+// list's segment when it overflowed), flags.
+__device__ __forceinline__ void put_words(const BatchDev& b, uint32_t qi, uint32_t n, uint32_t start, const QDev& Q)
+{
+    b.hqres[qi] = n;
+    b.hqres[b.nq + qi] = start;
+    b.hqres[2 * b.nq + qi] = b.dn ? b.dn[qi] : Q.nc;
+    b.hqres[3 * b.nq + qi] = Q.trivial ? kQflagTrivial : 0u;
+}
+
 // kW waves per query.  With 8-wave blocks at eight waves per SIMD (<= 64
 // VGPRs), a batch of 1024 queries is resident at once on 256 CUs; 4-wave
 // blocks may use twice the registers.  The autotuner picks the shape per
@@ -1087,9 +1098,8 @@ void k_batch_query(DsDev ds, BatchDev b)
     WaveLds* wl = reinterpret_cast<WaveLds*>(cand);
     uint32_t* res = reinterpret_cast<uint32_t*>(cand + kCandBytes);
     uint32_t* front = reinterpret_cast<uint32_t*>(cand + kCandBytes + kResBytes);
-    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
 
-    for (uint32_t i = tid; i < kPhaseItems * 64; i += kThreads) cover[i] = 0;
+    for (uint32_t i = threadIdx.x; i < kPhaseItems * 64; i += kThreads) cover[i] = 0;
     for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
         // per query: the thread's addresses are formed afresh
         const uint32_t tid = opaque_u32(threadIdx.x), wave = tid >> 6, lane = opaque_u32(lane_id());
@@ -1100,6 +1110,7 @@ void k_batch_query(DsDev ds, BatchDev b)
             if (tid == 0) {
                 b.qcount[qi] = 0;
                 b.qland[qi] = 0;
+                put_words(b, qi, 0, 0, Q);
             }
             continue;
         }
@@ -1189,7 +1200,7 @@ void k_batch_query(DsDev ds, BatchDev b)
                 __syncthreads();
                 landed = ctl.land;
                 // a full landing area: the sorted ids stay in the segment (the run is repeated)
-                int32_t* dst = landed != kNotLanded ? b.dland + landed : seg;
+                int32_t* dst = landed != kNotLanded ? b.hland + landed : seg;
                 for (uint32_t i = tid; i < nres; i += kThreads) dst[i] = (int32_t)(s_out[i] ^ 0x80000000u);
             } else { // bunched ids: the landing kernel's sort, from the segment
 #pragma unroll
@@ -1209,21 +1220,16 @@ void k_batch_query(DsDev ds, BatchDev b)
         if (tid == 0) {
             stat[kSortSlot] = later;
             b.qland[qi] = later ? kNotLanded : landed;
+            if (!later) put_words(b, qi, nres, landed, Q);
         }
         __syncthreads(); // before the next query resets the state
     }
 }
 
-// The step's last launch: a few blocks (BatchDev::land_grid), each taking
-// every land_grid-th query.  A query the traversal kernel left to this launch (results beyond
-// the block's LDS buffer, or ids its bucket sort could not place) is sorted
-// here first: up to kDeviceSortMax ids bucket-sorted in LDS (rocPRIM's block
-// radix sort via hipCUB when they bunch), larger segments by LSD radix passes
-// between the segment and the landing area.  Then a wave per query writes
-// its sorted ids, count and landing offset to pinned host memory.  Kernel
-// stores to pinned memory measured 53-56 GB/s, above the runtime's copy (49,
-// a full-grid copy kernel that takes CU slots from the next step's query
-// blocks); a few blocks keep PCIe busy while holding few slots.
+// Sorting in the step's last launch (k_batch_land): a query the traversal
+// left to it is sorted there, up to kDeviceSortMax ids bucket-sorted in LDS
+// (rocPRIM's block radix sort via hipCUB when they bunch), larger segments
+// by LSD radix passes between the segment and the device landing area.
 constexpr int kSortBlock = 512;
 constexpr uint32_t kSortBuckets = 4096;
 static_assert(kDeviceSortMax == kSortBlock * 16, "device sort capacity");
@@ -1288,128 +1294,155 @@ __device__ void radix_pass(const int32_t* __restrict__ src, int32_t* __restrict_
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kSortBlock) void k_batch_land(BatchDev b)
+union LandLds {
+    struct {
+        uint32_t keys[kDeviceSortMax];
+        uint32_t cnt[kSortBuckets];
+        uint32_t red[2 * kSortBlock / 64 + 1];
+    } bk;
+    typename SortLarge::TempStorage radix;
+    struct {
+        uint32_t hist[kLandWaves][kRadixDigits];
+        uint32_t red[kLandWaves];
+    } rx;
+};
+
+// Sort query qi's ids, left in its segment by the traversal (n <= seg_cap),
+// into a landing area range reserved here; returns the range's start, or
+// kNotLanded when the area is full.  Called by the whole block.
+__device__ uint32_t sort_left(const BatchDev& b, uint32_t qi, LandLds& lds, uint32_t& s_start)
 {
-    __shared__ union {
-        struct {
-            uint32_t keys[kDeviceSortMax];
-            uint32_t cnt[kSortBuckets];
-            uint32_t red[2 * kSortBlock / 64 + 1];
-        } bk;
-        typename SortLarge::TempStorage radix;
-        struct {
-            uint32_t hist[kLandWaves][kRadixDigits];
-            uint32_t red[kLandWaves];
-        } rx;
-    } lds;
-    __shared__ uint32_t s_start;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x) {
-        const uint32_t n = b.qcount[qi];
-        uint32_t start = b.qland[qi];
-        if (b.qstat[(uint64_t)qi * kQueryStats + kSortSlot]) { // n <= seg_cap: the traversal leaves no other here
-            int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
-            if (tid == 0) s_start = reserve_landing(b, n);
-            __syncthreads();
-            start = s_start;
-            if (start != kNotLanded) {
-                int32_t* dst = b.dland + start;
-                if (n <= kDeviceSortMax) {
-                    uint32_t key[16];
+    const uint32_t n = b.qcount[qi];
+    int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
+    if (tid == 0) s_start = reserve_landing(b, n);
+    __syncthreads();
+    const uint32_t start = s_start;
+    if (start != kNotLanded) {
+        int32_t* dst = b.dland + start;
+        if (n <= kDeviceSortMax) {
+            uint32_t key[16];
 #pragma unroll
-                    for (int i = 0; i < 16; ++i) {
-                        const uint32_t j = tid + i * kSortBlock;
-                        key[i] = j < n ? ((uint32_t)seg[j] ^ 0x80000000u) : 0u;
-                    }
-                    if (bucket_sort_regs<kSortBlock, 16, kSortBuckets>(key, n, lds.bk.keys, lds.bk.cnt,
-                                                                       lds.bk.red)) {
-                        for (uint32_t i = tid; i < n; i += kSortBlock)
-                            dst[i] = (int32_t)(lds.bk.keys[i] ^ 0x80000000u);
-                    } else {
-                        __syncthreads(); // every thread has read the bucket sort's flag: the union is free
-                        int32_t k2[16];
-                        const uint32_t base = tid * 16;
+            for (int i = 0; i < 16; ++i) {
+                const uint32_t j = tid + i * kSortBlock;
+                key[i] = j < n ? ((uint32_t)seg[j] ^ 0x80000000u) : 0u;
+            }
+            if (bucket_sort_regs<kSortBlock, 16, kSortBuckets>(key, n, lds.bk.keys, lds.bk.cnt, lds.bk.red)) {
+                for (uint32_t i = tid; i < n; i += kSortBlock) dst[i] = (int32_t)(lds.bk.keys[i] ^ 0x80000000u);
+            } else {
+                __syncthreads(); // every thread has read the bucket sort's flag: the union is free
+                int32_t k2[16];
+                const uint32_t base = tid * 16;
 #pragma unroll
-                        for (int i = 0; i < 16; ++i) k2[i] = base + i < n ? seg[base + i] : INT32_MAX;
-                        // only the bits below the largest id; padding sorts last (stable, placed last)
-                        SortLarge(lds.radix).SortBlockedToStriped(k2, 0, (int)b.id_bits);
+                for (int i = 0; i < 16; ++i) k2[i] = base + i < n ? seg[base + i] : INT32_MAX;
+                // only the bits below the largest id; padding sorts last (stable, placed last)
+                SortLarge(lds.radix).SortBlockedToStriped(k2, 0, (int)b.id_bits);
 #pragma unroll
-                        for (int i = 0; i < 16; ++i) {
-                            const uint32_t j = tid + i * kSortBlock;
-                            if (j < n) dst[j] = k2[i];
-                        }
-                    }
-                } else {
-                    // ids below 2^id_bits (non-negative), or any int32 (id_bits =
-                    // 32: the sign bit flipped so that unsigned digit order is id
-                    // order); passes alternate between the segment and the landing
-                    // area so that the last one ends in the landing area
-                    const uint32_t bits = b.id_bits < 32 ? b.id_bits : 32;
-                    const uint32_t flip = bits == 32 ? 0x80000000u : 0u;
-                    const uint32_t passes = (bits + 7) / 8;
-                    const uint32_t per = (n + kLandWaves - 1) / kLandWaves;
-                    int32_t* from = seg;
-                    int32_t* to = dst;
-                    if ((passes & 1u) == 0) { // an even pass count ends where it starts: start in the landing area
-                        for (uint32_t i = tid; i < n; i += kSortBlock) dst[i] = seg[i];
-                        __syncthreads();
-                        from = dst;
-                        to = seg;
-                    }
-                    for (uint32_t p = 0; p < passes; ++p) {
-                        radix_pass(from, to, n, per, 8 * p, flip, lds.rx.hist, lds.rx.red);
-                        int32_t* t = from;
-                        from = to;
-                        to = t;
-                    }
+                for (int i = 0; i < 16; ++i) {
+                    const uint32_t j = tid + i * kSortBlock;
+                    if (j < n) dst[j] = k2[i];
                 }
             }
-            __syncthreads(); // the sorted ids are all in place
-        }
-        if (tid == 0) b.qland[qi] = start;
-        __syncthreads(); // before the next query reuses the block's LDS
-    }
-    // to host memory, a wave per query: the ids (landed, of a result that fit
-    // its segment), the count and the landing offset
-    const uint32_t wave = tid >> 6, lane = lane_id();
-    uint32_t k = 0;
-    for (uint32_t qi = blockIdx.x; qi < b.nq; qi += gridDim.x, ++k) {
-        if (k % kLandWaves != wave) continue;
-        const uint32_t n = b.qcount[qi];
-        const uint32_t start = b.qland[qi];
-        if (start != kNotLanded && n <= b.seg_cap && n) {
-            const int32_t* src = b.dland + start;
-            int32_t* dst = b.hland + start;
-            // 16-byte lanes from the first 16-byte boundary (both areas are aligned alike)
-            const uint32_t head = min(n, (4u - (start & 3u)) & 3u);
-            if (lane < head) dst[lane] = src[lane];
-            const uint32_t nv = (n - head) / 4;
-            const int4* s4 = reinterpret_cast<const int4*>(src + head);
-            int4* d4 = reinterpret_cast<int4*>(dst + head);
-            uint32_t i = lane;
-            for (; i + 192 < nv; i += 256) { // four loads in flight before the stores
-                const int4 a0 = s4[i], a1 = s4[i + 64], a2 = s4[i + 128], a3 = s4[i + 192];
-                d4[i] = a0;
-                d4[i + 64] = a1;
-                d4[i + 128] = a2;
-                d4[i + 192] = a3;
+        } else {
+            // ids below 2^id_bits (non-negative), or any int32 (id_bits = 32:
+            // the sign bit flipped so that unsigned digit order is id order);
+            // passes alternate between the segment and the landing area so
+            // that the last one ends in the landing area
+            const uint32_t bits = b.id_bits < 32 ? b.id_bits : 32;
+            const uint32_t flip = bits == 32 ? 0x80000000u : 0u;
+            const uint32_t passes = (bits + 7) / 8;
+            const uint32_t per = (n + kLandWaves - 1) / kLandWaves;
+            int32_t* from = seg;
+            int32_t* to = dst;
+            if ((passes & 1u) == 0) { // an even pass count ends where it starts: start in the landing area
+                for (uint32_t i = tid; i < n; i += kSortBlock) dst[i] = seg[i];
+                __syncthreads();
+                from = dst;
+                to = seg;
             }
-            for (; i < nv; i += 64) d4[i] = s4[i];
-            for (uint32_t t = head + nv * 4 + lane; t < n; t += 64) dst[t] = src[t];
+            for (uint32_t p = 0; p < passes; ++p) {
+                radix_pass(from, to, n, per, 8 * p, flip, lds.rx.hist, lds.rx.red);
+                int32_t* t = from;
+                from = to;
+                to = t;
+            }
         }
-        if (lane == 0) {
-            const QDev& Q = b.q[qi];
-            b.hqres[qi] = n;
-            b.hqres[b.nq + qi] = start;
-            b.hqres[2 * b.nq + qi] = b.dn ? b.dn[qi] : Q.nc;
-            b.hqres[3 * b.nq + qi] = Q.trivial ? kQflagTrivial : 0u;
-        }
-        // the wave's stores complete before its next query: a bounded number
-        // of writes in flight to PCIe (measured: 5-10% faster steps than
-        // letting them queue)
-        __threadfence_system();
     }
-    __syncthreads();
+    __syncthreads(); // the sorted ids are all in place
+    return start;
+}
+
+// One wave writes the ids of query qi, sorted by this launch into the device
+// landing area, to host memory, and the query's four words.  The wave's
+// stores complete before it returns: a bounded number of writes in flight
+// to PCIe.
+__device__ __forceinline__ void land_out(const BatchDev& b, uint32_t qi, uint32_t lane)
+{
+    const QDev& Q = b.q[qi];
+    const uint32_t n = b.qcount[qi], start = b.qland[qi];
+    if (start != kNotLanded) {
+        const int32_t* src = b.dland + start;
+        int32_t* dst = b.hland + start;
+        // 16-byte lanes from the first 16-byte boundary (both areas are aligned alike)
+        const uint32_t head = min(n, (4u - (start & 3u)) & 3u);
+        if (lane < head) dst[lane] = src[lane];
+        const uint32_t nv = (n - head) / 4;
+        const int4* s4 = reinterpret_cast<const int4*>(src + head);
+        int4* d4 = reinterpret_cast<int4*>(dst + head);
+        uint32_t i = lane;
+        for (; i + 192 < nv; i += 256) { // four loads in flight before the stores
+            const int4 a0 = s4[i], a1 = s4[i + 64], a2 = s4[i + 128], a3 = s4[i + 192];
+            d4[i] = a0;
+            d4[i + 64] = a1;
+            d4[i + 128] = a2;
+            d4[i + 192] = a3;
+        }
+        for (; i < nv; i += 64) d4[i] = s4[i];
+        for (uint32_t t = head + nv * 4 + lane; t < n; t += 64) dst[t] = src[t];
+    }
+    if (lane == 0) put_words(b, qi, n, start, Q);
+    __threadfence_system();
+}
+
+// The step's last launch: a few blocks (BatchDev::land_grid), block g taking
+// queries g, g + gridDim.x, ...  It handles the queries the traversal left
+// to it (flagged in their statistics): results beyond the traversal block's
+// LDS buffer, or ids its bucket sort could not place.  Every other query's
+// ids and words are in host memory already, written by the traversal.
+__global__ __launch_bounds__(kSortBlock) void k_batch_land(BatchDev b)
+{
+    __shared__ LandLds lds;
+    __shared__ uint32_t s_start;
+    __shared__ uint32_t s_list[kSortBlock]; // flagged queries of the chunk
+    __shared__ uint32_t s_wn[kLandWaves];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
+    const uint64_t G = gridDim.x;
+    // the block's queries kSortBlock at a time: a thread each reads a flag,
+    // the flagged ones are listed in LDS, sorted by the block one at a time,
+    // then written out a wave each
+    for (uint64_t j0 = 0; blockIdx.x + j0 * G < b.nq; j0 += kSortBlock) {
+        const uint64_t qt = blockIdx.x + (j0 + tid) * G;
+        const bool flagged = qt < b.nq && b.qstat[qt * kQueryStats + kSortSlot] != 0;
+        const uint64_t m = ballot(flagged);
+        if (lane == 0) s_wn[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t off = 0, n_list = 0;
+        for (uint32_t w = 0; w < kLandWaves; ++w) {
+            const uint32_t c = s_wn[w];
+            off += w < wave ? c : 0u;
+            n_list += c;
+        }
+        if (flagged) s_list[off + rank_below(m)] = (uint32_t)qt;
+        __syncthreads();
+        for (uint32_t li = 0; li < n_list; ++li) {
+            const uint32_t qi = s_list[li];
+            const uint32_t start = sort_left(b, qi, lds, s_start);
+            if (tid == 0) b.qland[qi] = start;
+            __syncthreads(); // before the next query reuses the block's LDS
+        }
+        for (uint32_t li = wave; li < n_list; li += kLandWaves) land_out(b, s_list[li], lane);
+        __syncthreads(); // every wave is done with the chunk's list and counts
+    }
     // the last block to finish clears the counters for the next run (every
     // reservation of this run is done by then)
     if (tid == 0) {

@@ -53,9 +53,9 @@ struct BatchDev {
     uint32_t* qcount;       // results per query (may exceed seg_cap: overflow)
     uint32_t* qland;        // per query: offset of its sorted ids in land[], or kNotLanded
     uint32_t* qstat;        // nq x kQueryStats statistics
-    int32_t* dland;         // landing area on the device, land_cap ids
-    int32_t* hland;         // its host twin (pinned, written by the landing kernel), land_cap ids
-    uint32_t* hqres;        // pinned host words per query, written by the landing kernel: [0, nq) result
+    int32_t* dland;         // landing area on the device (ids the landing kernel sorts), land_cap ids
+    int32_t* hland;         // its host twin (pinned), land_cap ids: every landed query's sorted ids
+    uint32_t* hqres;        // pinned host words per query, written by the kernel that finishes it: [0, nq) result
                             // count, [nq, 2 nq) landing offset, [2 nq, 3 nq) candidates (the device
                             // filter's count, beyond its segment when it overflowed), [3 nq, 4 nq) flags
     const uint32_t* dn;     // device filter's candidate counts (device-filtered batches), else null
@@ -105,12 +105,13 @@ void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2*
 void launch_scatter_counts(const void* src_sorted, bool narrow, const uint32_t* perm, uint32_t n,
                            uint32_t* dst_orig, hipStream_t st);
 
-// batched form: the whole step is the traversal launch (which sorts and
-// lands results of up to kResLds ids in the landing area on the device) and
-// the landing launch (sorts the larger ones, then writes every query's ids,
-// count and landing offset to pinned host memory); capture them in a graph.
-// After the step, hqres says where each query's sorted ids are in hland;
-// the landing kernel's last block clears land_ctr for the next run.
+// batched form: the whole step is the traversal launch (which sorts a result
+// that its block's registers and LDS hold and writes it, with the query's words, to
+// pinned host memory) and the landing launch (sorts the queries the traversal
+// left to it in the device landing area, then writes their ids and words to
+// host memory); capture them in a graph.  After the step, hqres says where
+// each query's sorted ids are in hland; the landing kernel's last block
+// clears land_ctr for the next run.
 size_t batch_query_lds_bytes(uint32_t n_levels);
 // block shapes of the traversal kernel (waves per query), chosen by the autotuner
 constexpr int kShape8Waves = 0;
