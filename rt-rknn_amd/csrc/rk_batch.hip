@@ -125,21 +125,6 @@ rk_status ensure_qres(rk_batch* b, uint64_t nq)
     return RK_OK;
 }
 
-// Blocks of the landing kernel: kLandGridMin, or one per query whose
-// expected result (k N_u / N_f ids: every user is in the reverse k-NN set
-// of exactly k facilities) is over half of what a traversal block sorts
-// itself -- those are sorted by the landing kernel.
-uint32_t landing_grid(const rk_dataset* ds, const uint32_t* k, size_t nq, const QDev* hq)
-{
-    const double per_k = ds->nf ? (double)ds->nu / (double)ds->nf : 0.0;
-    uint32_t big = 0;
-    for (size_t i = 0; i < nq; ++i) {
-        const uint32_t ki = k ? k[i] : hq[i].k;
-        if ((!hq || !hq[i].trivial) && ki * per_k > 0.5 * kQueryBlock * 8) ++big;
-    }
-    return std::max(kLandGridMin, std::min<uint32_t>(big, 1024));
-}
-
 // The landed extent of a finished run: every query that fit its segment has
 // its ids at [qland, qland + count), densely packed from 0.  *short_area:
 // some such query found the landing area full; *need: ids those queries
@@ -188,7 +173,6 @@ void set_view(rk_batch* b, uint64_t seg_cap)
     v.hland = b->land_host ? b->land_host->dev : nullptr;
     v.hqres = b->h_qres.dev;
     v.dn = b->dfilter ? b->dn.p : nullptr;
-    v.land_grid = b->land_grid;
     v.land_cap = b->land_cap;
     v.land_ctr = b->land_ctr.p;
     v.id_bits = ds->id_bits;
@@ -368,7 +352,6 @@ rk_status batch_fill(rk_batch* b, const rk_query_desc* qs, size_t nq)
                                      st));
         RK_HIP(c, hipStreamSynchronize(st));
     }
-    b->land_grid = landing_grid(ds, nullptr, nq, b->hq.data());
     set_view(b, seg_cap);
     return RK_OK;
 }
@@ -450,7 +433,6 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
         RK_HIP(c, hipMemcpyAsync(b->dk.p, b->s_dk.p, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
         RK_HIP(c, hipStreamSynchronize(st));
     }
-    b->land_grid = landing_grid(ds, k, nq, nullptr);
     set_view(b, seg_cap);
     return RK_OK;
 }

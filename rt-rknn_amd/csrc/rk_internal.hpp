@@ -203,12 +203,11 @@ struct rk_batch {
     rk::DevBuf<int32_t> seg;     // per-query result segments (spill of results beyond the kernel's LDS)
     rk::DevBuf<uint32_t> qres;   // [0, nq): results per query, [nq, 2 nq): landing offsets
     rk::DevBuf<uint32_t> qstat;
-    // landing area: each query's sorted ids go to land_host (pinned,
-    // coherent host memory) at the offset qres[nq + i], written by the
-    // traversal kernel, or by the run's last kernel for a query it sorts in
-    // dland (HBM) first; the kernel that finishes a query writes its
-    // kQresWords words to h_qres (pinned, coherent); land_ctr: the run's
-    // reservation counters
+    // landing area: the traversal kernel writes each query's sorted ids to
+    // land_host (pinned, coherent host memory) at the offset qres[nq + i]
+    // (a radix-sorted result passes through the same range of dland, in
+    // HBM, first) and its kQresWords words to h_qres (pinned, coherent);
+    // land_ctr: the run's reservation counters
     rk::HostBuf<uint32_t> h_qres;
     std::shared_ptr<rk::PinnedIds> land_host;
     // where a new host side comes from (at least the ids asked for); empty:
@@ -217,7 +216,6 @@ struct rk_batch {
     rk::DevBuf<int32_t> dland;
     rk::DevBuf<unsigned long long> land_ctr;
     uint64_t land_cap = 0;
-    uint32_t land_grid = 0;
     // inputs staged in pinned, coherent host memory by every fill: a batch
     // refilled step after step (direct) has its kernels read them in place,
     // with no copy; a prepared batch (resident) copies them to HBM once

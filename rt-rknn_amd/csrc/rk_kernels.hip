@@ -12,8 +12,6 @@
 // rknn-rt raygen/any-hit pair (rknn-rt/rt.cu:46-81,95-111).
 #include <hip/hip_runtime.h>
 #include <algorithm>
-#include <hipcub/block/block_radix_sort.hpp>
-#include <hipcub/hipcub.hpp>
 
 #include "rk_common.hpp"
 #include "rk_kernels.hpp"
@@ -643,10 +641,11 @@ __global__ void k_group_mbr(const float4* __restrict__ tiles, uint32_t n_tiles, 
 // One block of kW (8 or 4) waves per query.  The block builds the query's
 // candidate records, walks the bounding-rectangle tree breadth-first
 // through per-level frontiers in LDS, tests the surviving tiles and appends
-// kept users to the query's result segment, which it then radix-sorts (a
-// second launch sorts segments too large for the block's sort).  Every
-// counter is an LDS counter of the block: the batch
-// touches no shared global counter, so queries proceed independently.
+// kept users to the query's result (in LDS, spilling to its segment), which
+// it then sorts and writes to pinned host memory with the query's words.
+// Every counter is an LDS counter of the block; the only global atomics
+// are the landing reservation (one per query) and the block's completion
+// ticket, so queries proceed independently.
 //
 // Frontiers are bounded: the deepest level with pending nodes is always
 // expanded first, at most kPhaseItems nodes at a time, so a level's frontier
@@ -688,9 +687,10 @@ constexpr size_t kCandBytes = sizeof(SharedRecsIdx<kRecLds>) > kQWaves * sizeof(
                                   ? sizeof(SharedRecsIdx<kRecLds>)
                                   : kQWaves * sizeof(WaveLds);
 constexpr size_t kResBytes = kResLds * sizeof(uint32_t); // the query's result ids
-// After the descent the block sorts up to 8 result ids per thread itself (in
-// the LDS from the cover counts up to the frontiers, free by then); larger
-// results go to the landing kernel.
+// After the descent the block bucket-sorts up to 8 result ids per thread in
+// registers and LDS (from the cover counts up to the frontiers, free by
+// then); larger results, or ids bunched in one bucket, are radix-sorted
+// through the query's segment.
 constexpr int kInItems = 8;
 constexpr uint32_t kResBuckets = 1024; // buckets of the in-block sort
 static_assert((kQueryBlock * kInItems + kResBuckets) * sizeof(uint32_t) <= kCoverBytes + kCandBytes + kResBytes,
@@ -1076,6 +1076,110 @@ __device__ __forceinline__ void put_words(const BatchDev& b, uint32_t qi, uint32
     b.hqres[3 * b.nq + qi] = Q.trivial ? kQflagTrivial : 0u;
 }
 
+// Stable LSD radix sort for results the block's in-LDS sort does not take
+// (more ids than it holds, or ids bunched in one of its buckets): 8-bit
+// digits over the bits below the largest id, passes between the query's
+// segment and its range of the device landing area.  Each of the block's kW
+// waves owns a contiguous kW-th of the keys: it counts its digits, and the
+// block turns the counts into a start per (wave, digit) (digit order first,
+// wave order within a digit).  Then each wave scatters its keys 64 at a
+// time in order, a key's place among the equal digits of its chunk coming
+// from eight ballots.  The cost does not depend on how the ids are spread.
+constexpr uint32_t kRadixDigits = 256;
+
+template <uint32_t kW>
+__device__ void radix_pass(const int32_t* __restrict__ src, int32_t* __restrict__ dst, uint32_t n, uint32_t per,
+                           uint32_t shift, uint32_t flip, uint32_t (*hist)[kRadixDigits], uint32_t* red,
+                           uint32_t tid, uint32_t wave, uint32_t lane)
+{
+    static_assert(kW * 64 >= kRadixDigits, "a thread per digit");
+    const uint32_t lo = min(n, wave * per), hi = min(n, lo + per);
+    for (uint32_t d = lane; d < kRadixDigits; d += 64) hist[wave][d] = 0; // the wave's own row
+    for (uint32_t j = lo + lane; j < hi; j += 64) atomicAdd(&hist[wave][(((uint32_t)src[j] ^ flip) >> shift) & 255u], 1u);
+    __syncthreads();
+    // thread d < 256 (the first four waves): digit d's total, its start (a
+    // scan over digits) and each wave's start within the digit (a scan over
+    // waves)
+    uint32_t tot = 0, inc = 0;
+    if (tid < kRadixDigits) {
+        for (uint32_t w = 0; w < kW; ++w) {
+            const uint32_t c = hist[w][tid];
+            hist[w][tid] = tot;
+            tot += c;
+        }
+        inc = tot;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = (uint32_t)__shfl_up((int)inc, o);
+            if ((int)lane >= o) inc += v;
+        }
+        if (lane == 63) red[wave] = inc;
+    }
+    __syncthreads();
+    if (tid < kRadixDigits) {
+        uint32_t base = inc - tot;
+        for (uint32_t w = 0; w < wave; ++w) base += red[w];
+        for (uint32_t w = 0; w < kW; ++w) hist[w][tid] += base;
+    }
+    __syncthreads();
+    for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
+        const uint32_t j = c0 + lane;
+        const bool valid = j < hi;
+        const int32_t key = valid ? src[j] : 0;
+        const uint32_t d = (((uint32_t)key ^ flip) >> shift) & 255u;
+        uint64_t peers = ballot(valid); // lanes of this chunk with the same digit
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const uint64_t m = ballot((d >> bit) & 1u);
+            peers &= ((d >> bit) & 1u) ? m : ~m;
+        }
+        const uint32_t r = rank_below(peers);
+        if (valid) dst[hist[wave][d] + r] = key;
+        if (valid && (peers >> lane) == 1) hist[wave][d] += r + 1; // the digit's last lane moves its start
+    }
+    __syncthreads();
+}
+
+// Sort the n ids of a query's segment into its landing range [start,
+// start + n) on the device, then write them to the same range of the host
+// area (16-byte lanes from the first 16-byte boundary; both areas are
+// aligned alike).  Ids are below 2^id_bits (non-negative), or any int32
+// (id_bits = 32: the sign bit flipped so that unsigned digit order is id
+// order); the passes alternate so that the last one ends in the landing
+// range.  Called by the whole block, with the thread's index, wave and lane.
+template <uint32_t kW>
+__device__ void radix_land(const BatchDev& b, int32_t* seg, uint32_t n, uint32_t start,
+                           uint32_t (*hist)[kRadixDigits], uint32_t* red, uint32_t tid, uint32_t wave, uint32_t lane)
+{
+    constexpr uint32_t kThreads = kW * 64;
+    const uint32_t bits = b.id_bits < 32 ? b.id_bits : 32;
+    const uint32_t flip = bits == 32 ? 0x80000000u : 0u;
+    const uint32_t passes = (bits + 7) / 8;
+    const uint32_t per = (n + kW - 1) / kW;
+    int32_t* dst = b.dland + start;
+    int32_t* from = seg;
+    int32_t* to = dst;
+    if ((passes & 1u) == 0) { // an even pass count ends where it starts: start in the landing range
+        for (uint32_t i = tid; i < n; i += kThreads) dst[i] = seg[i];
+        __syncthreads();
+        from = dst;
+        to = seg;
+    }
+    for (uint32_t p = 0; p < passes; ++p) {
+        radix_pass<kW>(from, to, n, per, 8 * p, flip, hist, red, tid, wave, lane);
+        int32_t* t = from;
+        from = to;
+        to = t;
+    }
+    int32_t* host = b.hland + start;
+    const uint32_t head = min(n, (4u - (start & 3u)) & 3u);
+    if (tid < head) host[tid] = dst[tid];
+    const uint32_t nv = (n - head) / 4;
+    const int4* s4 = reinterpret_cast<const int4*>(dst + head);
+    int4* d4 = reinterpret_cast<int4*>(host + head);
+    for (uint32_t i = tid; i < nv; i += kThreads) d4[i] = s4[i];
+    for (uint32_t t = head + nv * 4 + tid; t < n; t += kThreads) host[t] = dst[t];
+}
+
 // kW waves per query.  With 8-wave blocks at eight waves per SIMD (<= 64
 // VGPRs), a batch of 1024 queries is resident at once on 256 CUs; 4-wave
 // blocks may use twice the registers.  The autotuner picks the shape per
@@ -1089,6 +1193,7 @@ void k_batch_query(DsDev ds, BatchDev b)
     constexpr uint32_t kThreads = kW * 64;
     constexpr uint32_t kInSortMax = kThreads * kInItems; // result ids the block sorts itself
     static_assert(kInSortMax >= kResLds, "the in-block sort takes the whole LDS result buffer");
+    static_assert((kW * kRadixDigits + kW) * sizeof(uint32_t) <= kCandBytes, "radix histograms in the candidate area");
     extern __shared__ __align__(16) unsigned char smem[];
     QueryCtl& ctl = *reinterpret_cast<QueryCtl*>(smem);
     uint32_t* ctl_words = reinterpret_cast<uint32_t*>(smem);
@@ -1181,7 +1286,7 @@ void k_batch_query(DsDev ds, BatchDev b)
         }
         if (tid < kMaxLevels) stat[1 + tid] = ctl.items[tid];
         int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
-        uint32_t later = 0; // the landing kernel sorts and lands this query
+        uint32_t radix = 0; // the result goes through the radix sort
         uint32_t landed = kNotLanded;
         if (nres == 0) {
             landed = 0;
@@ -1202,251 +1307,45 @@ void k_batch_query(DsDev ds, BatchDev b)
                 // a full landing area: the sorted ids stay in the segment (the run is repeated)
                 int32_t* dst = landed != kNotLanded ? b.hland + landed : seg;
                 for (uint32_t i = tid; i < nres; i += kThreads) dst[i] = (int32_t)(s_out[i] ^ 0x80000000u);
-            } else { // bunched ids: the landing kernel's sort, from the segment
+            } else { // bunched ids: to the segment for the radix sort
 #pragma unroll
                 for (int i = 0; i < kInItems; ++i) {
                     const uint32_t j = tid + i * kThreads;
                     if (j < nres) seg[j] = (int32_t)(key[i] ^ 0x80000000u);
                 }
-                later = 1;
+                radix = 1;
             }
             // the cover counts start at zero for the next query
             __syncthreads();
             for (uint32_t i = tid; i < kPhaseItems * 64; i += kThreads) cover[i] = 0;
         } else if (nres <= b.seg_cap) { // the LDS part joins the spill in the segment
             for (uint32_t i = tid; i < kResLds; i += kThreads) seg[i] = (int32_t)res[i];
-            later = 1;
+            radix = 1;
+        }
+        if (radix) {
+            __syncthreads(); // the segment holds every id
+            if (tid == 0) ctl.land = reserve_landing(b, nres);
+            __syncthreads();
+            landed = ctl.land;
+            // a full landing area: the ids stay in the segment (the run is repeated)
+            if (landed != kNotLanded) {
+                // the radix histograms in the candidate area (free after the descent)
+                auto* hist = reinterpret_cast<uint32_t(*)[kRadixDigits]>(cand);
+                radix_land<kW>(b, seg, nres, landed, hist, reinterpret_cast<uint32_t*>(cand) + kW * kRadixDigits, tid,
+                               wave, lane);
+            }
         }
         if (tid == 0) {
-            stat[kSortSlot] = later;
-            b.qland[qi] = later ? kNotLanded : landed;
-            if (!later) put_words(b, qi, nres, landed, Q);
+            stat[kSortSlot] = radix;
+            b.qland[qi] = landed;
+            put_words(b, qi, nres, landed, Q);
         }
         __syncthreads(); // before the next query resets the state
     }
-}
-
-// Sorting in the step's last launch (k_batch_land): a query the traversal
-// left to it is sorted there, up to kDeviceSortMax ids bucket-sorted in LDS
-// (rocPRIM's block radix sort via hipCUB when they bunch), larger segments
-// by LSD radix passes between the segment and the device landing area.
-constexpr int kSortBlock = 512;
-constexpr uint32_t kSortBuckets = 4096;
-static_assert(kDeviceSortMax == kSortBlock * 16, "device sort capacity");
-using SortLarge = hipcub::BlockRadixSort<int32_t, kSortBlock, 16>;
-
-// Stable LSD radix sort passes for segments beyond kDeviceSortMax: 8-bit
-// digits over the bits below the largest id.  Each of the block's waves owns
-// a contiguous eighth of the keys: it counts its digits, and the block turns
-// the counts into a start per (wave, digit) (digit order first, wave order
-// within a digit).  Then each wave scatters its keys 64 at a time in order,
-// a key's place among the equal digits of its chunk coming from eight
-// ballots.  The cost does not depend on how the ids are spread.
-constexpr uint32_t kLandWaves = kSortBlock / 64;
-constexpr uint32_t kRadixDigits = 256;
-
-__device__ void radix_pass(const int32_t* __restrict__ src, int32_t* __restrict__ dst, uint32_t n, uint32_t per,
-                           uint32_t shift, uint32_t flip, uint32_t (*hist)[kRadixDigits], uint32_t* red)
-{
-    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
-    const uint32_t lo = min(n, wave * per), hi = min(n, lo + per);
-    for (uint32_t d = lane; d < kRadixDigits; d += 64) hist[wave][d] = 0; // the wave's own row
-    for (uint32_t j = lo + lane; j < hi; j += 64) atomicAdd(&hist[wave][(((uint32_t)src[j] ^ flip) >> shift) & 255u], 1u);
-    __syncthreads();
-    // thread d < 256 (waves 0-3): digit d's total, its start (a scan over
-    // digits) and each wave's start within the digit (a scan over waves)
-    uint32_t tot = 0, inc = 0;
-    if (tid < kRadixDigits) {
-        for (uint32_t w = 0; w < kLandWaves; ++w) {
-            const uint32_t c = hist[w][tid];
-            hist[w][tid] = tot;
-            tot += c;
-        }
-        inc = tot;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t v = (uint32_t)__shfl_up((int)inc, o);
-            if ((int)lane >= o) inc += v;
-        }
-        if (lane == 63) red[wave] = inc;
-    }
-    __syncthreads();
-    if (tid < kRadixDigits) {
-        uint32_t base = inc - tot;
-        for (uint32_t w = 0; w < wave; ++w) base += red[w];
-        for (uint32_t w = 0; w < kLandWaves; ++w) hist[w][tid] += base;
-    }
-    __syncthreads();
-    for (uint32_t c0 = lo; c0 < hi; c0 += 64) {
-        const uint32_t j = c0 + lane;
-        const bool valid = j < hi;
-        const int32_t key = valid ? src[j] : 0;
-        const uint32_t d = (((uint32_t)key ^ flip) >> shift) & 255u;
-        uint64_t peers = ballot(valid); // lanes of this chunk with the same digit
-#pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
-            const uint64_t m = ballot((d >> bit) & 1u);
-            peers &= ((d >> bit) & 1u) ? m : ~m;
-        }
-        const uint32_t r = rank_below(peers);
-        if (valid) dst[hist[wave][d] + r] = key;
-        if (valid && (peers >> lane) == 1) hist[wave][d] += r + 1; // the digit's last lane moves its start
-    }
-    __syncthreads();
-}
-
-union LandLds {
-    struct {
-        uint32_t keys[kDeviceSortMax];
-        uint32_t cnt[kSortBuckets];
-        uint32_t red[2 * kSortBlock / 64 + 1];
-    } bk;
-    typename SortLarge::TempStorage radix;
-    struct {
-        uint32_t hist[kLandWaves][kRadixDigits];
-        uint32_t red[kLandWaves];
-    } rx;
-};
-
-// Sort query qi's ids, left in its segment by the traversal (n <= seg_cap),
-// into a landing area range reserved here; returns the range's start, or
-// kNotLanded when the area is full.  Called by the whole block.
-__device__ uint32_t sort_left(const BatchDev& b, uint32_t qi, LandLds& lds, uint32_t& s_start)
-{
-    const uint32_t tid = threadIdx.x;
-    const uint32_t n = b.qcount[qi];
-    int32_t* seg = b.seg + (uint64_t)qi * b.seg_cap;
-    if (tid == 0) s_start = reserve_landing(b, n);
-    __syncthreads();
-    const uint32_t start = s_start;
-    if (start != kNotLanded) {
-        int32_t* dst = b.dland + start;
-        if (n <= kDeviceSortMax) {
-            uint32_t key[16];
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const uint32_t j = tid + i * kSortBlock;
-                key[i] = j < n ? ((uint32_t)seg[j] ^ 0x80000000u) : 0u;
-            }
-            if (bucket_sort_regs<kSortBlock, 16, kSortBuckets>(key, n, lds.bk.keys, lds.bk.cnt, lds.bk.red)) {
-                for (uint32_t i = tid; i < n; i += kSortBlock) dst[i] = (int32_t)(lds.bk.keys[i] ^ 0x80000000u);
-            } else {
-                __syncthreads(); // every thread has read the bucket sort's flag: the union is free
-                int32_t k2[16];
-                const uint32_t base = tid * 16;
-#pragma unroll
-                for (int i = 0; i < 16; ++i) k2[i] = base + i < n ? seg[base + i] : INT32_MAX;
-                // only the bits below the largest id; padding sorts last (stable, placed last)
-                SortLarge(lds.radix).SortBlockedToStriped(k2, 0, (int)b.id_bits);
-#pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    const uint32_t j = tid + i * kSortBlock;
-                    if (j < n) dst[j] = k2[i];
-                }
-            }
-        } else {
-            // ids below 2^id_bits (non-negative), or any int32 (id_bits = 32:
-            // the sign bit flipped so that unsigned digit order is id order);
-            // passes alternate between the segment and the landing area so
-            // that the last one ends in the landing area
-            const uint32_t bits = b.id_bits < 32 ? b.id_bits : 32;
-            const uint32_t flip = bits == 32 ? 0x80000000u : 0u;
-            const uint32_t passes = (bits + 7) / 8;
-            const uint32_t per = (n + kLandWaves - 1) / kLandWaves;
-            int32_t* from = seg;
-            int32_t* to = dst;
-            if ((passes & 1u) == 0) { // an even pass count ends where it starts: start in the landing area
-                for (uint32_t i = tid; i < n; i += kSortBlock) dst[i] = seg[i];
-                __syncthreads();
-                from = dst;
-                to = seg;
-            }
-            for (uint32_t p = 0; p < passes; ++p) {
-                radix_pass(from, to, n, per, 8 * p, flip, lds.rx.hist, lds.rx.red);
-                int32_t* t = from;
-                from = to;
-                to = t;
-            }
-        }
-    }
-    __syncthreads(); // the sorted ids are all in place
-    return start;
-}
-
-// One wave writes the ids of query qi, sorted by this launch into the device
-// landing area, to host memory, and the query's four words.  The wave's
-// stores complete before it returns: a bounded number of writes in flight
-// to PCIe.
-__device__ __forceinline__ void land_out(const BatchDev& b, uint32_t qi, uint32_t lane)
-{
-    const QDev& Q = b.q[qi];
-    const uint32_t n = b.qcount[qi], start = b.qland[qi];
-    if (start != kNotLanded) {
-        const int32_t* src = b.dland + start;
-        int32_t* dst = b.hland + start;
-        // 16-byte lanes from the first 16-byte boundary (both areas are aligned alike)
-        const uint32_t head = min(n, (4u - (start & 3u)) & 3u);
-        if (lane < head) dst[lane] = src[lane];
-        const uint32_t nv = (n - head) / 4;
-        const int4* s4 = reinterpret_cast<const int4*>(src + head);
-        int4* d4 = reinterpret_cast<int4*>(dst + head);
-        uint32_t i = lane;
-        for (; i + 192 < nv; i += 256) { // four loads in flight before the stores
-            const int4 a0 = s4[i], a1 = s4[i + 64], a2 = s4[i + 128], a3 = s4[i + 192];
-            d4[i] = a0;
-            d4[i + 64] = a1;
-            d4[i + 128] = a2;
-            d4[i + 192] = a3;
-        }
-        for (; i < nv; i += 64) d4[i] = s4[i];
-        for (uint32_t t = head + nv * 4 + lane; t < n; t += 64) dst[t] = src[t];
-    }
-    if (lane == 0) put_words(b, qi, n, start, Q);
-    __threadfence_system();
-}
-
-// The step's last launch: a few blocks (BatchDev::land_grid), block g taking
-// queries g, g + gridDim.x, ...  It handles the queries the traversal left
-// to it (flagged in their statistics): results beyond the traversal block's
-// LDS buffer, or ids its bucket sort could not place.  Every other query's
-// ids and words are in host memory already, written by the traversal.
-__global__ __launch_bounds__(kSortBlock) void k_batch_land(BatchDev b)
-{
-    __shared__ LandLds lds;
-    __shared__ uint32_t s_start;
-    __shared__ uint32_t s_list[kSortBlock]; // flagged queries of the chunk
-    __shared__ uint32_t s_wn[kLandWaves];
-    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = lane_id();
-    const uint64_t G = gridDim.x;
-    // the block's queries kSortBlock at a time: a thread each reads a flag,
-    // the flagged ones are listed in LDS, sorted by the block one at a time,
-    // then written out a wave each
-    for (uint64_t j0 = 0; blockIdx.x + j0 * G < b.nq; j0 += kSortBlock) {
-        const uint64_t qt = blockIdx.x + (j0 + tid) * G;
-        const bool flagged = qt < b.nq && b.qstat[qt * kQueryStats + kSortSlot] != 0;
-        const uint64_t m = ballot(flagged);
-        if (lane == 0) s_wn[wave] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t off = 0, n_list = 0;
-        for (uint32_t w = 0; w < kLandWaves; ++w) {
-            const uint32_t c = s_wn[w];
-            off += w < wave ? c : 0u;
-            n_list += c;
-        }
-        if (flagged) s_list[off + rank_below(m)] = (uint32_t)qt;
-        __syncthreads();
-        for (uint32_t li = 0; li < n_list; ++li) {
-            const uint32_t qi = s_list[li];
-            const uint32_t start = sort_left(b, qi, lds, s_start);
-            if (tid == 0) b.qland[qi] = start;
-            __syncthreads(); // before the next query reuses the block's LDS
-        }
-        for (uint32_t li = wave; li < n_list; li += kLandWaves) land_out(b, s_list[li], lane);
-        __syncthreads(); // every wave is done with the chunk's list and counts
-    }
-    // the last block to finish clears the counters for the next run (every
-    // reservation of this run is done by then)
-    if (tid == 0) {
-        __threadfence_system();
+    // the last block to finish clears the reservation counters for the next
+    // run (every reservation of this run is done by then)
+    if (threadIdx.x == 0) {
+        __threadfence();
         if (atomicAdd(&b.land_ctr[1], 1ull) == (unsigned long long)gridDim.x - 1) {
             b.land_ctr[0] = 0;
             b.land_ctr[1] = 0;
@@ -1565,8 +1464,6 @@ void launch_batch_query(const DsDev& ds, const BatchDev& b, int shape, hipStream
         k_batch_query<4><<<grid, 256, batch_query_lds_bytes(ds.n_levels), st>>>(ds, b);
     else
         k_batch_query<8><<<grid, 512, batch_query_lds_bytes(ds.n_levels), st>>>(ds, b);
-    const uint32_t lg = b.land_grid ? b.land_grid : kLandGridMin;
-    k_batch_land<<<grid < lg ? grid : lg, kSortBlock, 0, st>>>(b);
 }
 
 } // namespace rk

@@ -15,16 +15,14 @@ constexpr int kBuckets = 64;
 // Batched form: one block of up to kQueryBlock threads per query; tree frontiers
 // of kFront nodes per level in LDS.  A query's result ids stay in LDS (up to
 // kResLds of them), are sorted there and written once to the batch's landing
-// area in host memory; larger results spill to the query's segment in HBM
-// and are sorted and landed by a second kernel (bucket sort up to
-// kDeviceSortMax ids, LSD radix sort beyond).
+// area in host memory; larger results spill to the query's segment in HBM,
+// and a result the block's in-LDS sort does not take is radix-sorted by the
+// same block through the segment and the device landing area.
 constexpr int kQueryBlock = 512;
 constexpr uint32_t kFront = 512;
 constexpr uint32_t kResLds = 2048;
-constexpr uint32_t kDeviceSortMax = 8192;
 // per-query statistics: [0] binary64 tests, [1 + l] nodes entering level l,
-// [kSortSlot] nonzero when the traversal kernel left the segment to the
-// landing kernel
+// [kSortSlot] nonzero when the block radix-sorted the result
 constexpr uint32_t kSortSlot = 1 + kMaxLevels;
 constexpr uint32_t kQueryStats = 2 + kMaxLevels;
 // landing offset of a query whose ids did not reach the landing area (the
@@ -35,9 +33,6 @@ constexpr uint64_t kMaxLandIds = 0xfffffff0ull;
 // words per query of BatchDev::hqres, and its flag bits
 constexpr uint32_t kQresWords = 4;
 constexpr uint32_t kQflagTrivial = 1u; // k > |C|: every user qualifies (no traversal ran)
-// landing-kernel blocks: this many at least, more when many results are
-// larger than a traversal block sorts itself
-constexpr uint32_t kLandGridMin = 32;
 
 struct BatchDev {
     const QDev* q;          // nq queries
@@ -53,15 +48,14 @@ struct BatchDev {
     uint32_t* qcount;       // results per query (may exceed seg_cap: overflow)
     uint32_t* qland;        // per query: offset of its sorted ids in land[], or kNotLanded
     uint32_t* qstat;        // nq x kQueryStats statistics
-    int32_t* dland;         // landing area on the device (ids the landing kernel sorts), land_cap ids
+    int32_t* dland;         // landing area on the device (radix-sorted results), land_cap ids
     int32_t* hland;         // its host twin (pinned), land_cap ids: every landed query's sorted ids
-    uint32_t* hqres;        // pinned host words per query, written by the kernel that finishes it: [0, nq) result
+    uint32_t* hqres;        // pinned host words per query, written by the traversal kernel: [0, nq) result
                             // count, [nq, 2 nq) landing offset, [2 nq, 3 nq) candidates (the device
                             // filter's count, beyond its segment when it overflowed), [3 nq, 4 nq) flags
     const uint32_t* dn;     // device filter's candidate counts (device-filtered batches), else null
-    uint32_t land_grid;     // blocks of the landing kernel
     uint64_t land_cap;
-    unsigned long long* land_ctr; // [0] ids reserved by this run, [1] landing-kernel blocks done
+    unsigned long long* land_ctr; // [0] ids reserved by this run, [1] traversal blocks done
     uint32_t id_bits;       // user ids are non-negative and below 2^id_bits (32: any int32)
 };
 
@@ -105,13 +99,11 @@ void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2*
 void launch_scatter_counts(const void* src_sorted, bool narrow, const uint32_t* perm, uint32_t n,
                            uint32_t* dst_orig, hipStream_t st);
 
-// batched form: the whole step is the traversal launch (which sorts a result
-// that its block's registers and LDS hold and writes it, with the query's words, to
-// pinned host memory) and the landing launch (sorts the queries the traversal
-// left to it in the device landing area, then writes their ids and words to
-// host memory); capture them in a graph.  After the step, hqres says where
-// each query's sorted ids are in hland; the landing kernel's last block
-// clears land_ctr for the next run.
+// batched form: the whole step is one launch of the traversal kernel, which
+// sorts each query's result and writes its ids and words to pinned host
+// memory; captured in a graph.  After the step, hqres says where each query's
+// sorted ids are in hland; the kernel's last block clears land_ctr for the
+// next run.
 size_t batch_query_lds_bytes(uint32_t n_levels);
 // block shapes of the traversal kernel (waves per query), chosen by the autotuner
 constexpr int kShape8Waves = 0;

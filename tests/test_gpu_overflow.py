@@ -102,9 +102,10 @@ def test_served_beyond_both_segments(rk, ctx, data, monkeypatch):
 @pytest.mark.parametrize("negative_ids", [False, True])
 @pytest.mark.parametrize("filt", ["device", "host"])
 def test_large_results_sorted_on_the_device(rk, ctx, filt, negative_ids, monkeypatch):
-    """Results beyond the in-step sort (8,192 ids) are sorted by the
-    fetch-time radix sort: 1,000 users per facility, k up to 64, ids
-    positive (26 bits) or spread over the whole int32 range with signs."""
+    """Results beyond the traversal block's in-LDS sort (4,096 ids) are
+    radix-sorted by the same block through its segment and the device
+    landing area: 1,000 users per facility, k up to 64, ids positive (26
+    bits) or spread over the whole int32 range with signs."""
     monkeypatch.setenv("RK_FILTER", filt)
     seed = rk.default_seed(2)
     ux, uy, uid, fx, fy, fid = rk.generate(2, 400_000, 400, seed=seed)

@@ -255,10 +255,10 @@ def test_medium_random_vs_oracle(rk, ctx, config):
 
 def test_batch_sort_paths(rk, ctx):
     """Result segments of every size class and id layout against the oracle:
-    sorted by the traversal kernel (<= 4096 ids), by the large-segment kernel
-    (<= 8192, bucket sort or its radix fallback when ids bunch unevenly in
-    their range) and by the host beyond that; random, spatially coherent with
-    far outliers, and negative ids."""
+    up to 4,096 ids bucket-sorted in LDS by the traversal block, or radix-sorted
+    by it when they bunch unevenly in their range; beyond that (4,097-8,192
+    and more) radix-sorted by the block through its segment; random, spatially
+    coherent with far outliers, and negative ids."""
     r = np.random.default_rng(11)
     nu, nf = 300_000, 200
     ux = (r.random(nu) * 1000).astype(np.float32)
