@@ -253,12 +253,15 @@ def test_medium_random_vs_oracle(rk, ctx, config):
         ds.close()
 
 
-def test_batch_sort_paths(rk, ctx):
+@pytest.mark.parametrize("form", ["one-call", "4-wave"])
+def test_batch_sort_paths(rk, ctx, form, monkeypatch):
     """Result segments of every size class and id layout against the oracle:
-    up to 4,096 ids bucket-sorted in LDS by the traversal block, or radix-sorted
-    by it when they bunch unevenly in their range; beyond that (4,097-8,192
-    and more) radix-sorted by the block through its segment; random, spatially
-    coherent with far outliers, and negative ids."""
+    up to 4,096 ids (2,048 in the 4-wave shape) bucket-sorted in LDS by the
+    traversal block, or radix-sorted by it when they bunch unevenly in their
+    range; beyond that radix-sorted by the block through its segment; random,
+    spatially coherent with far outliers, and negative ids.  The one-call
+    form (device filter, 8 waves per query) and a prepared batch with host
+    candidate lists in the 4-wave shape."""
     r = np.random.default_rng(11)
     nu, nf = 300_000, 200
     ux = (r.random(nu) * 1000).astype(np.float32)
@@ -281,8 +284,18 @@ def test_batch_sort_paths(rk, ctx):
         uid = ids.astype(np.int32)
         ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
         try:
-            print("sort paths: ids", name, flush=True)
-            got = ds.query_batch([(int(q), k) for q, k in zip(qs, ks)])
+            print("sort paths:", form, "ids", name, flush=True)
+            if form == "one-call":
+                got = ds.query_batch([(int(q), k) for q, k in zip(qs, ks)])
+            else:
+                monkeypatch.setenv("RK_AUTOTUNE", "4")
+                b = ds.prepare_batch([(int(q), k, ds.filter(int(q), k)) for q, k in zip(qs, ks)])
+                try:
+                    assert b.waves_per_query() == 4
+                    b.run()
+                    got = b.fetch()
+                finally:
+                    b.close()
             for q, k, g in zip(qs, ks, got):
                 exp = O.rknn(ux, uy, uid, fx, fy, int(q), k)
                 sizes.append(len(exp))
@@ -290,7 +303,8 @@ def test_batch_sort_paths(rk, ctx):
         finally:
             ds.close()
     sizes = np.array(sizes)
-    assert (sizes <= 4096).any() and ((sizes > 4096) & (sizes <= 8192)).any() and (sizes > 8192).any()
+    assert (sizes <= 2048).any() and ((sizes > 2048) & (sizes <= 4096)).any()
+    assert ((sizes > 4096) & (sizes <= 8192)).any() and (sizes > 8192).any()
 
 
 def test_autotuned_shapes_agree(rk, ctx, tmp_path, monkeypatch):
