@@ -613,7 +613,7 @@ rk_status download(rk_ctx* c, void* dst, const void* src, size_t bytes)
     RK_HIP(c, c->ring.reserve(2 * kChunk));
     for (hipEvent_t& e : c->ring_ev)
         if (!e) RK_HIP(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    if (!c->gang) c->gang.reset(new Gang(std::min(8u, std::max(1u, std::thread::hardware_concurrency()))));
+    Gang& gang = host_gang(c);
     const size_t n = (bytes + kChunk - 1) / kChunk;
     auto issue = [&](size_t i) -> rk_status {
         const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
@@ -629,8 +629,8 @@ rk_status download(rk_ctx* c, void* dst, const void* src, size_t bytes)
         const size_t off = i * kChunk, len = std::min(kChunk, bytes - off);
         const unsigned char* from = c->ring.p + (i & 1) * kChunk;
         unsigned char* to = (unsigned char*)dst + off;
-        const unsigned t = c->gang->size();
-        c->gang->run([&](unsigned k) {
+        const unsigned t = gang.size();
+        gang.run([&](unsigned k) {
             const size_t lo = len * k / t, hi = len * (k + 1) / t;
             std::memcpy(to + lo, from + lo, hi - lo);
         });

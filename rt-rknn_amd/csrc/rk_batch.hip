@@ -18,7 +18,6 @@
 #include <map>
 #include <mutex>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "rk_internal.hpp"
@@ -564,19 +563,22 @@ rk_status run_on_stream(rk_batch* b)
     return RK_OK;
 }
 
-// Copies of (dst, src, n) id ranges, split over a few host threads when
-// large (one thread copies ~10 GB/s; a step's results are megabytes).
-
-void copy_ranges(const std::vector<IdRange>& r, uint64_t total)
+// Copies of (dst, src, n) id ranges, split over the context's host threads
+// when large (one thread copies ~10 GB/s into touched pages, less into fresh
+// ones, whose first-touch faults the threads share; a step's results are
+// megabytes).  Caller holds the context lock.
+void copy_ranges(rk_ctx* c, const std::vector<IdRange>& r, uint64_t total)
 {
-    constexpr uint64_t kPerThread = 1ull << 18; // ids
-    const unsigned t = (unsigned)std::min<uint64_t>(8, total / kPerThread);
-    if (t < 2) {
+    constexpr uint64_t kPerThread = 1ull << 16; // ids
+    if (total < 2 * kPerThread) {
         for (const IdRange& x : r) std::memcpy(x.dst, x.src, x.n * sizeof(int32_t));
         return;
     }
-    // thread k copies the ids [k, k+1) * total / t of the concatenated ranges
-    auto part = [&](unsigned k) {
+    Gang& gang = host_gang(c);
+    const unsigned t = (unsigned)std::min<uint64_t>(gang.size(), total / kPerThread);
+    // part k < t copies the ids [k, k+1) * total / t of the concatenated ranges
+    gang.run([&](unsigned k) {
+        if (k >= t) return;
         const uint64_t lo = total * k / t, hi = total * (k + 1) / t;
         uint64_t at = 0;
         for (const IdRange& x : r) {
@@ -585,13 +587,8 @@ void copy_ranges(const std::vector<IdRange>& r, uint64_t total)
             at += x.n;
             if (at >= hi) break;
         }
-    };
-    std::vector<std::thread> pool;
-    for (unsigned k = 1; k < t; ++k) pool.emplace_back(part, k);
-    part(0);
-    for (std::thread& th : pool) th.join();
+    });
 }
-
 
 } // namespace rk
 
@@ -818,7 +815,7 @@ rk_status fetch_impl(rk_batch* b, int32_t* out_ids, size_t cap, uint64_t* offset
     }
     offsets[nq] = total;
     if (total > cap) return fail(c, RK_ERR_CAPACITY, "output buffer too small; offsets[nq] holds the size needed");
-    copy_ranges(ranges, total);
+    copy_ranges(c, ranges, total);
     return RK_OK;
 }
 

@@ -147,8 +147,18 @@ struct rk_ctx {
     // DMA overlaps the host threads' copy of the previous chunk
     rk::HostBuf<unsigned char> ring;
     hipEvent_t ring_ev[2] = {nullptr, nullptr};
+    // host threads for large host copies (started on first use; one job at
+    // a time, under the context lock)
     std::unique_ptr<rk::Gang> gang;
 };
+
+// The context's host threads, started on first use.  Caller holds the
+// context lock.
+inline rk::Gang& host_gang(rk_ctx* c)
+{
+    if (!c->gang) c->gang.reset(new rk::Gang(std::min(8u, std::max(1u, std::thread::hardware_concurrency()))));
+    return *c->gang;
+}
 
 struct rk_dataset {
     rk_ctx* ctx = nullptr;
