@@ -294,8 +294,15 @@ std::shared_ptr<rk::PinnedIds> pooled_area(const std::shared_ptr<AreaPool>& pool
                 pool->free.erase(pool->free.begin() + (ptrdiff_t)i);
                 break;
             }
-        // areas of one size: any free one fits any batch
+        // areas of one size: any free one fits any batch; a new largest
+        // size is rounded up to a power of two, so that growing steps
+        // allocate on the device stage a few times, not at every new maximum
         want = std::max(n, pool->largest);
+        if (!a && want > pool->largest) {
+            size_t p2 = 1;
+            while (p2 < want) p2 <<= 1;
+            want = p2;
+        }
         if (!a) pool->largest = want;
         if (pool->free.size() < kSpare) pool->cv.notify_one();
     }
@@ -307,11 +314,14 @@ std::shared_ptr<rk::PinnedIds> pooled_area(const std::shared_ptr<AreaPool>& pool
         }
     }
     return std::shared_ptr<rk::PinnedIds>(a, [pool](rk::PinnedIds* x) {
-        std::lock_guard<std::mutex> lk(pool->mu);
-        if (pool->free.size() < kAreasKeep)
-            pool->free.push_back(x);
-        else
-            delete x;
+        {
+            std::lock_guard<std::mutex> lk(pool->mu);
+            if (pool->free.size() < kAreasKeep) {
+                pool->free.push_back(x);
+                return;
+            }
+        }
+        delete x; // freeing pinned memory can wait for the device: not under the pool's lock
     });
 }
 
