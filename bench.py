@@ -12,7 +12,9 @@ flight, each step's results taken in place (rk_batch_results) after its wait;
 `value_fetch` is the same with every step's ids copied into a caller buffer
 (rk_batch_fetch).  The JSON line also carries:
   roofline      dense counting kernel (count-all: every user x every candidate,
-                4 fp32 flops per test) timed with HIP events on its stream;
+                4 fp32 flops per test): the first queries' count launches back
+                to back in one rk_count_all_many call, HIP events on the
+                kernel's stream around them, divided by the launch count;
                 kmix_tflops: the same kernel over the k ~ U{1..50} lists
   count_all_call_ms  the whole count-all call, counts in caller order and in
                 the dataset's order, with the copy of 10M counts to the host
@@ -336,17 +338,27 @@ def main():
     # every variant (4, 6 or 8 users per lane; 3 VALU
     # instructions per clean test)
     variants = {}
-    kernel_names = {"valu4": "k_count_all<4,1>", "valu6": "k_count_all<6,1>", "valu8": "k_count_all<8,1>"}
+    # (8-bit counts: the k=10 lists hold fewer than 256 candidates)
+    kernel_names = {"valu4": "k_count_all<4,1,uint8_t>", "valu6": "k_count_all<6,1,uint8_t>",
+                    "valu8": "k_count_all<8,1,uint8_t>"}
     kts = {kern: [] for kern in kernel_names}
     flops = {kern: 0.0 for kern in kernel_names}
+    n_roof = min(args.roofline_queries, args.queries)
+    roof_queries = [(qs[i], cands[i]) for i in range(n_roof)]
+    roof_out = np.zeros((n_roof, args.users), dtype=np.uint32)
     for _ in range(3):  # the variants interleaved, three rounds each
         for kern in kernel_names:
             os.environ["RK_COUNT_KERNEL"] = kern
-            for i in range(min(args.roofline_queries, args.queries)):
-                _, t = ds.count_all(qs[i], cands[i], timing=True)
-                kts[kern].append(t.kernel_s)
-                flops[kern] += 4.0 * args.users * t.n_candidates
+            # the queries' count launches back to back on the kernel's stream,
+            # HIP events around them: kernel_s / n_roof is the average launch
+            _, t = ds.count_all_many(roof_queries, timing=True, out=roof_out)
+            kts[kern].append(t.kernel_s / n_roof)
+            flops[kern] += 4.0 * args.users * t.n_candidates / n_roof
     del os.environ["RK_COUNT_KERNEL"]
+    for i in range(n_roof):  # the call's counts are the single-query call's
+        if not np.array_equal(roof_out[i], ds.count_all(qs[i], cands[i])):
+            log("ERROR: count-all over many queries differs from the one-query call")
+            sys.exit(1)
     # the whole count-all call, counts in caller order (a scatter pass after
     # the kernel) and in the dataset's order (the kernel's output as it is),
     # each including its device-to-host copy of nu counts
@@ -367,7 +379,7 @@ def main():
             f"order + copy of {args.users} counts {call_ms[name]['order_and_copy']} ms)")
     for kern, kt in kts.items():
         variants[kern] = (flops[kern] / sum(kt) / 1e12, float(np.mean(kt)), flops[kern] / len(kt))
-        log(f"count-all kernel ({kern}): {variants[kern][0]:.2f} TFLOP/s over {len(kt)} launches, "
+        log(f"count-all kernel ({kern}): {variants[kern][0]:.2f} TFLOP/s over {len(kt) * n_roof} launches, "
             f"avg {variants[kern][1] * 1e3:.4f} ms")
     best = max(variants, key=lambda v: variants[v][0])
     achieved, avg_s, flops_launch = variants[best]
@@ -382,11 +394,9 @@ def main():
     # the same kernel over the batched workload's own k mix (k ~ U{1..50}: the
     # mixed-k step's candidate lists), flop-weighted over its queries
     os.environ["RK_COUNT_KERNEL"] = best
-    kmix_s, kmix_f = 0.0, 0.0
-    for i in range(min(2 * args.roofline_queries, args.queries)):
-        _, t = ds.count_all(qs[i], mcands[i], timing=True)
-        kmix_s += t.kernel_s
-        kmix_f += 4.0 * args.users * t.n_candidates
+    n_mix = min(2 * args.roofline_queries, args.queries)
+    _, t = ds.count_all_many([(qs[i], mcands[i]) for i in range(n_mix)], timing=True)
+    kmix_s, kmix_f = t.kernel_s, 4.0 * args.users * t.n_candidates
     del os.environ["RK_COUNT_KERNEL"]
     roofline["kmix_tflops"] = round(kmix_f / kmix_s / 1e12, 3)
     roofline["kmix_frac"] = round(kmix_f / kmix_s / 1e12 / PEAK_FP32_TFLOPS, 4)

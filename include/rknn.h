@@ -176,6 +176,21 @@ typedef struct rk_query_desc {
     uint64_t nc;
 } rk_query_desc;
 
+/* ---- count-all over many queries in one call: the queries' count kernels
+ *      run back to back on the device (one launch each, no host round trip
+ *      between them; rounds of at most 256 MiB of device counts), then each
+ *      query's counts are copied out: counts + i * stride (stride >= nu)
+ *      receives query i's nu counts, in caller order, or in the dataset's
+ *      order (rk_dataset_order) when sorted_order != 0.  A query with cand
+ *      NULL counts over the host filter's list at its k; otherwise k is
+ *      not used.  t->kernel_s is the time of all the count launches (HIP
+ *      events around each round's launches), t->n_candidates the sum of
+ *      the lists.  Same counts as rk_count_all per query (the rknn-rt
+ *      counter rt.cu:95-111 with k = inf, looped over queries as
+ *      rt.cpp:1121-1139 launches one query at a time). ---- */
+rk_status rk_count_all_many(rk_dataset* ds, const rk_query_desc* qs, size_t nq, uint32_t* counts,
+                            size_t stride, int sorted_order, rk_timing* t);
+
 /* One call: fill + run + fetch.  offsets has nq+1 entries; the ids of
  * query i are out_ids[offsets[i] .. offsets[i+1]), sorted ascending.  The
  * kernels are launched directly on a batch object the dataset keeps per

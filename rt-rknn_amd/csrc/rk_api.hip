@@ -645,6 +645,8 @@ rk_status download(rk_ctx* c, void* dst, const void* src, size_t bytes)
 // (32-bit counts straight from the count kernel).
 rk_status count_all_impl(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc, uint32_t* counts,
                          rk_timing* t, bool sorted_out);
+rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, uint32_t* counts, size_t stride,
+                          bool sorted_out, rk_timing* t);
 
 } // namespace
 
@@ -659,6 +661,12 @@ rk_status rk_count_all_sorted(rk_dataset* ds, uint32_t q, const uint32_t* cand, 
                               rk_timing* t)
 {
     return count_all_impl(ds, q, cand, nc, counts, t, true);
+}
+
+rk_status rk_count_all_many(rk_dataset* ds, const rk_query_desc* qs, size_t nq, uint32_t* counts, size_t stride,
+                            int sorted_order, rk_timing* t)
+{
+    return count_many_impl(ds, qs, nq, counts, stride, sorted_order != 0, t);
 }
 
 rk_status rk_dataset_order(const rk_dataset* ds, uint32_t* order)
@@ -677,6 +685,175 @@ rk_status rk_dataset_order(const rk_dataset* ds, uint32_t* order)
 } // extern "C"
 
 namespace {
+
+// The count kernel for qd's class: RK_COUNT_KERNEL, the autotuner's stored
+// choice, or -- on the first query of a class not yet timed -- every variant
+// timed twice on this query (records at rec, counts into counts) and the
+// fastest kept.  The binary64 counters are cleared again after a timing.
+rk_status pick_count_variant(rk_dataset* ds, const QDev& qd, const CandRec* rec, const float2* orig, void* counts,
+                             bool wide, int* out)
+{
+    rk_ctx* c = ds->ctx;
+    int variant = count_variant(ds, qd.nc);
+    if (variant >= 0) {
+        *out = variant;
+        return RK_OK;
+    }
+    float best = 0.f;
+    for (int v = 0; v < kCountVariants; ++v) {
+        float ms = 0.f;
+        for (int r = 0; r < 2; ++r) {
+            RK_HIP(c, hipEventRecord(c->ev0, c->stream));
+            launch_count_all(ds->view, qd, rec, orig, counts, ds->ctr.p + 1, v, c->stream, wide);
+            RK_HIP(c, hipGetLastError());
+            RK_HIP(c, hipEventRecord(c->ev1, c->stream));
+            RK_HIP(c, hipEventSynchronize(c->ev1));
+            RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        }
+        if (v == 0 || ms < best) {
+            best = ms;
+            variant = v;
+        }
+    }
+    tuned_store(count_key(ds, qd.nc), variant, count_variant_name(variant));
+    RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, kPlainCounters * sizeof(uint32_t), c->stream));
+    *out = variant;
+    return RK_OK;
+}
+
+// Device bytes of one round of rk_count_all_many's count slices (a single
+// query's slice may exceed it: then the round holds that query alone).
+constexpr size_t kCountManyBytes = size_t(256) << 20;
+
+rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, uint32_t* counts, size_t stride,
+                          bool sorted_out, rk_timing* t)
+{
+    const double t0 = now_s();
+    if (!ds) return RK_ERR_INVALID_ARG;
+    if (nq && !qs) return fail(ds->ctx, RK_ERR_INVALID_ARG, "NULL query array");
+    if (nq && ds->nu && !counts) return fail(ds->ctx, RK_ERR_INVALID_ARG, "NULL output");
+    if (nq && stride < ds->nu) return fail(ds->ctx, RK_ERR_INVALID_ARG, "stride must be at least the number of users");
+    if (t) *t = rk_timing{};
+    // every query's cleaned list; the host filter (at the query's k) for a
+    // query without one
+    std::vector<std::vector<uint32_t>> cls(nq);
+    double filter_s = 0;
+    for (size_t i = 0; i < nq; ++i) {
+        const rk_query_desc& d = qs[i];
+        rk_status s = check_query(ds, d.q, d.cand ? 1u : d.k);
+        if (s != RK_OK) return s;
+        if (d.cand) {
+            s = clean_candidates(ds, d.q, d.cand, d.nc, cls[i]);
+        } else {
+            std::vector<uint32_t> v;
+            const double f0 = now_s();
+            ds->index->filter(d.q, d.k, v);
+            filter_s += now_s() - f0;
+            s = clean_candidates(ds, d.q, v.data(), v.size(), cls[i]);
+        }
+        if (s != RK_OK) return s;
+    }
+    if (ds->nu == 0 || nq == 0) return RK_OK;
+    rk_ctx* c = ds->ctx;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    RK_HIP(c, hipSetDevice(c->device));
+    // every query's records, one upload
+    std::vector<size_t> roff(nq);
+    size_t total = 0;
+    for (size_t i = 0; i < nq; ++i) {
+        roff[i] = total;
+        total += cls[i].size();
+    }
+    RK_HIP(c, ds->h_rec.reserve(total));
+    RK_HIP(c, ds->h_orig.reserve(total));
+    RK_HIP(c, ds->rec.reserve(total));
+    RK_HIP(c, ds->orig.reserve(total));
+    for (size_t i = 0; i < nq; ++i) {
+        const uint32_t q = qs[i].q;
+        const float qx = ds->hfx[q], qy = ds->hfy[q];
+        double wx, wy;
+        axis_bounds(ds->ubox[0], ds->ubox[1], ds->ubox[2], ds->ubox[3], qx, qy, &wx, &wy);
+        for (size_t j = 0; j < cls[i].size(); ++j) {
+            const uint32_t f = cls[i][j];
+            ds->h_rec.p[roff[i] + j] = make_record(qx, qy, ds->hfx[f], ds->hfy[f], wx, wy);
+            ds->h_orig.p[roff[i] + j] = make_float2(ds->hfx[f], ds->hfy[f]);
+        }
+    }
+    if (total) {
+        RK_HIP(c, hipMemcpyAsync(ds->rec.p, ds->h_rec.p, total * sizeof(CandRec), hipMemcpyHostToDevice, c->stream));
+        RK_HIP(c, hipMemcpyAsync(ds->orig.p, ds->h_orig.p, total * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+    }
+    RK_HIP(c, ds->ctr.reserve(kPlainCounters));
+    RK_HIP(c, ds->h_ctr.reserve(kPlainCounters));
+    RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, kPlainCounters * sizeof(uint32_t), c->stream));
+    RK_HIP(c, ds->counts.reserve(ds->n_pad));
+    if (!sorted_out) RK_HIP(c, ds->counts_orig.reserve(ds->nu));
+    std::vector<QDev> qd(nq);
+    std::vector<int> var(nq), width(nq);
+    for (size_t i = 0; i < nq; ++i) {
+        qd[i] = make_qdev(ds, qs[i].q, 1, cls[i].size());
+        width[i] = sorted_out ? 4 : count_bytes(qd[i].nc);
+        rk_status s = pick_count_variant(ds, qd[i], ds->rec.p + roff[i], ds->orig.p + roff[i], ds->counts.p,
+                                         sorted_out, &var[i]);
+        if (s != RK_OK) return s;
+    }
+    const double t1 = now_s();
+    double kernel_s = 0, out_s = 0;
+    for (size_t i0 = 0; i0 < nq;) {
+        // this round: the queries whose count slices fit kCountManyBytes
+        std::vector<size_t> soff;
+        size_t used = 0, i1 = i0;
+        while (i1 < nq) {
+            const size_t b = ((size_t)ds->n_pad * width[i1] + 255) & ~size_t(255);
+            if (i1 > i0 && used + b > kCountManyBytes) break;
+            soff.push_back(used);
+            used += b;
+            ++i1;
+        }
+        RK_HIP(c, ds->counts_many.reserve(used));
+        // the count launches back to back, one event pair around them
+        RK_HIP(c, hipEventRecord(c->ev0, c->stream));
+        for (size_t i = i0; i < i1; ++i)
+            launch_count_all(ds->view, qd[i], ds->rec.p + roff[i], ds->orig.p + roff[i],
+                             ds->counts_many.p + soff[i - i0], ds->ctr.p + 1, var[i], c->stream, sorted_out);
+        RK_HIP(c, hipGetLastError());
+        RK_HIP(c, hipEventRecord(c->ev1, c->stream));
+        RK_HIP(c, hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        kernel_s += ms * 1e-3;
+        const double d0 = now_s();
+        for (size_t i = i0; i < i1; ++i) {
+            const void* slice = ds->counts_many.p + soff[i - i0];
+            rk_status s;
+            if (sorted_out) {
+                s = download(c, counts + i * stride, slice, ds->nu * sizeof(uint32_t));
+            } else {
+                launch_scatter_counts(slice, width[i], ds->perm.p, (uint32_t)ds->nu, ds->counts_orig.p, c->stream);
+                RK_HIP(c, hipGetLastError());
+                s = download(c, counts + i * stride, ds->counts_orig.p, ds->nu * sizeof(uint32_t));
+            }
+            if (s != RK_OK) return s;
+        }
+        out_s += now_s() - d0;
+        i0 = i1;
+    }
+    if (t) {
+        RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, kPlainCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                 c->stream));
+        RK_HIP(c, hipStreamSynchronize(c->stream));
+        uint64_t fp64 = 0;
+        for (int j = 1; j < kPlainCounters; ++j) fp64 += ds->h_ctr.p[j];
+        t->filter_s = filter_s;
+        t->upload_s = t1 - t0 - filter_s;
+        t->kernel_s = kernel_s;
+        t->download_s = out_s;
+        t->n_candidates = total;
+        t->n_fp64_rechecks = fp64;
+        t->total_s = now_s() - t0;
+    }
+    return RK_OK;
+}
 
 rk_status count_all_impl(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc, uint32_t* counts,
                          rk_timing* t, bool sorted_out)
@@ -703,41 +880,17 @@ rk_status count_all_impl(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_
     RK_HIP(c, ds->h_ctr.reserve(kPlainCounters));
     RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, kPlainCounters * sizeof(uint32_t), c->stream));
     QDev qd = make_qdev(ds, q, 1, cl.size());
-    // one timed launch of variant v
-    auto run = [&](int v, float* ms) -> rk_status {
-        RK_HIP(c, hipEventRecord(c->ev0, c->stream));
-        launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, v, c->stream, sorted_out);
-        RK_HIP(c, hipGetLastError());
-        RK_HIP(c, hipEventRecord(c->ev1, c->stream));
-        if (ms) {
-            RK_HIP(c, hipEventSynchronize(c->ev1));
-            RK_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
-        }
-        return RK_OK;
-    };
-    int variant = count_variant(ds, cl.size());
-    if (variant < 0) { // first call of this class: time every kernel on this query, keep the fastest
-        float best = 0.f;
-        for (int v = 0; v < kCountVariants; ++v) {
-            float ms = 0.f;
-            for (int r = 0; r < 2; ++r) {
-                s = run(v, &ms);
-                if (s != RK_OK) return s;
-            }
-            if (v == 0 || ms < best) {
-                best = ms;
-                variant = v;
-            }
-        }
-        tuned_store(count_key(ds, cl.size()), variant, count_variant_name(variant));
-        RK_HIP(c, hipMemsetAsync(ds->ctr.p, 0, kPlainCounters * sizeof(uint32_t), c->stream));
-    }
-    s = run(variant, nullptr);
+    int variant = 0;
+    s = pick_count_variant(ds, qd, ds->rec.p, ds->orig.p, ds->counts.p, sorted_out, &variant);
     if (s != RK_OK) return s;
+    RK_HIP(c, hipEventRecord(c->ev0, c->stream));
+    launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, variant, c->stream, sorted_out);
+    RK_HIP(c, hipGetLastError());
+    RK_HIP(c, hipEventRecord(c->ev1, c->stream));
     if (t) RK_HIP(c, hipEventSynchronize(c->ev1)); // download_s starts when the counts are ready
     const double t2 = now_s();
     if (!sorted_out) {
-        launch_scatter_counts(ds->counts.p, count_narrow(qd.nc), ds->perm.p, (uint32_t)ds->nu, ds->counts_orig.p,
+        launch_scatter_counts(ds->counts.p, count_bytes(qd.nc), ds->perm.p, (uint32_t)ds->nu, ds->counts_orig.p,
                               c->stream);
         RK_HIP(c, hipGetLastError());
     }

@@ -188,6 +188,7 @@ struct rk_dataset {
     rk::DevBuf<int32_t> out;
     rk::DevBuf<uint32_t> ctr;
     rk::DevBuf<uint32_t> counts, counts_orig;
+    rk::DevBuf<unsigned char> counts_many; // rk_count_all_many: one round's count slices
     rk::HostBuf<rk::CandRec> h_rec;
     rk::HostBuf<float2> h_orig;
     rk::HostBuf<int32_t> h_out;

@@ -318,9 +318,9 @@ __global__ void k_build_records(const uint32_t* __restrict__ cand, uint32_t nc, 
 // <= 0), because the fp32 error is < 1/16 per evaluation; the others take
 // the banded path, users on lanes, with the binary64 re-check.
 //
-// Counts are written in sorted user order, as 16-bit values when the query
-// has fewer than 2^16 candidates (10 instead of 12 bytes of HBM traffic per
-// user), else 32-bit.
+// Counts are written in sorted user order, as 8-bit values when the query
+// has fewer than 2^8 candidates and 16-bit ones below 2^16 (9 or 10 instead
+// of 12 bytes of HBM traffic per user), else 32-bit.
 
 constexpr uint32_t kCountLdsRecs = 1024;
 
@@ -1431,24 +1431,27 @@ const char* count_variant_name(int variant)
     return variant >= 0 && variant < kCountVariants ? names[variant] : "valu4";
 }
 
-bool count_narrow(uint32_t nc) { return nc < 65536u; }
+int count_bytes(uint32_t nc) { return nc < 256u ? 1 : nc < 65536u ? 2 : 4; }
 
 void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, void* counts,
                       uint32_t* rechecks, int variant, hipStream_t st, bool wide)
 {
-    if (!wide && count_narrow(q.nc))
-        launch_count_typed(ds, q, rec, orig, (uint16_t*)counts, rechecks, variant, st);
-    else
-        launch_count_typed(ds, q, rec, orig, (uint32_t*)counts, rechecks, variant, st);
+    switch (wide ? 4 : count_bytes(q.nc)) {
+    case 1: launch_count_typed(ds, q, rec, orig, (uint8_t*)counts, rechecks, variant, st); break;
+    case 2: launch_count_typed(ds, q, rec, orig, (uint16_t*)counts, rechecks, variant, st); break;
+    default: launch_count_typed(ds, q, rec, orig, (uint32_t*)counts, rechecks, variant, st); break;
+    }
 }
 
-void launch_scatter_counts(const void* src_sorted, bool narrow, const uint32_t* perm, uint32_t n,
+void launch_scatter_counts(const void* src_sorted, int bytes, const uint32_t* perm, uint32_t n,
                            uint32_t* dst_orig, hipStream_t st)
 {
-    if (narrow)
-        k_scatter_u32<<<grid_for(n, 256, 4096), 256, 0, st>>>((const uint16_t*)src_sorted, perm, n, dst_orig);
-    else
-        k_scatter_u32<<<grid_for(n, 256, 4096), 256, 0, st>>>((const uint32_t*)src_sorted, perm, n, dst_orig);
+    const int g = grid_for(n, 256, 4096);
+    switch (bytes) {
+    case 1: k_scatter_u32<<<g, 256, 0, st>>>((const uint8_t*)src_sorted, perm, n, dst_orig); break;
+    case 2: k_scatter_u32<<<g, 256, 0, st>>>((const uint16_t*)src_sorted, perm, n, dst_orig); break;
+    default: k_scatter_u32<<<g, 256, 0, st>>>((const uint32_t*)src_sorted, perm, n, dst_orig); break;
+    }
 }
 
 size_t batch_query_lds_bytes(uint32_t n_levels)

@@ -83,8 +83,9 @@ void launch_build_records(const uint32_t* cand, uint32_t nc, uint32_t q, float q
                           uint32_t n_ctr, hipStream_t st);
 
 // count-all form: exact per-user count (sorted order), no early exit.
-// rechecks has kBuckets slots.  Counts are 16-bit when count_narrow(nc)
-// (fewer than 2^16 candidates), else 32-bit; the buffer holds n_pad of them.
+// rechecks has kBuckets slots.  Counts take count_bytes(nc) bytes each: 1
+// below 2^8 candidates, 2 below 2^16, else 4 (always 4 when wide); the
+// buffer holds n_pad of them.
 // variants: 4, 6 or 8 users per lane (3 instructions per clean test in
 // each), one group of tiles per wave
 constexpr int kCountValu4 = 0;
@@ -92,11 +93,11 @@ constexpr int kCountValu8 = 1;
 constexpr int kCountValu6 = 2;
 constexpr int kCountVariants = 3;
 const char* count_variant_name(int variant);
-bool count_narrow(uint32_t nc);
+int count_bytes(uint32_t nc);
 // wide: 32-bit counts whatever the candidate count
 void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2* orig, void* counts,
                       uint32_t* rechecks, int variant, hipStream_t st, bool wide = false);
-void launch_scatter_counts(const void* src_sorted, bool narrow, const uint32_t* perm, uint32_t n,
+void launch_scatter_counts(const void* src_sorted, int bytes, const uint32_t* perm, uint32_t n,
                            uint32_t* dst_orig, hipStream_t st);
 
 // batched form: the whole step is one launch of the traversal kernel, which

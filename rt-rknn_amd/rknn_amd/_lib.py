@@ -85,6 +85,7 @@ SIGNATURES = {
     "rk_query": (ST, [P, C.c_uint32, C.c_uint32, I32P, C.c_size_t, SZP, TP]),
     "rk_count_all": (ST, [P, C.c_uint32, U32P, C.c_size_t, U32P, TP]),
     "rk_count_all_sorted": (ST, [P, C.c_uint32, U32P, C.c_size_t, U32P, TP]),
+    "rk_count_all_many": (ST, [P, C.POINTER(rk_query_desc), C.c_size_t, U32P, C.c_size_t, C.c_int, TP]),
     "rk_dataset_order": (ST, [P, U32P]),
     "rk_query_batch": (ST, [P, C.POINTER(rk_query_desc), C.c_size_t, I32P, C.c_size_t, U64P, TP]),
     "rk_batch_prepare": (ST, [P, C.POINTER(rk_query_desc), C.c_size_t, C.POINTER(P)]),

@@ -173,6 +173,28 @@ class Dataset:
         counts = counts[: self.nu]
         return (counts, t) if timing else counts
 
+    def count_all_many(self, queries, timing=False, sorted_order=False, out=None):
+        """Count-all for many queries in one call (their count kernels back to
+        back on the device).  queries: (q, cand) pairs, or (q, k, None) to
+        count over the host filter's list at k.  Returns an (nq, nu) uint32
+        array (rows in caller order, or the dataset's with sorted_order);
+        out: a C-contiguous (nq, >= nu) uint32 array to write into."""
+        qk = [(x[0], 1, x[1]) if len(x) == 2 else tuple(x) for x in queries]
+        descs, keep = self._descs(qk)
+        nq = len(qk)
+        if out is not None:
+            if (out.dtype != np.uint32 or out.ndim != 2 or not out.flags.c_contiguous or out.shape[0] < nq
+                    or out.shape[1] < max(1, self.nu)):
+                raise ValueError("out must be a C-contiguous (nq, >= nu) uint32 array")
+            counts = out
+        else:
+            counts = np.zeros((max(1, nq), max(1, self.nu)), dtype=np.uint32)
+        t = L.rk_timing()
+        self.ctx.check(self._lib.rk_count_all_many(self.h, descs, nq, _ptr(counts, C.c_uint32), counts.shape[1],
+                                                   1 if sorted_order else 0, C.byref(t)))
+        counts = counts[:nq, : self.nu]
+        return (counts, t) if timing else counts
+
     def order(self):
         """order[j]: index in the constructor's user arrays of the dataset's
         j-th user."""

@@ -345,6 +345,82 @@ def test_autotuned_shapes_agree(rk, ctx, tmp_path, monkeypatch):
         ds.close()
 
 
+@pytest.mark.parametrize("kernel", ["valu4", "valu6", "valu8", "tuned"])
+def test_count_all_many_matches_oracle(g1, gties, kernel, monkeypatch):
+    """Many queries' counts in one call (count kernels back to back): lists
+    from the filter (8-bit counts), every facility (16-bit), a query left to
+    the host filter (cand None) and an empty list, in both orders, each row
+    against the oracle's counts; and the call's argument checks."""
+    if kernel != "tuned":
+        monkeypatch.setenv("RK_COUNT_KERNEL", kernel)
+    for d, ds, qk, _ in (g1, gties):
+        allf = np.arange(ds.nf, dtype=np.uint32)
+        queries, want = [], []
+        for j, (q, k) in enumerate(qk[:5]):
+            if j == 2:
+                queries.append((q, k, None))
+                cand = ds.filter(q, k)
+            else:
+                cand = ds.filter(q, k) if j % 2 == 0 else allf
+                queries.append((q, cand))
+            want.append(O.counts(d["ux"], d["uy"], d["fx"], d["fy"], q, cand=cand))
+        q0 = qk[0][0]
+        queries.append((q0, np.zeros(0, dtype=np.uint32)))
+        want.append(np.zeros(ds.nu, dtype=np.uint32))
+        got, t = ds.count_all_many(queries, timing=True)
+        assert got.shape == (len(queries), ds.nu)
+        for r in range(len(queries)):
+            assert np.array_equal(got[r], want[r]), (kernel, r)
+        order = ds.order()
+        out = np.full((len(queries), ds.nu + 3), 0xFFFFFFFF, dtype=np.uint32)  # stride beyond nu
+        ds.count_all_many(queries, sorted_order=True, out=out)
+        for r in range(len(queries)):
+            assert np.array_equal(out[r, : ds.nu], want[r][order]), (kernel, r)
+            assert (out[r, ds.nu:] == 0xFFFFFFFF).all()
+    d, ds, qk, _ = g1
+    lib, h = ds._lib, ds.h
+    descs, keep = ds._descs([(qk[0][0], 5)])
+    buf = np.zeros(ds.nu, dtype=np.uint32)
+    import ctypes as C
+    p = buf.ctypes.data_as(C.POINTER(C.c_uint32))
+    assert lib.rk_count_all_many(h, descs, 1, p, ds.nu - 1, 0, None) != 0  # stride below nu
+    assert lib.rk_count_all_many(h, descs, 1, None, ds.nu, 0, None) != 0  # NULL output
+    assert lib.rk_count_all_many(h, None, 1, p, ds.nu, 0, None) != 0  # NULL queries
+    assert lib.rk_count_all_many(h, descs, 0, None, 0, 0, None) == 0  # nothing to do
+    bad, keep2 = ds._descs([(ds.nf, 5)])
+    assert lib.rk_count_all_many(h, bad, 1, p, ds.nu, 0, None) != 0  # q out of range
+
+
+@pytest.mark.parametrize("kernel", ["valu4", "valu6", "valu8"])
+def test_count_all_count_widths(rk, ctx, kernel, monkeypatch):
+    """The count kernel stores 8-bit counts below 256 candidates and 16-bit
+    ones from 256: lists of 255 and 256 candidates whose every count is the
+    list's length (q far outside the users' box, so every candidate is
+    closer), and a 255-candidate list over uniform data, in both orders."""
+    monkeypatch.setenv("RK_COUNT_KERNEL", kernel)
+    r = np.random.default_rng(23)
+    nu, nf = 5000, 600
+    ux, uy = (r.random(nu) * 1000).astype(np.float32), (r.random(nu) * 1000).astype(np.float32)
+    fx, fy = (r.random(nf) * 1000).astype(np.float32), (r.random(nf) * 1000).astype(np.float32)
+    fx[0], fy[0] = 9000.0, 9000.0
+    uid = np.arange(nf, nf + nu, dtype=np.int32)
+    fid = np.arange(nf, dtype=np.int32)
+    ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    try:
+        order = ds.order()
+        for q, cand in ((0, np.arange(1, 256, dtype=np.uint32)), (0, np.arange(1, 257, dtype=np.uint32)),
+                        (7, np.arange(300, 555, dtype=np.uint32))):
+            exp = O.counts(ux, uy, fx, fy, q, cand=cand)
+            if q == 0:
+                assert exp.min() == len(cand)  # the widest count the width holds
+            got, t = ds.count_all(q, cand, timing=True)
+            assert t.n_candidates == len(cand)
+            assert np.array_equal(got, exp), (kernel, q, len(cand))
+            assert np.array_equal(ds.count_all(q, cand, sorted_order=True), exp[order]), (kernel, q, len(cand))
+    finally:
+        ds.close()
+
+
 @pytest.mark.parametrize("kernel", ["valu4", "valu6", "valu8"])
 def test_count_all_large_vs_oracle(rk, ctx, kernel, monkeypatch):
     """Count-all on clustered data with exact bisector ties (config 3) and
