@@ -7,7 +7,7 @@ candidate lists (from the host influence-zone filter) are resident in HBM;
 the users are Hilbert-sorted and resident.  The step ends with every query's
 sorted result ids in pinned host memory: its last kernel writes them there.
 `value` counts queries/s over exactly --steps timed steps that rotate over
---batches distinct batches (8 by default: 8,192 distinct queries), two in
+--batches distinct batches (8 by default: 8,192 distinct queries), three in
 flight, each step's results taken in place (rk_batch_results) after its wait;
 `value_fetch` is the same with every step's ids copied into a caller buffer
 (rk_batch_fetch).  The JSON line also carries:
@@ -56,8 +56,9 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per count-all launch on the 10M-user, 100k-facility k=10 workload,
 # from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known
 # size (profiles/k_count_all_10M_100k.md); the kernel reads every user once and
-# writes one 16-bit count per user, so the figure does not depend on the query
-MEASURED_TRAFFIC = {"valu4": 101.4e6, "valu6": 101.1e6, "valu8": 100.8e6}
+# writes one 8-bit count per user (the k=10 lists hold fewer than 256
+# candidates), so the figure does not depend on the query
+MEASURED_TRAFFIC = {"valu4": 91.4e6, "valu6": 91.0e6, "valu8": 90.8e6}
 
 
 def log(*a):
@@ -106,7 +107,7 @@ def main():
     ap.add_argument("--plain", type=int, default=1024, help="queries timed in the plain form")
     ap.add_argument("--roofline-queries", type=int, default=8)
     ap.add_argument("--cpu-queries", type=int, default=2, help="queries of the brute-force CPU context run")
-    ap.add_argument("--inflight", type=int, default=2,
+    ap.add_argument("--inflight", type=int, default=3,
                     help="batches in flight in the timed steps, each on its own stream")
     ap.add_argument("--batches", type=int, default=8,
                     help="distinct batches of --queries queries rotated through the timed steps")
@@ -162,7 +163,7 @@ def main():
 
     # ---- batched form: nb batches of distinct queries, each prepared once
     # (candidate lists uploaded, graph captured) and rotated step after step,
-    # two in flight (rk_batch_enqueue / rk_batch_wait, each batch on its own
+    # --inflight of them in flight (rk_batch_enqueue / rk_batch_wait, each batch on its own
     # stream); a step is done when its results are in host memory.  Their
     # first run sizes each batch's landing area (and captures its graph again).
     batches = [ds.prepare_batch([(q, args.k, c) for q, c in zip(qs_all[j * nq:(j + 1) * nq],
@@ -179,11 +180,13 @@ def main():
     n_inflight = max(1, args.inflight)
 
     def rotate(steps, take):
-        """Seconds per step over the rotation; take(b) after each wait."""
+        """Seconds per step over the rotation; take(b) after each wait.  A
+        batch holds one run at a time, so at most nb are in flight."""
+        depth = min(n_inflight, nb)
         pending = []
         for s in range(max(steps, 0)):
             b = batches[s % nb]
-            if len(pending) == n_inflight:
+            if len(pending) == depth:
                 c = pending.pop(0)
                 c.wait()
                 take(c)
@@ -504,7 +507,7 @@ def main():
                                f"{args.queries} queries per step, k={args.k}, results in host memory",
                    "users": args.users, "facilities": args.facilities, "queries_per_step": args.queries,
                    "k": args.k, "candidates_mean": round(float(ncs.mean()), 2), "generator_config": args.config,
-                   "distinct_batches": nb, "batches_in_flight": n_inflight},
+                   "distinct_batches": nb, "batches_in_flight": min(n_inflight, nb)},
         "value_fetch": round(value_fetch, 1),
         "plain_qps": round(plain_qps, 1),
         "batched_vs_plain": round(value / plain_qps, 2),
