@@ -12,10 +12,12 @@ flight, each step's results taken in place (rk_batch_results) after its wait;
 `value_fetch` is the same with every step's ids copied into a caller buffer
 (rk_batch_fetch).  The JSON line also carries:
   roofline      dense counting kernel (count-all: every user x every candidate,
-                4 fp32 flops per test): the first queries' count launches back
-                to back in one rk_count_all_many call, HIP events on the
-                kernel's stream around them, divided by the launch count;
-                kmix_tflops: the same kernel over the k ~ U{1..50} lists
+                4 fp32 flops per test): the first 16 queries in one
+                rk_count_all_many call, which counts up to 16 queries per
+                launch (k_count_set), HIP events on the kernel's stream around
+                the launches, divided by the launch count; beside it the same
+                queries at one launch each (k_count_all) and, as kmix_tflops,
+                the k ~ U{1..50} lists
   count_all_call_ms  the whole count-all call, counts in caller order and in
                 the dataset's order, with the copy of 10M counts to the host
   cpu_baseline  the reference's tree-culled verification (rknn-inf) restated
@@ -53,12 +55,14 @@ for _p in (ROOT, os.path.join(ROOT, "rt-rknn_amd")):
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 
-# HBM bytes per count-all launch on the 10M-user, 100k-facility k=10 workload,
-# from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a copy kernel of known
-# size (profiles/k_count_all_10M_100k.md); the kernel reads every user once and
-# writes one 8-bit count per user (the k=10 lists hold fewer than 256
-# candidates), so the figure does not depend on the query
-MEASURED_TRAFFIC = {"valu4": 91.4e6, "valu6": 91.0e6, "valu8": 90.8e6}
+# HBM bytes per count-all launch of 16 queries on the 10M-user, 100k-facility
+# k=10 workload, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by a copy
+# kernel of known size (profiles/k_count_all_10M_100k.md); the kernel reads
+# every user once and writes one 8-bit count per user and query (the k=10
+# lists hold fewer than 256 candidates), so the figure depends on the number
+# of queries in the launch, not on which they are
+MEASURED_TRAFFIC = {"valu4": 244.9e6, "valu6": 246.1e6, "valu8": 245.9e6}
+MEASURED_TRAFFIC_QUERIES = 16
 
 
 def log(*a):
@@ -105,7 +109,7 @@ def main():
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--config", type=int, default=4)
     ap.add_argument("--plain", type=int, default=1024, help="queries timed in the plain form")
-    ap.add_argument("--roofline-queries", type=int, default=8)
+    ap.add_argument("--roofline-queries", type=int, default=16)
     ap.add_argument("--cpu-queries", type=int, default=2, help="queries of the brute-force CPU context run")
     ap.add_argument("--inflight", type=int, default=3,
                     help="batches in flight in the timed steps, each on its own stream")
@@ -338,30 +342,47 @@ def main():
         sys.exit(1)
 
     # ---- roofline: dense counting kernel (count-all), HIP events on its stream;
-    # every variant (4, 6 or 8 users per lane; 3 VALU
-    # instructions per clean test)
+    # every variant (4, 6 or 8 users per lane; 3 VALU instructions per clean
+    # test).  rk_count_all_many counts consecutive queries whose lists fit the
+    # kernel's LDS staging together in one launch (k_count_set: up to 16
+    # queries, 1,024 records); a launch's algorithmic flops are its queries'
+    # 4 * nu * |C| summed
     variants = {}
     # (8-bit counts: the k=10 lists hold fewer than 256 candidates)
-    kernel_names = {"valu4": "k_count_all<4,1,uint8_t>", "valu6": "k_count_all<6,1,uint8_t>",
-                    "valu8": "k_count_all<8,1,uint8_t>"}
+    kernel_names = {"valu4": "k_count_set<4,uint8_t>", "valu6": "k_count_set<6,uint8_t>",
+                    "valu8": "k_count_set<8,uint8_t>"}
     kts = {kern: [] for kern in kernel_names}
     flops = {kern: 0.0 for kern in kernel_names}
     n_roof = min(args.roofline_queries, args.queries)
     roof_queries = [(qs[i], cands[i]) for i in range(n_roof)]
     roof_out = np.zeros((n_roof, args.users), dtype=np.uint32)
+    launches = 0
     for _ in range(3):  # the variants interleaved, three rounds each
         for kern in kernel_names:
             os.environ["RK_COUNT_KERNEL"] = kern
             # the queries' count launches back to back on the kernel's stream,
-            # HIP events around them: kernel_s / n_roof is the average launch
+            # HIP events around them: kernel_s / n_launches is the average launch
             _, t = ds.count_all_many(roof_queries, timing=True, out=roof_out)
-            kts[kern].append(t.kernel_s / n_roof)
-            flops[kern] += 4.0 * args.users * t.n_candidates / n_roof
-    del os.environ["RK_COUNT_KERNEL"]
+            launches = t.n_launches
+            kts[kern].append(t.kernel_s / t.n_launches)
+            flops[kern] += 4.0 * args.users * t.n_candidates / t.n_launches
     for i in range(n_roof):  # the call's counts are the single-query call's
         if not np.array_equal(roof_out[i], ds.count_all(qs[i], cands[i])):
             log("ERROR: count-all over many queries differs from the one-query call")
             sys.exit(1)
+    # the same queries with one launch each (k_count_all), for comparison
+    per_query = {}
+    os.environ["RK_COUNT_SET"] = "1"
+    for kern in kernel_names:
+        os.environ["RK_COUNT_KERNEL"] = kern
+        best_q = None
+        for _ in range(3):
+            _, t = ds.count_all_many(roof_queries, timing=True, out=roof_out)
+            tf = 4.0 * args.users * t.n_candidates / t.kernel_s / 1e12
+            best_q = tf if best_q is None else max(best_q, tf)
+        per_query[kern] = round(best_q, 3)
+    del os.environ["RK_COUNT_SET"]
+    del os.environ["RK_COUNT_KERNEL"]
     # the whole count-all call, counts in caller order (a scatter pass after
     # the kernel) and in the dataset's order (the kernel's output as it is),
     # each including its device-to-host copy of nu counts
@@ -382,18 +403,21 @@ def main():
             f"order + copy of {args.users} counts {call_ms[name]['order_and_copy']} ms)")
     for kern, kt in kts.items():
         variants[kern] = (flops[kern] / sum(kt) / 1e12, float(np.mean(kt)), flops[kern] / len(kt))
-        log(f"count-all kernel ({kern}): {variants[kern][0]:.2f} TFLOP/s over {len(kt) * n_roof} launches, "
-            f"avg {variants[kern][1] * 1e3:.4f} ms")
+        log(f"count-all kernel ({kern}): {variants[kern][0]:.2f} TFLOP/s, {n_roof} queries in {launches} launches "
+            f"(avg {variants[kern][1] * 1e3:.4f} ms); one launch per query {per_query[kern]:.2f} TFLOP/s")
     best = max(variants, key=lambda v: variants[v][0])
     achieved, avg_s, flops_launch = variants[best]
     roofline = {"bound": "valu", "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": (MEASURED_TRAFFIC.get(best) if (args.users, args.facilities, args.k, args.config)
-                            == (10_000_000, 100_000, 10, 4) else None),
+                "traffic": (MEASURED_TRAFFIC.get(best) if (args.users, args.facilities, args.k, args.config,
+                                                               n_roof, launches)
+                            == (10_000_000, 100_000, 10, 4, MEASURED_TRAFFIC_QUERIES, 1) else None),
                 "traffic_source": "profiles/k_count_all_10M_100k.md (calibrated PMC, bytes per launch)",
                 "kernel": kernel_names[best],
                 "avg_launch_ms": round(avg_s * 1e3, 4), "flops_per_launch": flops_launch,
-                "variants_tflops": {k: round(v[0], 3) for k, v in variants.items()}}
+                "queries": n_roof, "launches": launches,
+                "variants_tflops": {k: round(v[0], 3) for k, v in variants.items()},
+                "one_launch_per_query_tflops": per_query}
     # the same kernel over the batched workload's own k mix (k ~ U{1..50}: the
     # mixed-k step's candidate lists), flop-weighted over its queries
     os.environ["RK_COUNT_KERNEL"] = best

@@ -59,6 +59,8 @@ typedef struct rk_timing {
     uint64_t n_candidates;
     uint64_t n_fp64_rechecks; /* (user, candidate) tests whose FP32 value fell
                                  in the uncertainty band, settled in binary64 */
+    uint64_t n_launches;      /* counting kernels launched (count-all calls; a
+                                 launch may count several queries)          */
 } rk_timing;
 
 /* ---- lifecycle: replaces cuda_init / cuda_cleanup / is_cuda_available
@@ -176,18 +178,21 @@ typedef struct rk_query_desc {
     uint64_t nc;
 } rk_query_desc;
 
-/* ---- count-all over many queries in one call: the queries' count kernels
- *      run back to back on the device (one launch each, no host round trip
- *      between them; rounds of at most 256 MiB of device counts), then each
+/* ---- count-all over many queries in one call: consecutive queries whose
+ *      lists fit 1,024 records together (at most 16 of them) are counted in
+ *      one launch, each wave loading its users once for the whole set; a
+ *      longer list gets a launch of its own (RK_COUNT_SET=1: every query
+ *      does).  The launches run back to back, no host round trip between
+ *      them, in rounds of at most 256 MiB of device counts; then each
  *      query's counts are copied out: counts + i * stride (stride >= nu)
  *      receives query i's nu counts, in caller order, or in the dataset's
  *      order (rk_dataset_order) when sorted_order != 0.  A query with cand
  *      NULL counts over the host filter's list at its k; otherwise k is
  *      not used.  t->kernel_s is the time of all the count launches (HIP
- *      events around each round's launches), t->n_candidates the sum of
- *      the lists.  Same counts as rk_count_all per query (the rknn-rt
- *      counter rt.cu:95-111 with k = inf, looped over queries as
- *      rt.cpp:1121-1139 launches one query at a time). ---- */
+ *      events around each round's launches), t->n_launches their number,
+ *      t->n_candidates the sum of the lists.  Same counts as rk_count_all
+ *      per query (the rknn-rt counter rt.cu:95-111 with k = inf, looped
+ *      over queries as rt.cpp:1121-1139 launches one query at a time). ---- */
 rk_status rk_count_all_many(rk_dataset* ds, const rk_query_desc* qs, size_t nq, uint32_t* counts,
                             size_t stride, int sorted_order, rk_timing* t);
 

@@ -797,25 +797,89 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
                                          sorted_out, &var[i]);
         if (s != RK_OK) return s;
     }
+    // the launches: consecutive queries of one count variant whose records
+    // fit the LDS staging together are counted in one launch (k_count_set:
+    // each wave's users loaded once for the whole set); a longer list, or
+    // every query when RK_COUNT_SET=1, gets a launch of its own.  The
+    // queries of a set share its widest count type.
+    size_t set_max = kCountSetMax;
+    if (const char* e = std::getenv("RK_COUNT_SET")) {
+        const long v = std::strtol(e, nullptr, 10);
+        set_max = v < 1 ? 1 : v > (long)kCountSetMax ? kCountSetMax : (size_t)v;
+    }
+    struct Launch {
+        size_t i0, i1;
+        uint32_t n_recs;
+    };
+    std::vector<Launch> launches;
+    for (size_t i = 0; i < nq;) {
+        Launch L{i, i + 1, qd[i].nc};
+        if (qd[i].nc <= kCountSetRecs) {
+            while (L.i1 < nq && L.i1 - L.i0 < set_max && var[L.i1] == var[L.i0] &&
+                   L.n_recs + qd[L.i1].nc <= kCountSetRecs) {
+                L.n_recs += qd[L.i1].nc;
+                ++L.i1;
+            }
+            int w = 0;
+            for (size_t j = L.i0; j < L.i1; ++j) w = std::max(w, width[j]);
+            for (size_t j = L.i0; j < L.i1; ++j) width[j] = w;
+        }
+        launches.push_back(L);
+        i = L.i1;
+    }
+    RK_HIP(c, ds->setq.reserve(nq));
+    RK_HIP(c, ds->h_setq.reserve(nq));
     const double t1 = now_s();
     double kernel_s = 0, out_s = 0;
-    for (size_t i0 = 0; i0 < nq;) {
-        // this round: the queries whose count slices fit kCountManyBytes
+    uint64_t n_launches = 0;
+    for (size_t l0 = 0; l0 < launches.size();) {
+        // this round: the launches whose count slices fit kCountManyBytes
+        const size_t i0 = launches[l0].i0;
         std::vector<size_t> soff;
-        size_t used = 0, i1 = i0;
-        while (i1 < nq) {
-            const size_t b = ((size_t)ds->n_pad * width[i1] + 255) & ~size_t(255);
-            if (i1 > i0 && used + b > kCountManyBytes) break;
-            soff.push_back(used);
-            used += b;
-            ++i1;
+        size_t used = 0, l1 = l0;
+        while (l1 < launches.size()) {
+            size_t b = 0;
+            for (size_t i = launches[l1].i0; i < launches[l1].i1; ++i)
+                b += ((size_t)ds->n_pad * width[i] + 255) & ~size_t(255);
+            if (l1 > l0 && used + b > kCountManyBytes) break;
+            for (size_t i = launches[l1].i0; i < launches[l1].i1; ++i) {
+                soff.push_back(used);
+                used += ((size_t)ds->n_pad * width[i] + 255) & ~size_t(255);
+            }
+            ++l1;
         }
+        const size_t i1 = launches[l1 - 1].i1;
         RK_HIP(c, ds->counts_many.reserve(used));
+        // the sets' descriptors, one upload for the round
+        for (size_t i = i0; i < i1; ++i) {
+            CountSetQ& d = ds->h_setq.p[i - i0];
+            d = CountSetQ{};
+            d.qx = qd[i].qx;
+            d.qy = qd[i].qy;
+            d.nc = qd[i].nc;
+            d.coff = soff[i - i0] / width[i];
+        }
+        for (size_t l = l0; l < l1; ++l)
+            for (size_t i = launches[l].i0; i < launches[l].i1; ++i)
+                ds->h_setq.p[i - i0].roff = (uint32_t)(roff[i] - roff[launches[l].i0]);
+        RK_HIP(c, hipMemcpyAsync(ds->setq.p, ds->h_setq.p, (i1 - i0) * sizeof(CountSetQ), hipMemcpyHostToDevice,
+                                 c->stream));
         // the count launches back to back, one event pair around them
         RK_HIP(c, hipEventRecord(c->ev0, c->stream));
-        for (size_t i = i0; i < i1; ++i)
-            launch_count_all(ds->view, qd[i], ds->rec.p + roff[i], ds->orig.p + roff[i],
-                             ds->counts_many.p + soff[i - i0], ds->ctr.p + 1, var[i], c->stream, sorted_out);
+        for (size_t l = l0; l < l1; ++l) {
+            const Launch& L = launches[l];
+            if (L.i1 - L.i0 == 1) {
+                launch_count_all(ds->view, qd[L.i0], ds->rec.p + roff[L.i0], ds->orig.p + roff[L.i0],
+                                 ds->counts_many.p + soff[L.i0 - i0], ds->ctr.p + 1, var[L.i0], c->stream,
+                                 sorted_out);
+            } else {
+                // every slice of the round is addressed from the round's base
+                launch_count_set(ds->view, ds->setq.p + (L.i0 - i0), (uint32_t)(L.i1 - L.i0), L.n_recs,
+                                 ds->rec.p + roff[L.i0], ds->orig.p + roff[L.i0], ds->counts_many.p,
+                                 width[L.i0], ds->ctr.p + 1, var[L.i0], c->stream);
+            }
+        }
+        n_launches += l1 - l0;
         RK_HIP(c, hipGetLastError());
         RK_HIP(c, hipEventRecord(c->ev1, c->stream));
         RK_HIP(c, hipEventSynchronize(c->ev1));
@@ -836,7 +900,7 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
             if (s != RK_OK) return s;
         }
         out_s += now_s() - d0;
-        i0 = i1;
+        l0 = l1;
     }
     if (t) {
         RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, kPlainCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -850,6 +914,7 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
         t->download_s = out_s;
         t->n_candidates = total;
         t->n_fp64_rechecks = fp64;
+        t->n_launches = n_launches;
         t->total_s = now_s() - t0;
     }
     return RK_OK;
@@ -908,6 +973,7 @@ rk_status count_all_impl(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_
         t->download_s = t3 - t2; // the scatter to caller order, if asked, and the copy of nu counts
         t->n_candidates = cl.size();
         t->n_fp64_rechecks = fp64;
+        t->n_launches = 1;
         t->upload_s = 0;
         t->total_s = t3 - t0;
         t->filter_s = t1 - t0;

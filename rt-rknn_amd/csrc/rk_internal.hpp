@@ -189,6 +189,8 @@ struct rk_dataset {
     rk::DevBuf<uint32_t> ctr;
     rk::DevBuf<uint32_t> counts, counts_orig;
     rk::DevBuf<unsigned char> counts_many; // rk_count_all_many: one round's count slices
+    rk::DevBuf<rk::CountSetQ> setq;        // rk_count_all_many: one round's query-set descriptors
+    rk::HostBuf<rk::CountSetQ> h_setq;
     rk::HostBuf<rk::CandRec> h_rec;
     rk::HostBuf<float2> h_orig;
     rk::HostBuf<int32_t> h_out;

@@ -524,6 +524,48 @@ void k_count_all(DsDev ds, QDev q, const CandRec* __restrict__ rec, const float2
     if (lane_id() == 0 && rech) atomicAdd(&rech_out[(blockIdx.x * kWaves + wave) % kBuckets], rech);
 }
 
+// Count-all over a set of queries in one launch.  Each wave loads its
+// group's users once and counts them against every query of the set in
+// turn; the block stages all the set's records (at most kCountSetRecs) in
+// LDS once.  A wave's start (argument and user loads, record staging) and
+// the launch's first and last rounds are then paid once per set instead of
+// once per query.  Query i's counts go to counts + qs[i].coff, its records
+// are rec[qs[i].roff ..) and orig[qs[i].roff ..).
+template <int U, typename OutT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(U == 4 ? 8 : U == 6 ? 6 : 5)))
+void k_count_set(DsDev ds, const CountSetQ* __restrict__ qs, uint32_t nq, uint32_t n_recs,
+                 const CandRec* __restrict__ rec, const float2* __restrict__ orig, OutT* __restrict__ counts,
+                 uint32_t* __restrict__ rech_out)
+{
+    extern __shared__ CandRec s_recs[];   // every query's records, query-major
+    __shared__ CandRec s_w[kWaves][128];  // per chunk: clean ones at [0, 64), the others at [64, 128)
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t n_groups = (ds.n_tiles + U - 1) / U;
+    const uint32_t stride = gridDim.x * kWaves;
+    uint32_t g = uniform(blockIdx.x * kWaves + wave);
+    float px[U], py[U];
+    float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (g < n_groups) load_group<U>(ds, g, px, py, m);
+    for (uint32_t i = threadIdx.x; i < n_recs; i += kBlock) s_recs[i] = rec[i];
+    __syncthreads();
+    uint32_t rech = 0;
+    while (g < n_groups) {
+        const uint32_t t0 = g * U, nt = min((uint32_t)U, ds.n_tiles - t0);
+        for (uint32_t i = 0; i < nq; ++i) {
+            const CountSetQ d = qs[i]; // uniform: scalar loads
+            QDev q{};
+            q.qx = d.qx;
+            q.qy = d.qy;
+            q.nc = d.nc;
+            count_group<U, true, OutT>(ds, q, d.nc, s_recs + d.roff, rec + d.roff, orig + d.roff,
+                                       counts + d.coff, s_w[wave], t0, nt, px, py, m, rech);
+        }
+        g += stride;
+        if (g < n_groups) load_group<U>(ds, g, px, py, m);
+    }
+    if (lane_id() == 0 && rech) atomicAdd(&rech_out[(blockIdx.x * kWaves + wave) % kBuckets], rech);
+}
+
 template <typename SrcT>
 __global__ void k_scatter_u32(const SrcT* __restrict__ src, const uint32_t* __restrict__ perm, uint32_t n,
                               uint32_t* __restrict__ dst)
@@ -1422,6 +1464,33 @@ void launch_count_typed(const DsDev& ds, QDev q, const CandRec* rec, const float
     } else {
         const int g = grid_for((ds.n_tiles + 3) / 4, kWaves, 1 << 20);
         k_count_all<4, 1, OutT><<<g, kBlock, lds, st>>>(ds, q, rec, orig, counts, rechecks);
+    }
+}
+
+template <typename OutT>
+void launch_count_set_typed(const DsDev& ds, const CountSetQ* qs, uint32_t nq, uint32_t n_recs, const CandRec* rec,
+                            const float2* orig, OutT* counts, uint32_t* rechecks, int variant, hipStream_t st)
+{
+    const size_t lds = (size_t)n_recs * sizeof(CandRec);
+    if (variant == kCountValu8) {
+        const int g = grid_for((ds.n_tiles + 7) / 8, kWaves, 1 << 20);
+        k_count_set<8, OutT><<<g, kBlock, lds, st>>>(ds, qs, nq, n_recs, rec, orig, counts, rechecks);
+    } else if (variant == kCountValu6) {
+        const int g = grid_for((ds.n_tiles + 5) / 6, kWaves, 1 << 20);
+        k_count_set<6, OutT><<<g, kBlock, lds, st>>>(ds, qs, nq, n_recs, rec, orig, counts, rechecks);
+    } else {
+        const int g = grid_for((ds.n_tiles + 3) / 4, kWaves, 1 << 20);
+        k_count_set<4, OutT><<<g, kBlock, lds, st>>>(ds, qs, nq, n_recs, rec, orig, counts, rechecks);
+    }
+}
+
+void launch_count_set(const DsDev& ds, const CountSetQ* qs, uint32_t nq, uint32_t n_recs, const CandRec* rec,
+                      const float2* orig, void* counts, int bytes, uint32_t* rechecks, int variant, hipStream_t st)
+{
+    switch (bytes) {
+    case 1: launch_count_set_typed(ds, qs, nq, n_recs, rec, orig, (uint8_t*)counts, rechecks, variant, st); break;
+    case 2: launch_count_set_typed(ds, qs, nq, n_recs, rec, orig, (uint16_t*)counts, rechecks, variant, st); break;
+    default: launch_count_set_typed(ds, qs, nq, n_recs, rec, orig, (uint32_t*)counts, rechecks, variant, st); break;
     }
 }
 

@@ -100,6 +100,24 @@ void launch_count_all(const DsDev& ds, QDev q, const CandRec* rec, const float2*
 void launch_scatter_counts(const void* src_sorted, int bytes, const uint32_t* perm, uint32_t n,
                            uint32_t* dst_orig, hipStream_t st);
 
+// count-all over a set of queries in one launch: every wave counts its group
+// of tiles against each query of the set.  The set's records (n_recs in all,
+// query i's at rec + qs[i].roff) are staged in LDS, so n_recs <=
+// kCountSetRecs; at most kCountSetMax queries.  Every query's counts take
+// bytes (1, 2 or 4) each: query i's n_pad counts at (char*)counts + bytes *
+// qs[i].coff.  qs is device memory.
+constexpr uint32_t kCountSetMax = 16;
+constexpr uint32_t kCountSetRecs = 1024;
+struct CountSetQ {
+    float qx, qy;
+    uint32_t nc;   // records of the query
+    uint32_t roff; // first record in the set's rec[] / orig[]
+    uint64_t coff; // first count, in counts
+    uint64_t pad;
+};
+void launch_count_set(const DsDev& ds, const CountSetQ* qs, uint32_t nq, uint32_t n_recs, const CandRec* rec,
+                      const float2* orig, void* counts, int bytes, uint32_t* rechecks, int variant, hipStream_t st);
+
 // batched form: the whole step is one launch of the traversal kernel, which
 // sorts each query's result and writes its ids and words to pinned host
 // memory; captured in a graph.  After the step, hqres says where each query's

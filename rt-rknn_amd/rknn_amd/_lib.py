@@ -35,6 +35,7 @@ class rk_timing(C.Structure):
         ("total_s", C.c_double),
         ("n_candidates", C.c_uint64),
         ("n_fp64_rechecks", C.c_uint64),
+        ("n_launches", C.c_uint64),
     ]
 
 

@@ -443,3 +443,47 @@ def test_count_all_large_vs_oracle(rk, ctx, kernel, monkeypatch):
                 assert np.array_equal(got, exp[order]), (kernel, q, len(cand))
     finally:
         ds.close()
+
+
+@pytest.mark.parametrize("kernel", ["valu4", "valu6", "valu8"])
+def test_count_all_query_sets(rk, ctx, kernel, monkeypatch):
+    """rk_count_all_many counts consecutive queries whose lists fit the LDS
+    staging together (at most 16 queries and 1,024 records) in one launch,
+    all of them at the set's widest count type; a longer list runs alone.
+    Clustered data with exact bisector ties (the banded path and its binary64
+    re-check): a full set of 16 lists, a set mixing an 8-bit (255) and a
+    16-bit (256) list, a 1,100-candidate list, an empty list, and lists that
+    overflow the staging together.  Every row equals the oracle's counts in
+    both orders at set sizes 1, 3 and 16, and the launches are the sets':
+    23, 9 and 5."""
+    monkeypatch.setenv("RK_COUNT_KERNEL", kernel)
+    seed = rk.default_seed(3)
+    nu, nf = 40_000, 2_000
+    ux, uy, uid, fx, fy, fid = rk.generate(3, nu, nf, seed=seed, nq_ties=4, seed_ties=seed)
+    ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    O.set_threads(16)
+    r = np.random.default_rng(5)
+    qs = [int(q) for q in rk.generate_queries(nf, 24, seed)]
+
+    def pick(q, n):  # n distinct candidates other than q
+        return r.choice(np.setdiff1d(np.arange(nf, dtype=np.uint32), [q]), n, replace=False).astype(np.uint32)
+
+    sizes = [50] * 16 + [255, 256, 1100, 0, 600, 500]
+    queries = [(qs[i], pick(qs[i], n)) for i, n in enumerate(sizes)]
+    queries.append((qs[22], ds.filter(qs[22], 25)))
+    assert len(queries[-1][1]) <= 424  # joins the 500-list's set
+    try:
+        want = [O.counts(ux, uy, fx, fy, q, cand=c) for q, c in queries]
+        order = ds.order()
+        for set_max, launches in ((1, 23), (3, 9), (16, 5)):
+            monkeypatch.setenv("RK_COUNT_SET", str(set_max))
+            got, t = ds.count_all_many(queries, timing=True)
+            assert t.n_launches == launches, (kernel, set_max)
+            assert t.n_candidates == sum(len(c) for _, c in queries)
+            for i in range(len(queries)):
+                assert np.array_equal(got[i], want[i]), (kernel, set_max, i)
+            got = ds.count_all_many(queries, sorted_order=True)
+            for i in range(len(queries)):
+                assert np.array_equal(got[i], want[i][order]), (kernel, set_max, i)
+    finally:
+        ds.close()
