@@ -811,17 +811,20 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
         size_t i0, i1;
         uint32_t n_recs;
     };
+    // (a set's count slices also fit one round, kCountManyBytes)
+    auto slice_bytes = [&](int w) { return ((size_t)ds->n_pad * w + 255) & ~size_t(255); };
     std::vector<Launch> launches;
     for (size_t i = 0; i < nq;) {
         Launch L{i, i + 1, qd[i].nc};
         if (qd[i].nc <= kCountSetRecs) {
+            int w = width[i];
             while (L.i1 < nq && L.i1 - L.i0 < set_max && var[L.i1] == var[L.i0] &&
-                   L.n_recs + qd[L.i1].nc <= kCountSetRecs) {
+                   L.n_recs + qd[L.i1].nc <= kCountSetRecs &&
+                   (L.i1 - L.i0 + 1) * slice_bytes(std::max(w, width[L.i1])) <= kCountManyBytes) {
+                w = std::max(w, width[L.i1]);
                 L.n_recs += qd[L.i1].nc;
                 ++L.i1;
             }
-            int w = 0;
-            for (size_t j = L.i0; j < L.i1; ++j) w = std::max(w, width[j]);
             for (size_t j = L.i0; j < L.i1; ++j) width[j] = w;
         }
         launches.push_back(L);

@@ -168,9 +168,9 @@ def test_deep_count_all(deep):
 def test_deep_count_all_many(deep):
     """Count-all over five queries in one call at 17M users: in the dataset's
     order each query's 32-bit counts take 68 MB of device memory, so the
-    call runs two rounds of back-to-back launches (three queries, then two);
-    in caller order the 8-bit counts of all five fit one round.  Every row
-    equals the oracle's counts."""
+    call runs two rounds (three queries, then two) and no launch counts more
+    queries than a round holds; in caller order the 8-bit counts of all five
+    fit one round.  Every row equals the oracle's counts."""
     arrs, ds, qs, ks, cands, exp = deep
     ux, uy, uid, fx, fy, fid = arrs
     order = ds.order()
@@ -178,6 +178,7 @@ def test_deep_count_all_many(deep):
     want = [O.counts(ux, uy, fx, fy, qs[i], cand=cands[i]) for i in sel]
     got, t = ds.count_all_many([(qs[i], cands[i]) for i in sel], timing=True, sorted_order=True)
     assert t.n_candidates == sum(len(cands[i]) for i in sel)
+    assert t.n_launches >= 2
     for r, i in enumerate(sel):
         assert np.array_equal(got[r], want[r][order]), qs[i]
     del got
