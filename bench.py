@@ -12,7 +12,7 @@ flight, each step's results taken in place (rk_batch_results) after its wait;
 `value_fetch` is the same with every step's ids copied into a caller buffer
 (rk_batch_fetch).  The JSON line also carries:
   roofline      dense counting kernel (count-all: every user x every candidate,
-                4 fp32 flops per test): the first 16 queries in one
+                4 fp32 flops per test): the first 32 queries in one
                 rk_count_all_many call, which counts up to 16 queries per
                 launch (k_count_set), HIP events on the kernel's stream around
                 the launches, divided by the launch count; beside it the same
@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--k", type=int, default=10)
     ap.add_argument("--config", type=int, default=4)
     ap.add_argument("--plain", type=int, default=1024, help="queries timed in the plain form")
-    ap.add_argument("--roofline-queries", type=int, default=16)
+    ap.add_argument("--roofline-queries", type=int, default=32)
     ap.add_argument("--cpu-queries", type=int, default=2, help="queries of the brute-force CPU context run")
     ap.add_argument("--inflight", type=int, default=3,
                     help="batches in flight in the timed steps, each on its own stream")
@@ -390,7 +390,7 @@ def main():
     counts_out = np.zeros(args.users, dtype=np.uint32)
     for name, so in (("caller_order", False), ("dataset_order", True)):
         tt, kk, dd = [], [], []
-        for i in range(min(args.roofline_queries, args.queries)):
+        for i in range(min(8, args.queries)):
             t0 = time.perf_counter()
             _, t = ds.count_all(qs[i], cands[i], timing=True, sorted_order=so, out=counts_out)
             tt.append(time.perf_counter() - t0)
@@ -409,9 +409,9 @@ def main():
     achieved, avg_s, flops_launch = variants[best]
     roofline = {"bound": "valu", "achieved": round(achieved, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": (MEASURED_TRAFFIC.get(best) if (args.users, args.facilities, args.k, args.config,
-                                                               n_roof, launches)
-                            == (10_000_000, 100_000, 10, 4, MEASURED_TRAFFIC_QUERIES, 1) else None),
+                "traffic": (MEASURED_TRAFFIC.get(best) if (args.users, args.facilities, args.k, args.config)
+                            == (10_000_000, 100_000, 10, 4) and n_roof == MEASURED_TRAFFIC_QUERIES * launches
+                            else None),
                 "traffic_source": "profiles/k_count_all_10M_100k.md (calibrated PMC, bytes per launch)",
                 "kernel": kernel_names[best],
                 "avg_launch_ms": round(avg_s * 1e3, 4), "flops_per_launch": flops_launch,
@@ -421,7 +421,7 @@ def main():
     # the same kernel over the batched workload's own k mix (k ~ U{1..50}: the
     # mixed-k step's candidate lists), flop-weighted over its queries
     os.environ["RK_COUNT_KERNEL"] = best
-    n_mix = min(2 * args.roofline_queries, args.queries)
+    n_mix = min(args.roofline_queries, args.queries)
     _, t = ds.count_all_many([(qs[i], mcands[i]) for i in range(n_mix)], timing=True)
     kmix_s, kmix_f = t.kernel_s, 4.0 * args.users * t.n_candidates
     del os.environ["RK_COUNT_KERNEL"]
