@@ -1,5 +1,7 @@
 """GPU parity at the sizes the configurations run: the rknn-inf configuration
-in full (1M users, 10k facilities, 100 queries, k=10), and a dataset deep
+in full (1M users, 10k facilities, 100 queries, k=10), the clustered
+configuration with ties at its full size (4M users, 50k facilities), and a
+dataset deep
 enough for a 4-level tile tree (more than 64^4 = 16,777,216 users), as the
 50M-user serving dataset has.  Every form -- plain, graph-captured batch,
 device-filtered batch, one call, served -- against the FP64 oracle over the
@@ -63,6 +65,49 @@ def test_config2_full_size(rk, ctx):
             b.close()
         for i, g in enumerate(ds.query_batch(list(zip(qs, ks)))):
             assert np.array_equal(g, exp[i]), qs[i]
+    finally:
+        ds.close()
+
+
+def test_config3_full_size(rk, ctx):
+    """Clustered form with near ties: 4,000,000 users and 50,000 facilities
+    (Gaussian mixture and polylines on a 2^-7 grid, users placed exactly on
+    the queries' bisectors or duplicated), 16 of its 200 queries at each
+    k in {1, 5, 10, 25}: plain, graph-captured batch (host filter's lists)
+    and device-filtered batch, against the oracle; the exact ties reach the
+    binary64 re-check."""
+    seed = rk.default_seed(3)
+    nu, nf = 4_000_000, 50_000
+    qs_all = [int(q) for q in rk.generate_queries(nf, 200, seed)]
+    arrs = rk.generate(3, nu, nf, seed=seed, nq_ties=200, seed_ties=seed)
+    qs = [q for q in qs_all[:16] for _ in range(4)]
+    ks = [1, 5, 10, 25] * 16
+    ds = rk.Dataset(ctx, *arrs)
+    try:
+        cands = [ds.filter(q, k) for q, k in zip(qs, ks)]
+        exp = oracle_sets(arrs, qs, ks, cands)
+        print("config 3 full size: |C| mean", np.mean([len(c) for c in cands]), "|R| mean",
+              np.mean([len(e) for e in exp]), flush=True)
+        rech = 0
+        for i, (q, k, c) in enumerate(zip(qs, ks, cands)):
+            got, t = ds.query_plain(q, k, c, timing=True)
+            assert np.array_equal(got, exp[i]), (q, k)
+            rech += t.n_fp64_rechecks
+        assert rech > 0  # the ties were settled in binary64
+        b = ds.prepare_batch(list(zip(qs, ks, cands)))
+        try:
+            b.run()
+            for i, g in enumerate(b.fetch()):
+                assert np.array_equal(g, exp[i]), (qs[i], ks[i])
+        finally:
+            b.close()
+        b = ds.prepare_batch(list(zip(qs, ks)), device_filter=True)
+        try:
+            b.run()
+            for i, g in enumerate(b.fetch()):
+                assert np.array_equal(g, exp[i]), (qs[i], ks[i])
+        finally:
+            b.close()
     finally:
         ds.close()
 
