@@ -1,7 +1,8 @@
 """GPU parity at the sizes the configurations run: the rknn-inf configuration
 in full (1M users, 10k facilities, 100 queries, k=10), the clustered
-configuration with ties at its full size (4M users, 50k facilities), and a
-dataset deep
+configuration with ties at its full size (4M users, 50k facilities), the
+batched configuration's dataset (10M users, 100k facilities) with its k mix,
+and a dataset deep
 enough for a 4-level tile tree (more than 64^4 = 16,777,216 users), as the
 50M-user serving dataset has.  Every form -- plain, graph-captured batch,
 device-filtered batch, one call, served -- against the FP64 oracle over the
@@ -108,6 +109,43 @@ def test_config3_full_size(rk, ctx):
                 assert np.array_equal(g, exp[i]), (qs[i], ks[i])
         finally:
             b.close()
+    finally:
+        ds.close()
+
+
+def test_config4_full_size(rk, ctx):
+    """Batched form at its full size: 10,000,000 users and 100,000
+    facilities (half uniform, half clustered with ties), 32 queries with k
+    drawn uniformly from 1..50 as the configuration's batch has them, through
+    the graph batch (host filter's lists), the device-filtered batch and the
+    server, against the oracle."""
+    seed = rk.default_seed(4)
+    nu, nf = 10_000_000, 100_000
+    arrs = rk.generate(4, nu, nf, seed=seed, nq_ties=32, seed_ties=seed)
+    qs = [int(q) for q in rk.generate_queries(nf, 32, seed)]
+    ks = [int(k) for k in np.random.default_rng(seed).integers(1, 51, size=32)]
+    ds = rk.Dataset(ctx, *arrs)
+    try:
+        cands = [ds.filter(q, k) for q, k in zip(qs, ks)]
+        exp = oracle_sets(arrs, qs, ks, cands)
+        print("config 4 full size: |C| mean", np.mean([len(c) for c in cands]), "|R| mean",
+              np.mean([len(e) for e in exp]), flush=True)
+        b = ds.prepare_batch(list(zip(qs, ks, cands)))
+        try:
+            b.run()
+            for i, g in enumerate(b.fetch()):
+                assert np.array_equal(g, exp[i]), (qs[i], ks[i])
+        finally:
+            b.close()
+        b = ds.prepare_batch(list(zip(qs, ks)), device_filter=True)
+        try:
+            b.run()
+            for i, g in enumerate(b.fetch()):
+                assert np.array_equal(g, exp[i]), (qs[i], ks[i])
+        finally:
+            b.close()
+        for i, g in enumerate(served(rk, ctx, ds, qs, ks)):
+            assert np.array_equal(g, exp[i]), (qs[i], ks[i])
     finally:
         ds.close()
 
