@@ -2,7 +2,8 @@
 in full (1M users, 10k facilities, 100 queries, k=10), the clustered
 configuration with ties at its full size (4M users, 50k facilities), the
 batched configuration's dataset (10M users, 100k facilities) with its k mix,
-and a dataset deep
+the serving configuration's three datasets (1M, 10M and 50M users) served
+together, and a dataset deep
 enough for a 4-level tile tree (more than 64^4 = 16,777,216 users), as the
 50M-user serving dataset has.  Every form -- plain, graph-captured batch,
 device-filtered batch, one call, served -- against the FP64 oracle over the
@@ -148,6 +149,63 @@ def test_config4_full_size(rk, ctx):
             assert np.array_equal(g, exp[i]), (qs[i], ks[i])
     finally:
         ds.close()
+
+
+def test_config5_full_size(rk, ctx):
+    """Serving configuration at its full size: three datasets resident
+    together (1M users / 10k facilities, 10M / 100k, 50M / 200k; the
+    config-4 generator), one server, requests of all three interleaved, k
+    drawn from 1..50; every result against the oracle over the host filter's
+    list (the server filters on the device)."""
+    sizes = [(1_000_000, 10_000), (10_000_000, 100_000), (50_000_000, 200_000)]
+    rng = np.random.default_rng(5)
+    sets = []
+    try:
+        for j, (nu, nf) in enumerate(sizes):
+            seed = rk.default_seed(4) + 1000 * (j + 1)
+            arrs = rk.generate(4, nu, nf, seed=seed)
+            ds = rk.Dataset(ctx, *arrs)
+            qs = [int(q) for q in rk.generate_queries(nf, 4, seed)]
+            ks = [int(k) for k in rng.integers(1, 51, size=4)]
+            cands = [ds.filter(q, k) for q, k in zip(qs, ks)]
+            exp = oracle_sets(arrs, qs, ks, cands)
+            del arrs
+            sets.append((ds, qs, ks, exp))
+        ds, qs, ks, exp = sets[2]  # 50M users: a 4-level tile tree
+        cands = [ds.filter(q, k) for q, k in zip(qs, ks)]
+        b = ds.prepare_batch(list(zip(qs, ks, cands)))
+        try:
+            b.run()
+            items, _, _ = b.stats()
+            assert len(items) == 4, items
+            for i, g in enumerate(b.fetch()):
+                assert np.array_equal(g, exp[i]), (qs[i], ks[i])
+        finally:
+            b.close()
+        srv = rk.Server(ctx, max_batch=64, max_wait_s=0.002)
+        try:
+            tickets = []
+            for i in range(4):  # interleaved over the datasets
+                for ds, qs, ks, exp in sets:
+                    tickets.append((srv.submit(ds, qs[i], ks[i]), ds, exp[i], qs[i]))
+            deadline = time.time() + 90
+            left = list(tickets)
+            while left and time.time() < deadline:
+                nxt = []
+                for t, ds, e, q in left:
+                    got = srv.poll(t, ds.nu)
+                    if got is None:
+                        nxt.append((t, ds, e, q))
+                    else:
+                        assert np.array_equal(got[0], e), (ds.nu, q)
+                left = nxt
+                time.sleep(0.001)
+            assert not left
+        finally:
+            srv.stop()
+    finally:
+        for ds, *_ in sets:
+            ds.close()
 
 
 @pytest.fixture(scope="module")
