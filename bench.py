@@ -357,7 +357,7 @@ def main():
     roof_queries = [(qs[i], cands[i]) for i in range(n_roof)]
     roof_out = np.zeros((n_roof, args.users), dtype=np.uint32)
     launches = 0
-    for _ in range(3):  # the variants interleaved, three rounds each
+    for _ in range(5):  # the variants interleaved, five rounds each
         for kern in kernel_names:
             os.environ["RK_COUNT_KERNEL"] = kern
             # the queries' count launches back to back on the kernel's stream,
