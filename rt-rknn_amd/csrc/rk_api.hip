@@ -797,11 +797,12 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
                                          sorted_out, &var[i]);
         if (s != RK_OK) return s;
     }
-    // the launches: consecutive queries of one count variant whose records
-    // fit the LDS staging together are counted in one launch (k_count_set:
-    // each wave's users loaded once for the whole set); a longer list, or
-    // every query when RK_COUNT_SET=1, gets a launch of its own.  The
-    // queries of a set share its widest count type.
+    // the launches: consecutive queries whose records fit the LDS staging
+    // together are counted in one launch (k_count_set: each wave's users
+    // loaded once for the whole set); a longer list, or every query when
+    // RK_COUNT_SET=1, gets a launch of its own.  The queries of a set share
+    // its widest count type and its first query's count variant (every
+    // variant gives the same counts).
     size_t set_max = kCountSetMax;
     if (const char* e = std::getenv("RK_COUNT_SET")) {
         const long v = std::strtol(e, nullptr, 10);
@@ -818,8 +819,7 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
         Launch L{i, i + 1, qd[i].nc};
         if (qd[i].nc <= kCountSetRecs) {
             int w = width[i];
-            while (L.i1 < nq && L.i1 - L.i0 < set_max && var[L.i1] == var[L.i0] &&
-                   L.n_recs + qd[L.i1].nc <= kCountSetRecs &&
+            while (L.i1 < nq && L.i1 - L.i0 < set_max && L.n_recs + qd[L.i1].nc <= kCountSetRecs &&
                    (L.i1 - L.i0 + 1) * slice_bytes(std::max(w, width[L.i1])) <= kCountManyBytes) {
                 w = std::max(w, width[L.i1]);
                 L.n_recs += qd[L.i1].nc;
