@@ -487,3 +487,45 @@ def test_count_all_query_sets(rk, ctx, kernel, monkeypatch):
                 assert np.array_equal(got[i], want[i][order]), (kernel, set_max, i)
     finally:
         ds.close()
+
+
+@pytest.mark.parametrize("scale,shift", [(1.0, -500.0), (1.0, 1.0e6), (1.0e-3, 0.0), (37.5, -2.0e4)])
+def test_coordinate_ranges(rk, ctx, scale, shift):
+    """The certified fp32 band and its binary64 re-check hold wherever the
+    coordinates lie: negative, offset by 10^6 (fp32 steps of 1/16, so many
+    exact ties), scaled to 10^-3, scaled and shifted.  The rknn-rt mesh
+    builder assumes the origin at 0 (rt.cpp:268-269); here every form --
+    plain, graph batch, device-filtered batch, count-all -- equals the oracle
+    on the transformed fp32 coordinates."""
+    seed = rk.default_seed(2)
+    ux, uy, uid, fx, fy, fid = rk.generate(4, 50_000, 2_000, seed=seed, nq_ties=8, seed_ties=seed)
+    t = lambda v: (v.astype(np.float64) * scale + shift).astype(np.float32)
+    ux, uy, fx, fy = t(ux), t(uy), t(fx), t(fy)
+    qs = [int(q) for q in rk.generate_queries(2_000, 8, seed)]
+    ks = [1, 10, 25, 50] * 2
+    ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    O.set_threads(16)
+    try:
+        cands = [ds.filter(q, k) for q, k in zip(qs, ks)]
+        exp = [O.rknn(ux, uy, uid, fx, fy, q, k, cand=c) for q, k, c in zip(qs, ks, cands)]
+        for q, k, c, e in zip(qs, ks, cands, exp):
+            assert np.array_equal(ds.query_plain(q, k, c), e), (scale, shift, q, k)
+        b = ds.prepare_batch(list(zip(qs, ks, cands)))
+        try:
+            b.run()
+            for i, g in enumerate(b.fetch()):
+                assert np.array_equal(g, exp[i]), (scale, shift, qs[i], ks[i])
+        finally:
+            b.close()
+        b = ds.prepare_batch(list(zip(qs, ks)), device_filter=True)
+        try:
+            b.run()
+            for i, g in enumerate(b.fetch()):
+                assert np.array_equal(g, exp[i]), (scale, shift, qs[i], ks[i])
+        finally:
+            b.close()
+        got = ds.count_all_many([(q, c) for q, c in zip(qs, cands)])
+        for i, (q, c) in enumerate(zip(qs, cands)):
+            assert np.array_equal(got[i], O.counts(ux, uy, fx, fy, q, cand=c)), (scale, shift, q)
+    finally:
+        ds.close()
