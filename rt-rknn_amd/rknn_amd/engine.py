@@ -174,11 +174,14 @@ class Dataset:
         return (counts, t) if timing else counts
 
     def count_all_many(self, queries, timing=False, sorted_order=False, out=None):
-        """Count-all for many queries in one call (their count kernels back to
-        back on the device).  queries: (q, cand) pairs, or (q, k, None) to
-        count over the host filter's list at k.  Returns an (nq, nu) uint32
-        array (rows in caller order, or the dataset's with sorted_order);
-        out: a C-contiguous (nq, >= nu) uint32 array to write into."""
+        """Count-all for many queries in one call: up to 16 consecutive
+        queries whose lists fit 1,024 records together share a launch, and
+        the launches run back to back on the device.  queries: (q, cand)
+        pairs, or (q, k, None) to count over the host filter's list at k.
+        Returns an (nq, nu) uint32 array (rows in caller order, or the
+        dataset's with sorted_order), with the rk_timing when timing
+        (n_launches: the launches made); out: a C-contiguous (nq, >= nu)
+        uint32 array to write into."""
         qk = [(x[0], 1, x[1]) if len(x) == 2 else tuple(x) for x in queries]
         descs, keep = self._descs(qk)
         nq = len(qk)
