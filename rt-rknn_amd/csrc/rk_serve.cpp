@@ -286,6 +286,7 @@ std::shared_ptr<rk::PinnedIds> pooled_area(const std::shared_ptr<AreaPool>& pool
 {
     rk::PinnedIds* a = nullptr;
     size_t want = n;
+    std::vector<rk::PinnedIds*> drop; // free areas below the largest size: freed outside the lock
     {
         std::lock_guard<std::mutex> lk(pool->mu);
         for (size_t i = 0; i < pool->free.size(); ++i)
@@ -304,8 +305,20 @@ std::shared_ptr<rk::PinnedIds> pooled_area(const std::shared_ptr<AreaPool>& pool
             want = p2;
         }
         if (!a) pool->largest = want;
+        // an area smaller than the largest size fits no batch that holds a
+        // larger one, yet it would count as a spare and keep the keeper idle
+        for (size_t i = 0; i < pool->free.size();) {
+            if (pool->free[i]->n < pool->largest) {
+                drop.push_back(pool->free[i]);
+                pool->free[i] = pool->free.back();
+                pool->free.pop_back();
+            } else {
+                ++i;
+            }
+        }
         if (pool->free.size() < kSpare) pool->cv.notify_one();
     }
+    for (auto* x : drop) delete x;
     if (!a) {
         a = new rk::PinnedIds();
         if (a->alloc(want) != hipSuccess) {
@@ -316,7 +329,7 @@ std::shared_ptr<rk::PinnedIds> pooled_area(const std::shared_ptr<AreaPool>& pool
     return std::shared_ptr<rk::PinnedIds>(a, [pool](rk::PinnedIds* x) {
         {
             std::lock_guard<std::mutex> lk(pool->mu);
-            if (pool->free.size() < kAreasKeep) {
+            if (pool->free.size() < kAreasKeep && x->n >= pool->largest) {
                 pool->free.push_back(x);
                 return;
             }
