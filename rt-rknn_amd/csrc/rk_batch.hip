@@ -359,7 +359,7 @@ void batch_launch(rk_batch* b, hipStream_t st)
 {
     rk_dataset* ds = b->ds;
     if (b->dfilter) {
-        launch_dfilter(ds->fgrid, dfilter_q(b), dfilter_k(b), b->nq, b->cand.p, b->cand_cap, b->dn.p, st);
+        launch_dfilter(ds->fgrid, dfilter_q(b), dfilter_k(b), b->nq, b->dlists, b->cand.p, b->cand_cap, b->dn.p, st);
         launch_dfilter_setup(ds->fgrid, dfilter_q(b), dfilter_k(b), b->nq, b->dn.p, b->cand_cap, (uint32_t)ds->nu,
                              b->q.p, b->wb.p, st);
     }
@@ -384,6 +384,7 @@ rk_status batch_fill_device(rk_batch* b, const uint32_t* q, const uint32_t* k, s
     b->total_cand = 0;
     b->hq.assign(nq, QDev{});
     for (size_t i = 0; i < nq; ++i) b->hq[i].k = k[i]; // the rest comes from each run
+    b->dlists = dfilter_lists(k, nq);
     b->h_dn.assign(nq, 0);
 
     // candidate segment per query: every facility at most, 4096 slots at most
@@ -607,7 +608,7 @@ rk_status rk_batch_prepare_device(rk_dataset* ds, const uint32_t* q, const uint3
         rk_ctx* c = ds->ctx;
         std::lock_guard<std::recursive_mutex> lk(c->mu);
         // one filter pass first: the autotuner times the traversal on these lists
-        launch_dfilter(ds->fgrid, b->dq.p, b->dk.p, b->nq, b->cand.p, b->cand_cap, b->dn.p, c->stream);
+        launch_dfilter(ds->fgrid, b->dq.p, b->dk.p, b->nq, b->dlists, b->cand.p, b->cand_cap, b->dn.p, c->stream);
         launch_dfilter_setup(ds->fgrid, b->dq.p, b->dk.p, b->nq, b->dn.p, b->cand_cap, (uint32_t)ds->nu, b->q.p,
                              b->wb.p, c->stream);
         RK_HIP(c, hipGetLastError());

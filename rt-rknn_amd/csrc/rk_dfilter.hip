@@ -385,6 +385,14 @@ __device__ uint32_t filter_query(const FacGridDev& g, DfShared& sh, uint32_t q, 
     return n;
 }
 
+// One wave per query.  K = 16, 32 or 64: the kernel runs the batch's
+// queries whose k takes lists of K entries and skips the others, with the
+// registers of its own list size (118, 134 and 166 VGPRs), so a batch of
+// small k leaves room for the traversal kernels of the steps beside it.
+// K = 0: every query, with the list size its k takes (the registers of the
+// largest), for batches that mix list sizes -- one launch per size would
+// run the sizes' slowest queries one after another.
+template <int K>
 __global__ __launch_bounds__(64) void k_dfilter(FacGridDev g, const uint32_t* __restrict__ qs,
                                                 const uint32_t* __restrict__ ks, uint32_t nq,
                                                 uint32_t* __restrict__ cand, uint32_t cap,
@@ -400,13 +408,14 @@ __global__ __launch_bounds__(64) void k_dfilter(FacGridDev g, const uint32_t* __
     }
     __syncthreads();
     for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
-        const uint32_t q = qs[qi];
         const uint32_t k = min(ks[qi], kDfMaxK);
+        if (K != 0 && dfilter_list(k) != (uint32_t)K) continue; // another kernel's query (uniform)
+        const uint32_t q = qs[qi];
         uint32_t* out = cand + (uint64_t)qi * cap;
         uint32_t n;
-        if (k <= 16)
+        if (K == 16 || (K == 0 && k <= 16))
             n = filter_query<16>(g, sh, q, k, out, cap);
-        else if (k <= 32)
+        else if (K == 32 || (K == 0 && k <= 32))
             n = filter_query<32>(g, sh, q, k, out, cap);
         else
             n = filter_query<64>(g, sh, q, k, out, cap);
@@ -519,11 +528,29 @@ rk_status build_facility_grid(rk_dataset* ds)
     return RK_OK;
 }
 
-void launch_dfilter(const FacGridDev& g, const uint32_t* q, const uint32_t* k, uint32_t nq, uint32_t* cand,
-                    uint32_t cap, uint32_t* ncand, hipStream_t st)
+uint32_t dfilter_lists(const uint32_t* k, size_t n)
+{
+    uint32_t m = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t l = dfilter_list(std::min(k[i], kDfMaxK));
+        m |= l == 16 ? 1u : l == 32 ? 2u : 4u;
+    }
+    return m;
+}
+
+void launch_dfilter(const FacGridDev& g, const uint32_t* q, const uint32_t* k, uint32_t nq, uint32_t lists,
+                    uint32_t* cand, uint32_t cap, uint32_t* ncand, hipStream_t st)
 {
     if (!nq) return;
-    k_dfilter<<<nq < 65535 ? nq : 65535, 64, 0, st>>>(g, q, k, nq, cand, cap, ncand);
+    const uint32_t grid = nq < 65535 ? nq : 65535;
+    if (lists == 1u)
+        k_dfilter<16><<<grid, 64, 0, st>>>(g, q, k, nq, cand, cap, ncand);
+    else if (lists == 2u)
+        k_dfilter<32><<<grid, 64, 0, st>>>(g, q, k, nq, cand, cap, ncand);
+    else if (lists == 4u)
+        k_dfilter<64><<<grid, 64, 0, st>>>(g, q, k, nq, cand, cap, ncand);
+    else // mixed list sizes (or unknown): one launch
+        k_dfilter<0><<<grid, 64, 0, st>>>(g, q, k, nq, cand, cap, ncand);
 }
 
 void launch_dfilter_setup(const FacGridDev& g, const uint32_t* q, const uint32_t* k, uint32_t nq,
@@ -562,7 +589,7 @@ extern "C" rk_status rk_filter_device(rk_dataset* ds, const uint32_t* q, const u
     RK_HIP(c, dc.reserve((uint64_t)nq * capq));
     RK_HIP(c, hipMemcpyAsync(dq.p, q, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
     RK_HIP(c, hipMemcpyAsync(dk.p, k, nq * sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    launch_dfilter(ds->fgrid, dq.p, dk.p, (uint32_t)nq, dc.p, capq, dn.p, st);
+    launch_dfilter(ds->fgrid, dq.p, dk.p, (uint32_t)nq, dfilter_lists(k, nq), dc.p, capq, dn.p, st);
     RK_HIP(c, hipGetLastError());
     std::vector<uint32_t> cnt(nq);
     RK_HIP(c, hipMemcpyAsync(cnt.data(), dn.p, nq * sizeof(uint32_t), hipMemcpyDeviceToHost, st));

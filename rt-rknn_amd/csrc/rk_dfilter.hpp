@@ -30,8 +30,14 @@ struct FacGridDev {
 
 // One wave per query: candidate facility indices of query i at
 // cand[i * cap ..], their number in ncand[i] (may exceed cap: overflow).
-void launch_dfilter(const FacGridDev& g, const uint32_t* q, const uint32_t* k, uint32_t nq, uint32_t* cand,
-                    uint32_t cap, uint32_t* ncand, hipStream_t st);
+// Entries of a query's sector lists: 16, 32 or 64, the fewest that hold k.
+RK_HD inline uint32_t dfilter_list(uint32_t k) { return k <= 16 ? 16u : k <= 32 ? 32u : 64u; }
+// Bit 0, 1, 2: some k of k[0 .. n) takes lists of 16, 32, 64 entries.
+uint32_t dfilter_lists(const uint32_t* k, size_t n);
+// lists: dfilter_lists of the queries' k (a kernel of that list size when
+// there is one, else one for every size)
+void launch_dfilter(const FacGridDev& g, const uint32_t* q, const uint32_t* k, uint32_t nq, uint32_t lists,
+                    uint32_t* cand, uint32_t cap, uint32_t* ncand, hipStream_t st);
 // QDev and record-scaling bounds of a device-filtered batch from the counts
 void launch_dfilter_setup(const FacGridDev& g, const uint32_t* q, const uint32_t* k, uint32_t nq,
                           const uint32_t* ncand, uint32_t cap, uint32_t nu, QDev* qd, double2* wb, hipStream_t st);

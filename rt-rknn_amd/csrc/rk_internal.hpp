@@ -260,6 +260,7 @@ struct rk_batch {
     double filter_s = 0.0;
     // device-filtered batch: (q, k) on the device, the filter inside the step
     bool dfilter = false;
+    uint32_t dlists = 0;  // device filter: list sizes of the queries' k (dfilter_lists)
     uint32_t cand_cap = 0;       // candidate slots per query (the filter's segment)
     bool cand_overflow = false;  // the last run had a candidate list beyond cand_cap
     rk::DevBuf<uint32_t> dq, dk, dn;
