@@ -19,7 +19,8 @@ flight, each step's results taken in place (rk_batch_results) after its wait;
                 queries at one launch each (k_count_all) and, as kmix_tflops,
                 the k ~ U{1..50} lists
   count_all_call_ms  the whole count-all call, counts in caller order and in
-                the dataset's order, with the copy of 10M counts to the host
+                the dataset's order, with the copy of 10M counts to the host,
+                and the same calls into device memory (no PCIe copy)
   cpu_baseline  the reference's tree-culled verification (rknn-inf) restated
                 in C, exact in binary64, on the step's queries and candidate
                 lists, one host thread (cpu_baseline_mt: every host core;
@@ -401,6 +402,32 @@ def main():
                          "order_and_copy": round(float(np.median(dd)) * 1e3, 3)}
         log(f"count-all call, {name}: {call_ms[name]['call']} ms (kernel {call_ms[name]['kernel']} ms, "
             f"order + copy of {args.users} counts {call_ms[name]['order_and_copy']} ms)")
+    # the same calls into device memory (the caller keeps the counts on the
+    # device, as rknn-rt's result buffer is): no PCIe copy; library clock
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    dptr = ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(dptr), ctypes.c_size_t(4 * args.users)) == 0:
+        lib = R.load()
+        out_p = ctypes.cast(dptr, ctypes.POINTER(ctypes.c_uint32))
+        for name, fn in (("caller_order_device", lib.rk_count_all), ("dataset_order_device", lib.rk_count_all_sorted)):
+            tt, kk = [], []
+            for i in range(min(8, args.queries)):
+                c = np.ascontiguousarray(cands[i], dtype=np.uint32)
+                t = R._lib.rk_timing()
+                st = fn(ds.h, qs[i], c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(c), out_p,
+                        ctypes.byref(t))
+                if st != 0:
+                    log(f"ERROR: count-all into device memory failed ({st})")
+                    sys.exit(1)
+                tt.append(t.total_s)
+                kk.append(t.kernel_s)
+            call_ms[name] = {"call": round(float(np.median(tt)) * 1e3, 3),
+                             "kernel": round(float(np.median(kk)) * 1e3, 4)}
+            log(f"count-all call, {name}: {call_ms[name]['call']} ms on the library's clock "
+                f"(kernel {call_ms[name]['kernel']} ms)")
+        hip.hipFree(dptr)
     for kern, kt in kts.items():
         variants[kern] = (flops[kern] / sum(kt) / 1e12, float(np.mean(kt)), flops[kern] / len(kt))
         log(f"count-all kernel ({kern}): {variants[kern][0]:.2f} TFLOP/s, {n_roof} queries in {launches} launches "

@@ -150,7 +150,11 @@ rk_status rk_query(rk_dataset* ds, uint32_t q, uint32_t k,
 
 /* ---- count-all: per-user count of candidates strictly closer than q, no
  *      early exit (the rknn-rt any-hit counter rt.cu:95-111 with k = inf).
- *      counts[i] belongs to the i-th user of rk_dataset_create's arrays. ---- */
+ *      counts[i] belongs to the i-th user of rk_dataset_create's arrays.
+ *      counts may be host memory or device memory of the context's device
+ *      (as rknn-rt's Params::rslt, rt.h:40-49): device counts never cross
+ *      PCIe, which bounds the call otherwise (4 bytes per user).  The same
+ *      holds for rk_count_all_sorted and rk_count_all_many. ---- */
 rk_status rk_count_all(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_t nc,
                        uint32_t* counts, rk_timing* t);
 

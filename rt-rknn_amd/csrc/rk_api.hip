@@ -592,12 +592,22 @@ rk_status rk_query_plain_noalloc(rk_dataset* ds, uint32_t q, uint32_t k, const u
 
 namespace {
 
-// Copies bytes from device memory to host memory at dst, in stream order on
-// the context's stream, and waits for them.  A pinned destination
-// (hipHostMalloc) takes one DMA at the PCIe rate; pageable memory, which a
-// plain hipMemcpy fills at about a third of it, goes through two pinned
-// chunks: the DMA of one chunk runs while the context's host threads copy
-// the other into place.
+// dst is device (or managed) memory: results written there stay off PCIe.
+bool device_dst(const void* dst)
+{
+    hipPointerAttribute_t a{};
+    const hipError_t pe = hipPointerGetAttributes(&a, dst);
+    (void)hipGetLastError(); // a pageable pointer may leave an error behind
+    return pe == hipSuccess &&
+           (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.type == hipMemoryTypeUnified);
+}
+
+// Copies bytes from device memory to dst, in stream order on the context's
+// stream, and waits for them.  Device memory at dst takes a device copy; a
+// pinned destination (hipHostMalloc) one DMA at the PCIe rate; pageable
+// memory, which a plain hipMemcpy fills at about a third of it, goes
+// through two pinned chunks: the DMA of one chunk runs while the context's
+// host threads copy the other into place.
 rk_status download(rk_ctx* c, void* dst, const void* src, size_t bytes)
 {
     constexpr size_t kChunk = size_t(8) << 20;
@@ -605,6 +615,12 @@ rk_status download(rk_ctx* c, void* dst, const void* src, size_t bytes)
     const hipError_t pe = hipPointerGetAttributes(&a, dst);
     (void)hipGetLastError(); // a pageable pointer may leave an error behind
     const bool pinned = pe == hipSuccess && a.type == hipMemoryTypeHost;
+    if (pe == hipSuccess && (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged ||
+                             a.type == hipMemoryTypeUnified)) {
+        RK_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, c->stream));
+        RK_HIP(c, hipStreamSynchronize(c->stream));
+        return RK_OK;
+    }
     if (pinned || bytes < 2 * kChunk) {
         RK_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
         RK_HIP(c, hipStreamSynchronize(c->stream));
@@ -832,6 +848,7 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
     }
     RK_HIP(c, ds->setq.reserve(nq));
     RK_HIP(c, ds->h_setq.reserve(nq));
+    const bool to_device = !sorted_out && device_dst(counts);
     const double t1 = now_s();
     double kernel_s = 0, out_s = 0;
     uint64_t n_launches = 0;
@@ -895,6 +912,10 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
             rk_status s;
             if (sorted_out) {
                 s = download(c, counts + i * stride, slice, ds->nu * sizeof(uint32_t));
+            } else if (to_device) { // the scatter writes the caller's device array
+                launch_scatter_counts(slice, width[i], ds->perm.p, (uint32_t)ds->nu, counts + i * stride, c->stream);
+                RK_HIP(c, hipGetLastError());
+                s = RK_OK;
             } else {
                 launch_scatter_counts(slice, width[i], ds->perm.p, (uint32_t)ds->nu, ds->counts_orig.p, c->stream);
                 RK_HIP(c, hipGetLastError());
@@ -902,6 +923,7 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
             }
             if (s != RK_OK) return s;
         }
+        if (to_device) RK_HIP(c, hipStreamSynchronize(c->stream));
         out_s += now_s() - d0;
         l0 = l1;
     }
@@ -957,15 +979,21 @@ rk_status count_all_impl(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_
     RK_HIP(c, hipEventRecord(c->ev1, c->stream));
     if (t) RK_HIP(c, hipEventSynchronize(c->ev1)); // download_s starts when the counts are ready
     const double t2 = now_s();
+    // caller order into device memory: the scatter writes the caller's array
+    const bool to_device = !sorted_out && device_dst(counts);
     if (!sorted_out) {
-        launch_scatter_counts(ds->counts.p, count_bytes(qd.nc), ds->perm.p, (uint32_t)ds->nu, ds->counts_orig.p,
-                              c->stream);
+        launch_scatter_counts(ds->counts.p, count_bytes(qd.nc), ds->perm.p, (uint32_t)ds->nu,
+                              to_device ? counts : ds->counts_orig.p, c->stream);
         RK_HIP(c, hipGetLastError());
     }
     RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, kPlainCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
                              c->stream));
-    s = download(c, counts, sorted_out ? ds->counts.p : ds->counts_orig.p, ds->nu * sizeof(uint32_t));
-    if (s != RK_OK) return s;
+    if (to_device) {
+        RK_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+        s = download(c, counts, sorted_out ? ds->counts.p : ds->counts_orig.p, ds->nu * sizeof(uint32_t));
+        if (s != RK_OK) return s;
+    }
     float ms = 0.f;
     RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     if (t) {

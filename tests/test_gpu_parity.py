@@ -529,3 +529,59 @@ def test_coordinate_ranges(rk, ctx, scale, shift):
             assert np.array_equal(got[i], O.counts(ux, uy, fx, fy, q, cand=c)), (scale, shift, q)
     finally:
         ds.close()
+
+
+def test_count_all_into_device_memory(rk, ctx):
+    """rk_count_all, rk_count_all_sorted and rk_count_all_many write device
+    destinations (hipMalloc'd memory of the context's device, through the
+    HIP runtime the library uses) without a host copy: the same counts as
+    the oracle's, in both orders, and nothing past each row's nu counts."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+
+    def dev_alloc(n):
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), C.c_size_t(4 * n)) == 0
+        assert hip.hipMemset(p, 0x7F, C.c_size_t(4 * n)) == 0
+        return p
+
+    def dev_read(p, n):
+        out = np.zeros(n, dtype=np.uint32)
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), p, C.c_size_t(4 * n), 2) == 0  # device to host
+        return out
+
+    seed = rk.default_seed(3)
+    ux, uy, uid, fx, fy, fid = rk.generate(3, 30_000, 1_500, seed=seed, nq_ties=4, seed_ties=seed)
+    ds = rk.Dataset(ctx, ux, uy, uid, fx, fy, fid)
+    O.set_threads(16)
+    bufs = []
+    try:
+        qs = [int(q) for q in rk.generate_queries(1_500, 4, seed)]
+        cands = [ds.filter(q, 10) for q in qs]
+        order = ds.order()
+        lib, h = ds._lib, ds.h
+        nu = ds.nu
+        wants = [O.counts(ux, uy, fx, fy, q, cand=c) for q, c in zip(qs, cands)]
+        for q, c, want in zip(qs, cands, wants):
+            cp = c.ctypes.data_as(C.POINTER(C.c_uint32))
+            for fn, exp in ((lib.rk_count_all, want), (lib.rk_count_all_sorted, want[order])):
+                d = dev_alloc(nu)
+                bufs.append(d)
+                assert fn(h, q, cp, len(c), C.cast(d, C.POINTER(C.c_uint32)), None) == 0
+                assert np.array_equal(dev_read(d, nu), exp), (fn, q)
+        descs, keep = ds._descs([(q, 1, c) for q, c in zip(qs, cands)])
+        for sorted_order in (0, 1):
+            stride = nu + 5
+            d = dev_alloc(len(qs) * stride)
+            bufs.append(d)
+            assert lib.rk_count_all_many(h, descs, len(qs), C.cast(d, C.POINTER(C.c_uint32)), stride, sorted_order,
+                                         None) == 0
+            got = dev_read(d, len(qs) * stride).reshape(len(qs), stride)
+            for i, want in enumerate(wants):
+                assert np.array_equal(got[i, :nu], want[order] if sorted_order else want), (sorted_order, i)
+                assert (got[i, nu:] == 0x7F7F7F7F).all()
+    finally:
+        for d in bufs:
+            hip.hipFree(d)
+        ds.close()
