@@ -592,14 +592,15 @@ rk_status rk_query_plain_noalloc(rk_dataset* ds, uint32_t q, uint32_t k, const u
 
 namespace {
 
-// dst is device (or managed) memory: results written there stay off PCIe.
-bool device_dst(const void* dst)
+// dst is device memory of the given device: kernels running there may write
+// it (results stay off PCIe).  Device memory of another device is reached by
+// a copy instead (download).
+bool device_dst(const void* dst, int device)
 {
     hipPointerAttribute_t a{};
     const hipError_t pe = hipPointerGetAttributes(&a, dst);
     (void)hipGetLastError(); // a pageable pointer may leave an error behind
-    return pe == hipSuccess &&
-           (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged || a.type == hipMemoryTypeUnified);
+    return pe == hipSuccess && a.type == hipMemoryTypeDevice && a.device == device;
 }
 
 // Copies bytes from device memory to dst, in stream order on the context's
@@ -848,7 +849,7 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
     }
     RK_HIP(c, ds->setq.reserve(nq));
     RK_HIP(c, ds->h_setq.reserve(nq));
-    const bool to_device = !sorted_out && device_dst(counts);
+    const bool to_device = !sorted_out && device_dst(counts, c->device);
     const double t1 = now_s();
     double kernel_s = 0, out_s = 0;
     uint64_t n_launches = 0;
@@ -980,7 +981,7 @@ rk_status count_all_impl(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_
     if (t) RK_HIP(c, hipEventSynchronize(c->ev1)); // download_s starts when the counts are ready
     const double t2 = now_s();
     // caller order into device memory: the scatter writes the caller's array
-    const bool to_device = !sorted_out && device_dst(counts);
+    const bool to_device = !sorted_out && device_dst(counts, c->device);
     if (!sorted_out) {
         launch_scatter_counts(ds->counts.p, count_bytes(qd.nc), ds->perm.p, (uint32_t)ds->nu,
                               to_device ? counts : ds->counts_orig.p, c->stream);
