@@ -430,6 +430,27 @@ def main():
             log(f"count-all call, {name}: {call_ms[name]['call']} ms on the library's clock "
                 f"(kernel {call_ms[name]['kernel']} ms)")
         hip.hipFree(dptr)
+    # the roofline's queries in one rk_count_all_many call, in the dataset's
+    # order into device memory (the count kernels write the caller's rows)
+    mptr = ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(mptr), ctypes.c_size_t(4 * args.users * n_roof)) == 0:
+        lib = R.load()
+        descs, keep = ds._descs([(q, 1, c) for q, c in roof_queries])
+        tt, kk = [], []
+        for _ in range(3):
+            t = R._lib.rk_timing()
+            st = lib.rk_count_all_many(ds.h, descs, n_roof, ctypes.cast(mptr, ctypes.POINTER(ctypes.c_uint32)),
+                                       args.users, 1, ctypes.byref(t))
+            if st != 0:
+                log(f"ERROR: count-all over many queries into device memory failed ({st})")
+                sys.exit(1)
+            tt.append(t.total_s)
+            kk.append(t.kernel_s)
+        call_ms["many_dataset_order_device"] = {"queries": n_roof, "call": round(float(np.median(tt)) * 1e3, 3),
+                                                "kernel": round(float(np.median(kk)) * 1e3, 4)}
+        log(f"count-all over {n_roof} queries into device memory: {call_ms['many_dataset_order_device']['call']} ms "
+            f"on the library's clock (kernels {call_ms['many_dataset_order_device']['kernel']} ms)")
+        hip.hipFree(mptr)
     for kern, kt in kts.items():
         variants[kern] = (flops[kern] / sum(kt) / 1e12, float(np.mean(kt)), flops[kern] / len(kt))
         log(f"count-all kernel ({kern}): {variants[kern][0]:.2f} TFLOP/s, {n_roof} queries in {launches} launches "

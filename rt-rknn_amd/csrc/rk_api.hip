@@ -849,7 +849,10 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
     }
     RK_HIP(c, ds->setq.reserve(nq));
     RK_HIP(c, ds->h_setq.reserve(nq));
+    // counts into the context device's memory: in caller order the scatter
+    // writes the caller's array; in the dataset's order the count kernels do
     const bool to_device = !sorted_out && device_dst(counts, c->device);
+    const bool direct = sorted_out && device_dst(counts, c->device);
     const double t1 = now_s();
     double kernel_s = 0, out_s = 0;
     uint64_t n_launches = 0;
@@ -870,7 +873,12 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
             ++l1;
         }
         const size_t i1 = launches[l1 - 1].i1;
-        RK_HIP(c, ds->counts_many.reserve(used));
+        if (!direct) RK_HIP(c, ds->counts_many.reserve(used));
+        // a query's counts: its slice of the round, or its row of the caller's array
+        unsigned char* const obase = direct ? reinterpret_cast<unsigned char*>(counts) : ds->counts_many.p;
+        auto out_off = [&](size_t i) -> uint64_t { // in counts of width[i]
+            return direct ? (uint64_t)i * stride : soff[i - i0] / width[i];
+        };
         // the sets' descriptors, one upload for the round
         for (size_t i = i0; i < i1; ++i) {
             CountSetQ& d = ds->h_setq.p[i - i0];
@@ -878,7 +886,7 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
             d.qx = qd[i].qx;
             d.qy = qd[i].qy;
             d.nc = qd[i].nc;
-            d.coff = soff[i - i0] / width[i];
+            d.coff = out_off(i);
         }
         for (size_t l = l0; l < l1; ++l)
             for (size_t i = launches[l].i0; i < launches[l].i1; ++i)
@@ -891,13 +899,13 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
             const Launch& L = launches[l];
             if (L.i1 - L.i0 == 1) {
                 launch_count_all(ds->view, qd[L.i0], ds->rec.p + roff[L.i0], ds->orig.p + roff[L.i0],
-                                 ds->counts_many.p + soff[L.i0 - i0], ds->ctr.p + 1, var[L.i0], c->stream,
+                                 obase + out_off(L.i0) * width[L.i0], ds->ctr.p + 1, var[L.i0], c->stream,
                                  sorted_out);
             } else {
-                // every slice of the round is addressed from the round's base
+                // every query's counts are addressed from the round's base
                 launch_count_set(ds->view, ds->setq.p + (L.i0 - i0), (uint32_t)(L.i1 - L.i0), L.n_recs,
-                                 ds->rec.p + roff[L.i0], ds->orig.p + roff[L.i0], ds->counts_many.p,
-                                 width[L.i0], ds->ctr.p + 1, var[L.i0], c->stream);
+                                 ds->rec.p + roff[L.i0], ds->orig.p + roff[L.i0], obase, width[L.i0],
+                                 ds->ctr.p + 1, var[L.i0], c->stream);
             }
         }
         n_launches += l1 - l0;
@@ -908,7 +916,7 @@ rk_status count_many_impl(rk_dataset* ds, const rk_query_desc* qs, size_t nq, ui
         RK_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
         kernel_s += ms * 1e-3;
         const double d0 = now_s();
-        for (size_t i = i0; i < i1; ++i) {
+        for (size_t i = i0; i < i1 && !direct; ++i) {
             const void* slice = ds->counts_many.p + soff[i - i0];
             rk_status s;
             if (sorted_out) {
@@ -974,8 +982,12 @@ rk_status count_all_impl(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_
     int variant = 0;
     s = pick_count_variant(ds, qd, ds->rec.p, ds->orig.p, ds->counts.p, sorted_out, &variant);
     if (s != RK_OK) return s;
+    // the dataset's order into the context device's memory: the kernel writes
+    // the caller's array
+    const bool direct = sorted_out && device_dst(counts, c->device);
     RK_HIP(c, hipEventRecord(c->ev0, c->stream));
-    launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, ds->counts.p, ds->ctr.p + 1, variant, c->stream, sorted_out);
+    launch_count_all(ds->view, qd, ds->rec.p, ds->orig.p, direct ? (void*)counts : (void*)ds->counts.p, ds->ctr.p + 1,
+                     variant, c->stream, sorted_out);
     RK_HIP(c, hipGetLastError());
     RK_HIP(c, hipEventRecord(c->ev1, c->stream));
     if (t) RK_HIP(c, hipEventSynchronize(c->ev1)); // download_s starts when the counts are ready
@@ -989,7 +1001,7 @@ rk_status count_all_impl(rk_dataset* ds, uint32_t q, const uint32_t* cand, size_
     }
     RK_HIP(c, hipMemcpyAsync(ds->h_ctr.p, ds->ctr.p, kPlainCounters * sizeof(uint32_t), hipMemcpyDeviceToHost,
                              c->stream));
-    if (to_device) {
+    if (to_device || direct) {
         RK_HIP(c, hipStreamSynchronize(c->stream));
     } else {
         s = download(c, counts, sorted_out ? ds->counts.p : ds->counts_orig.p, ds->nu * sizeof(uint32_t));

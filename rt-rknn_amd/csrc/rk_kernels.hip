@@ -477,10 +477,13 @@ __device__ __forceinline__ void count_group(const DsDev& ds, const QDev& q, uint
         if (n_slow) count_slow<U>(wl + 64, orig, n_slow, ds, t0, nt, wx, wy, q.qx, q.qy, acc, rech);
         __builtin_amdgcn_wave_barrier();
     }
+    // the padding users of the last tile are not stored (the output may be
+    // the caller's array of exactly nu counts)
     OutT* __restrict__ out = counts + ((uint64_t)t0 * kTile + lane);
+    const uint32_t u0 = t0 * kTile + lane;
 #pragma unroll
     for (int j = 0; j < U; ++j)
-        if ((uint32_t)j < nt) out[j * kTile] = (OutT)(uint32_t)acc[j];
+        if ((uint32_t)j < nt && u0 + (uint32_t)j * kTile < ds.nu) out[j * kTile] = (OutT)(uint32_t)acc[j];
 }
 
 // The first group's users are requested before the records are staged, so
